@@ -1,0 +1,118 @@
+/*
+ * oxiblas_mi355x.h -- C ABI of liboxiblas_mi355x.so, the MI355X (gfx950)
+ * implementation of OxiBLAS.jl's level-3 hot path: gemm!, the blocked LU with
+ * partial pivoting (lu / lu! / lu_factor) and its solves (ldiv / F \ b,
+ * transposed solves), the mixed-precision LU with iterative refinement and
+ * the pieces the adjoints of x = A\B and gemm! need.
+ *
+ * Conventions (those of the reference's Julia API):
+ *   - matrices are column-major FP64 (FP32 for the low-precision factors) in
+ *     device memory, with a leading dimension >= max(1, rows);
+ *   - pivots are int64, 1-based global row indices in LAPACK ipiv form, as
+ *     `LUFactor.pivots` (src/lapack/factorizations.jl:5-10,118);
+ *   - trans characters are 'N', 'T', 'C' ('C' equals 'T' for real data),
+ *     as `_trans_kind` (src/blas/gemm.jl:156-161);
+ *   - return value: OXI_OK (0); -i when argument i is invalid (nothing is
+ *     written, as the reference validates before any mutation,
+ *     src/blas/gemm.jl:673-677); a positive OXI_ERR_* code for a runtime
+ *     failure.  A singular pivot is not an error of the call: it is reported
+ *     through `info` (> 0, the 1-based column), as `LUFactor.info`
+ *     (src/lapack/factorizations.jl:125-129); the binding turns it into
+ *     SingularException(info) when check=true (src/api/linalg_api.jl:163-167).
+ *   - every call is enqueued on the handle's stream; calls that return host
+ *     values (info, iteration counts, errors) synchronise that stream.
+ *   - a handle serialises its callers (the GemmWorkspace lock contract,
+ *     src/blas/gemm.jl:37-44); separate handles run independently.
+ */
+#ifndef OXIBLAS_MI355X_H
+#define OXIBLAS_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oxi_handle_s* oxi_handle_t;
+
+enum {
+    OXI_OK = 0,
+    OXI_ERR_HIP = 1,          /* a HIP runtime call failed */
+    OXI_ERR_ALLOC = 2,        /* device allocation failed */
+    OXI_ERR_UNSUPPORTED = 3,  /* a size beyond what the kernels support */
+    OXI_ERR_SINGULAR = 4,     /* zero on a non-unit triangular diagonal (trsm/trsv/getrs) */
+    OXI_ERR_TIMEOUT = 5,      /* a device-side barrier gave up (never expected) */
+    OXI_ERR_NOT_FINITE = 6,   /* non-finite input where finite input is required */
+    OXI_ERR_OVERFLOW = 7      /* low-precision conversion overflowed */
+};
+
+/* ---- handle, stream and memory ----------------------------------------- */
+/* Replaces the per-type GemmWorkspace pool (src/blas/gemm.jl:107-153): the
+ * handle owns the device workspace and the streams of one device. */
+int oxi_create(int device, oxi_handle_t* handle);
+int oxi_destroy(oxi_handle_t handle);
+/* Use `stream` (a hipStream_t; NULL = default stream) for every call. */
+int oxi_set_stream(oxi_handle_t handle, void* stream);
+int oxi_sync(oxi_handle_t handle);
+/* Text of the last error recorded on this handle. */
+const char* oxi_last_error(oxi_handle_t handle);
+const char* oxi_version(void);
+int oxi_malloc(oxi_handle_t handle, void** ptr, uint64_t bytes);
+int oxi_free(oxi_handle_t handle, void* ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device; synchronous. */
+int oxi_memcpy(oxi_handle_t handle, void* dst, const void* src, uint64_t bytes, int kind);
+
+/* ---- level 3 ----------------------------------------------------------- */
+/* gemm!(C, A, B, alpha, beta; transA, transB)  (src/blas/gemm.jl:625-703):
+ * C <- alpha*op(A)*op(B) + beta*C.  m==0||n==0 returns at once; alpha==0 or
+ * k==0 scales C only and never reads A or B; beta==0 never reads C (NaN in C
+ * is overwritten); an A or B overlapping C is snapshotted before C is written
+ * (src/blas/gemm.jl:682-688).  FP64 MFMA (v_mfma_f64_16x16x4). */
+int oxi_dgemm(oxi_handle_t handle, char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
+              const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc);
+
+/* trsm!(B, A, alpha; side, uplo, trans, unit)  (src/blas/level3.jl:518-544):
+ * B <- alpha*op(A)^-1*B (side 'L') or alpha*B*op(A)^-1 (side 'R').  A zero on a
+ * non-unit diagonal returns OXI_ERR_SINGULAR before B is touched
+ * (src/blas/level3.jl:476-483); *singular_index (may be NULL) gets its
+ * 1-based position. */
+int oxi_dtrsm(oxi_handle_t handle, char side, char uplo, char trans, char diag, int64_t m, int64_t n, double alpha,
+              const double* A, int64_t lda, double* B, int64_t ldb, int64_t* singular_index);
+
+/* _lu_swap! / laswp (src/lapack/factorizations.jl:80-88) and _rowswap!
+ * (src/api/linalg_api.jl:100-110): interchanges k1..k2 (1-based, inclusive)
+ * of ipiv on columns 0..ncols-1 of A, in reverse order when inverse != 0. */
+int oxi_dlaswp(oxi_handle_t handle, int64_t ncols, double* A, int64_t lda, int64_t k1, int64_t k2,
+               const int64_t* ipiv, int inverse);
+
+/* ---- LU with partial pivoting ------------------------------------------- */
+/* lu!(A) / _lu_factor!(F; blocksize, check)  (src/lapack/factorizations.jl:30-77,
+ * src/api/linalg_api.jl:218): A (m x n, device) is overwritten by the unit
+ * lower and upper factors; ipiv (device, min(m,n) entries) receives the
+ * 1-based global pivot rows; *info (host) receives 0 or the first zero pivot
+ * column (1-based).  nb is the panel width (the reference's blocksize; 0 picks
+ * the tuned default).  Pivots are the first row of largest magnitude, as
+ * _lu_pivot (src/lapack/factorizations.jl:89-107). */
+int oxi_dgetrf(oxi_handle_t handle, int64_t m, int64_t n, double* A, int64_t lda, int64_t* ipiv, int64_t nb,
+               int64_t* info);
+
+/* ldiv!(F, B) and ldiv!(transpose(F), B)  (src/api/linalg_api.jl:168-184):
+ * solves op(A)*X = B in place with the factors of oxi_dgetrf.  trans 'N':
+ * row interchanges, unit-lower then upper solve; 'T'/'C': upper-transposed,
+ * unit-lower-transposed solve, then the inverse interchanges (the transposed
+ * solve used by the adjoint B-bar = A^-T X-bar, src/lapack/expert_refinement.jl:2-23).
+ * A zero on U's diagonal returns OXI_ERR_SINGULAR with B untouched. */
+int oxi_dgetrs(oxi_handle_t handle, char trans, int64_t n, int64_t nrhs, const double* LU, int64_t lda,
+               const int64_t* ipiv, double* B, int64_t ldb);
+
+/* ---- profiling ---------------------------------------------------------- */
+/* When enabled, oxi_dgetrf brackets every trailing-update GEMM launch with
+ * HIP events on the stream it runs on and accumulates its time and flops. */
+int oxi_profile_enable(oxi_handle_t handle, int enable);
+int oxi_profile_read(oxi_handle_t handle, double* gemm_ms, double* gemm_flops, int64_t* gemm_launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OXIBLAS_MI355X_H */

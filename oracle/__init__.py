@@ -1,0 +1,13 @@
+"""CPU oracle for the OxiBLAS.jl level-3 hot path (test infrastructure only).
+
+``oracle/oxo.cpp`` restates the reference's algorithms in C++; this module
+loads the built ``liboxo.so`` with ctypes and wraps it for numpy arrays in
+column-major (Fortran) order.  Only tests, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` use it, as the checker and the timed CPU
+baseline.  The product library never imports it.
+"""
+from .oxo import (  # noqa: F401
+    available, build, lib, max_threads,
+    dgemm, sgemm, dtrsm, dtrsv, dgetrf, sgetrf, dgetrs, sgetrs,
+    dequilibrate, refinement_residual, mixed_solve, perm_from_ipiv,
+)

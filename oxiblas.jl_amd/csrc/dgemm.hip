@@ -1,0 +1,273 @@
+// FP64 GEMM on CDNA4 matrix cores: C <- alpha*op(A)*op(B) + beta*C.
+//
+// Replaces gemm! (src/blas/gemm.jl:625-703) and its packed engine
+// (_packed_region! :414-485, _simd_micro! src/blas/simd.jl:151-199).  The CPU
+// engine packs mc x kc / kc x nc panels into cache-resident buffers; here the
+// "packing" is the staging of BK-deep slabs of op(A) and op(B) into LDS with
+// coalesced global loads, double-buffered so that the next slab is in flight
+// while the matrix cores consume the current one.
+//
+// Work decomposition: a 256-thread workgroup owns a 128x128 tile of C; its
+// four waves each own 64x64 = 4x4 blocks of 16x16 (64 FP64 accumulators per
+// lane), each block computed as four v_mfma_f64_4x4x4_4b with A broadcast.  The epilogue applies alpha and beta as the
+// reference's full tiles do: v = alpha*acc; C = beta == 0 ? v : fma(beta, C, v)
+// (src/blas/simd.jl:190-196); beta == 0 never reads C.
+#include "oxi_internal.hpp"
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
+constexpr int PAD = 1;                 // one double of padding per LDS row
+constexpr int LDS_A = BM + PAD, LDS_B = BN + PAD;
+constexpr int LOADS = BM * BK / NT;    // doubles per thread per operand per slab (8)
+
+// Load one BK-deep slab of an operand tile into registers.
+//   OUTER_CONTIG: element (o, kk) at X[(o0+o) + (k0+kk)*ldx] (op(A) for A 'N',
+//                 op(B) for B 'T'); otherwise at X[(k0+kk) + (o0+o)*ldx].
+// Out-of-range elements are zero so partial tiles need no special casing in
+// the MFMA loop.
+template <bool OUTER_CONTIG>
+__device__ __forceinline__ void load_slab(double (&r)[LOADS], const double* __restrict__ X, i64 ldx, i64 o0, i64 k0,
+                                          i64 outer, i64 K, int t) {
+#pragma unroll
+    for (int i = 0; i < LOADS; ++i) {
+        int o, kk;
+        if (OUTER_CONTIG) { o = t % BM; kk = t / BM + i * (NT / BM); }
+        else { kk = t % BK; o = t / BK + i * (NT / BK); }
+        i64 go = o0 + o, gk = k0 + kk;
+        r[i] = (go < outer && gk < K) ? (OUTER_CONTIG ? X[go + gk * ldx] : X[gk + go * ldx]) : 0.0;
+    }
+}
+
+template <bool OUTER_CONTIG, int LDS>
+__device__ __forceinline__ void store_slab(double* __restrict__ S, const double (&r)[LOADS], int t) {
+#pragma unroll
+    for (int i = 0; i < LOADS; ++i) {
+        int o, kk;
+        if (OUTER_CONTIG) { o = t % BM; kk = t / BM + i * (NT / BM); }
+        else { kk = t % BK; o = t / BK + i * (NT / BK); }
+        S[kk * LDS + o] = r[i];
+    }
+}
+
+// Map the linear workgroup id to a (tile_m, tile_n) pair.  Workgroups are
+// dispatched round-robin over the 8 XCDs (id % 8); each XCD gets a contiguous
+// range of the logical order, and the logical order walks groups of 8 tile
+// rows column by column, so the tiles resident on one XCD share A row-panels
+// and B column-panels in that XCD's L2.
+__device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int& tm, int& tn) {
+    const int total = tiles_m * tiles_n;
+    const int nx = 8;
+    int logical = bid;
+    if (total >= nx) {
+        int xcd = bid % nx, local = bid / nx, per = total / nx, rem = total % nx;
+        logical = xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+    }
+    const int G = 8;
+    int group_size = G * tiles_n;
+    int group = logical / group_size;
+    int first_m = group * G;
+    int gm = min(tiles_m - first_m, G);
+    int in = logical % group_size;
+    tm = first_m + in % gm;
+    tn = in / gm;
+}
+
+template <bool TA, bool TB, bool BETA0>
+__global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, double alpha,
+                                                        const double* __restrict__ A, i64 lda,
+                                                        const double* __restrict__ B, i64 ldb, double beta,
+                                                        double* __restrict__ C, i64 ldc, int tiles_m, int tiles_n) {
+    __shared__ double As[2][BK * LDS_A];
+    __shared__ double Bs[2][BK * LDS_B];
+
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    int tm, tn;
+    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const i64 m0 = i64(tm) * BM, n0 = i64(tn) * BN;
+
+    // op(A) rows contiguous when A is 'N'; op(B) columns contiguous when B is 'T'
+    constexpr bool A_OC = !TA, B_OC = TB;
+
+    d4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+    double ra[LOADS], rb[LOADS];
+    const int nslab = int((K + BK - 1) / BK);
+    load_slab<A_OC>(ra, A, lda, m0, 0, M, K, t);
+    load_slab<B_OC>(rb, B, ldb, n0, 0, N, K, t);
+    store_slab<A_OC, LDS_A>(As[0], ra, t);
+    store_slab<B_OC, LDS_B>(Bs[0], rb, t);
+    __syncthreads();
+
+    const int lrow = lane >> 4, lcol = lane & 15;
+    for (int s = 0; s < nslab; ++s) {
+        const int cur = s & 1;
+        if (s + 1 < nslab) {
+            load_slab<A_OC>(ra, A, lda, m0, i64(s + 1) * BK, M, K, t);
+            load_slab<B_OC>(rb, B, ldb, n0, i64(s + 1) * BK, N, K, t);
+        }
+        const double* as = As[cur];
+        const double* bs = Bs[cur];
+#pragma unroll
+        for (int ks = 0; ks < BK / 4; ++ks) {
+            const int kr = ks * 4 + lrow;
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = as[kr * LDS_A + wm * 64 + i * 16 + lcol];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = bs[kr * LDS_B + wn * 64 + j * 16 + lcol];
+            // v_mfma_f64_4x4x4_4b with A broadcast (CBSZ=2, ABID=r): the four
+            // 4x4 blocks share the A block held by lane group r, so one
+            // instruction computes rows 4r..4r+3 of the 16x16 block for k=4
+            // with the same A/B register layout as v_mfma_f64_16x16x4, at the
+            // full FP64 matrix rate on gfx950.
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    acc[i][j][0] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[i], b[j], acc[i][j][0], 2, 0, 0);
+                    acc[i][j][1] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[i], b[j], acc[i][j][1], 2, 1, 0);
+                    acc[i][j][2] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[i], b[j], acc[i][j][2], 2, 2, 0);
+                    acc[i][j][3] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[i], b[j], acc[i][j][3], 2, 3, 0);
+                }
+        }
+        if (s + 1 < nslab) {
+            store_slab<A_OC, LDS_A>(As[cur ^ 1], ra, t);
+            store_slab<B_OC, LDS_B>(Bs[cur ^ 1], rb, t);
+        }
+        __syncthreads();
+    }
+
+    // Accumulator layout (measured on gfx950, the same for v_mfma_f64_16x16x4
+    // and for the four broadcast 4x4x4_4b): lane l, item r holds row
+    // 4*r + l/16, column l%16 of the 16x16 block.
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const i64 col = n0 + wn * 64 + j * 16 + lcol;
+            if (col >= N) continue;
+            double* cc = C + col * ldc;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const i64 row = m0 + wm * 64 + i * 16 + r * 4 + lrow;
+                if (row < M) {
+                    const double v = alpha * acc[i][j][r];
+                    cc[row] = BETA0 ? v : __builtin_fma(beta, cc[row], v);
+                }
+            }
+        }
+}
+
+__global__ void dscale_kernel(i64 m, i64 n, double beta, double* __restrict__ C, i64 ldc) {
+    const i64 total = m * n;
+    for (i64 idx = i64(blockIdx.x) * blockDim.x + threadIdx.x; idx < total; idx += i64(gridDim.x) * blockDim.x) {
+        const i64 j = idx / m, i = idx - j * m;
+        double& c = C[i + j * ldc];
+        c = (beta == 0.0) ? 0.0 : c * beta;
+    }
+}
+
+template <bool TA, bool TB>
+void launch_variant(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda, const double* B,
+                    i64 ldb, double beta, double* C, i64 ldc) {
+    const int tiles_m = int(cdiv(m, BM)), tiles_n = int(cdiv(n, BN));
+    dim3 grid(tiles_m * tiles_n), block(NT);
+    if (beta == 0.0)
+        hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, true>), grid, block, 0, s, m, n, k, alpha, A, lda, B, ldb, beta,
+                           C, ldc, tiles_m, tiles_n);
+    else
+        hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, false>), grid, block, 0, s, m, n, k, alpha, A, lda, B, ldb,
+                           beta, C, ldc, tiles_m, tiles_n);
+}
+
+}  // namespace
+
+int oxi_launch_dscale(hipStream_t s, i64 m, i64 n, double beta, double* C, i64 ldc) {
+    if (m <= 0 || n <= 0 || beta == 1.0) return OXI_OK;
+    i64 total = m * n;
+    int blocks = int(std::min<i64>(cdiv(total, 256), 4096));
+    hipLaunchKernelGGL(dscale_kernel, dim3(blocks), dim3(256), 0, s, m, n, beta, C, ldc);
+    return hipGetLastError() == hipSuccess ? OXI_OK : OXI_ERR_HIP;
+}
+
+int oxi_launch_dgemm(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n, i64 k, double alpha,
+                     const double* A, i64 lda, const double* B, i64 ldb, double beta, double* C, i64 ldc) {
+    if (m <= 0 || n <= 0) return OXI_OK;
+    if (alpha == 0.0 || k <= 0) return oxi_launch_dscale(s, m, n, beta, C, ldc);
+    if (cdiv(m, BM) * cdiv(n, BN) > (i64(1) << 31) - 1)
+        return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "dgemm: %lld x %lld has too many tiles", (long long)m,
+                             (long long)n);
+    if (!ta && !tb) launch_variant<false, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (!ta && tb) launch_variant<false, true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (ta && !tb) launch_variant<true, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else launch_variant<true, true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "dgemm launch: %s", hipGetErrorString(e));
+    return OXI_OK;
+}
+
+namespace {
+int trans_kind(char t) {
+    if (t == 'N' || t == 'n') return 0;
+    if (t == 'T' || t == 't' || t == 'C' || t == 'c') return 1;
+    return -1;
+}
+bool overlap(const void* a, size_t ab, const void* b, size_t bb) {
+    const char* x = (const char*)a; const char* y = (const char*)b;
+    return x < y + bb && y < x + ab;
+}
+size_t span_bytes(i64 rows, i64 cols, i64 ld) { return rows <= 0 || cols <= 0 ? 0 : size_t((cols - 1) * ld + rows) * 8; }
+}  // namespace
+
+extern "C" int oxi_dgemm(oxi_handle_t h, char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
+                         const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                         int64_t ldc) {
+    if (!h) return -1;
+    const int ka = trans_kind(transA), kb = trans_kind(transB);
+    if (ka < 0) return -2;
+    if (kb < 0) return -3;
+    if (m < 0) return -4;
+    if (n < 0) return -5;
+    if (k < 0) return -6;
+    const i64 ar = ka ? k : m, ac = ka ? m : k, br = kb ? n : k, bc = kb ? k : n;
+    if (lda < std::max<i64>(1, ar)) return -9;
+    if (ldb < std::max<i64>(1, br)) return -11;
+    if (ldc < std::max<i64>(1, m)) return -14;
+    std::lock_guard<std::recursive_mutex> lk(h->guard);
+    if (m == 0 || n == 0) return OXI_OK;
+    (void)hipSetDevice(h->device);
+    hipStream_t s = h->stream;
+    if (alpha == 0.0 || k == 0) {
+        OXI_TRY(oxi_launch_dscale(s, m, n, beta, C, ldc));
+        return OXI_OK;
+    }
+    // Overlapping operands are copied before C is written (src/blas/gemm.jl:682-688).
+    const size_t cbytes = span_bytes(m, n, ldc);
+    const bool ca = overlap(C, cbytes, A, span_bytes(ar, ac, lda));
+    const bool cb = overlap(C, cbytes, B, span_bytes(br, bc, ldb));
+    if (ca || cb) {
+        size_t need = (ca ? size_t(ar) * ac * 8 : 0) + (cb ? size_t(br) * bc * 8 : 0) + 256;
+        void* ws;
+        OXI_TRY(oxi_workspace(h, need, &ws));
+        double* p = (double*)ws;
+        if (ca) {
+            OXI_HIP(h, hipMemcpy2DAsync(p, ar * 8, A, lda * 8, ar * 8, ac, hipMemcpyDeviceToDevice, s));
+            A = p; lda = ar; p += size_t(ar) * ac;
+            p = (double*)(((uintptr_t)p + 255) & ~uintptr_t(255));
+        }
+        if (cb) {
+            OXI_HIP(h, hipMemcpy2DAsync(p, br * 8, B, ldb * 8, br * 8, bc, hipMemcpyDeviceToDevice, s));
+            B = p; ldb = br;
+        }
+    }
+    return oxi_launch_dgemm(h, s, ka, kb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}

@@ -1,0 +1,251 @@
+// Triangular solves (trsm!, src/blas/level3.jl:309-544) on gfx950: leaves of
+// at most 64 rows/columns solved in registers or with one wave per right-hand
+// side, and the reference's recursive halving with FP64 MFMA GEMM updates in
+// between (_trsm_blocks_left! / _trsm_blocks_right!, level3.jl:338-376).
+#include "oxi_internal.hpp"
+
+#include <algorithm>
+
+namespace {
+constexpr int TRL = 64;
+
+// Right side: X (rows x n) <- X * op(T)^-1, one thread per row of X.  Element
+// order matches the reference's right leaf (src/blas/level3.jl:319-333):
+// op(T) upper: x_c = (b_c - sum_{p<c} x_p U[p][c]) / U[c][c], c ascending;
+// op(T) lower: x_c = (b_c - sum_{p>c} x_p L[p][c]) / L[c][c], c descending.
+template <bool LOWER, bool UNIT>
+__global__ __launch_bounds__(256) void trsm_right_leaf(double* __restrict__ X, i64 ldx, i64 rows, int n,
+                                                       const double* __restrict__ T, i64 ldt, int trans) {
+    __shared__ double U[TRL * TRL];  // U[p*TRL + c] = op(T)[p][c]
+    for (int e = threadIdx.x; e < TRL * TRL; e += blockDim.x) {
+        const int pp = e / TRL, c = e % TRL;
+        double val = 0.0;
+        if (pp < n && c < n) val = trans ? T[c + i64(pp) * ldt] : T[pp + i64(c) * ldt];
+        U[pp * TRL + c] = val;
+    }
+    __syncthreads();
+    const i64 r = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    double x[TRL];
+#pragma unroll
+    for (int c = 0; c < TRL; ++c) x[c] = c < n ? X[r + c * ldx] : 0.0;
+    if (!LOWER) {
+#pragma unroll
+        for (int c = 0; c < TRL; ++c) {
+            if (c < n) {
+                double s = x[c];
+#pragma unroll
+                for (int pp = 0; pp < c; ++pp) s = __builtin_fma(-x[pp], U[pp * TRL + c], s);
+                x[c] = UNIT ? s : s / U[c * TRL + c];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int c = TRL - 1; c >= 0; --c) {
+            if (c < n) {
+                double s = x[c];
+#pragma unroll
+                for (int pp = TRL - 1; pp > c; --pp)
+                    if (pp < n) s = __builtin_fma(-x[pp], U[pp * TRL + c], s);
+                x[c] = UNIT ? s : s / U[c * TRL + c];
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < TRL; ++c) if (c < n) X[r + c * ldx] = x[c];
+}
+
+// Left side: X (n x nrhs) <- op(T)^-1 X, one wave per right-hand side, lane =
+// row.  LOWER refers to op(T).  Per element: s = b_i; s = fma(-T[i][p], x_p, s)
+// in order of p; s / T[i][i] when not unit (src/blas/level3.jl:309-318,
+// src/blas/triangular_microkernels.jl:31-59).
+template <bool LOWER, bool UNIT>
+__global__ __launch_bounds__(256) void trsm_left_leaf(double* __restrict__ X, i64 ldx, i64 nrhs, int n,
+                                                      const double* __restrict__ T, i64 ldt, int trans) {
+    __shared__ double S[TRL * (TRL + 1)];  // S[p*(TRL+1) + i] = op(T)[i][p]
+    for (int e = threadIdx.x; e < TRL * TRL; e += blockDim.x) {
+        const int pp = e / TRL, i = e % TRL;
+        double val = 0.0;
+        if (pp < n && i < n) val = trans ? T[pp + i64(i) * ldt] : T[i + i64(pp) * ldt];
+        S[pp * (TRL + 1) + i] = val;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const i64 col = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (col >= nrhs) return;
+    double* xc = X + col * ldx;
+    double x = lane < n ? xc[lane] : 0.0;
+    if (LOWER) {
+        for (int pp = 0; pp < n; ++pp) {
+            if (!UNIT && lane == pp) x = x / S[pp * (TRL + 1) + pp];
+            const double xp = __shfl(x, pp);
+            if (lane > pp) x = __builtin_fma(-S[pp * (TRL + 1) + lane], xp, x);
+        }
+    } else {
+        for (int pp = n - 1; pp >= 0; --pp) {
+            if (!UNIT && lane == pp) x = x / S[pp * (TRL + 1) + pp];
+            const double xp = __shfl(x, pp);
+            if (lane < pp) x = __builtin_fma(-S[pp * (TRL + 1) + lane], xp, x);
+        }
+    }
+    if (lane < n) xc[lane] = x;
+}
+
+// Diagonal check for the non-unit solves: first zero on the diagonal of T (1-based) or 0.
+__global__ void diag_zero_kernel(const double* __restrict__ T, i64 ldt, i64 n, i64* __restrict__ out) {
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += i64(gridDim.x) * blockDim.x)
+        if (T[i + i * ldt] == 0.0) atomicMin((unsigned long long*)out, (unsigned long long)(i + 1));
+}
+__global__ void set_i64_kernel(i64* p, i64 v) { *p = v; }
+
+
+int launch_check(oxi_handle_t h, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "%s launch: %s", what, hipGetErrorString(e));
+    return OXI_OK;
+}
+}  // namespace
+
+// Right-side triangular solve X <- X * op(T)^-1 where op(T) is upper
+// triangular (T upper and trans == 0, or T lower and trans == 1), recursive
+// halving with GEMM updates (src/blas/level3.jl:358-376).
+static int trsm_right_upper(oxi_handle_t h, hipStream_t s, int trans, int unit, i64 rows, i64 n, const double* T,
+                            i64 ldt, double* X, i64 ldx) {
+    if (rows <= 0 || n <= 0) return OXI_OK;
+    if (n <= TRL) {
+        dim3 grid(unsigned(cdiv(rows, 256)));
+        if (unit) hipLaunchKernelGGL((trsm_right_leaf<false, true>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
+        else hipLaunchKernelGGL((trsm_right_leaf<false, false>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
+        return launch_check(h, "trsm_right_leaf");
+    }
+    i64 hh = n / 2;
+    if (hh > TRL) hh = (hh / TRL) * TRL;
+    // X1 = X[:, :hh], X2 = X[:, hh:]; op(T) = [U11 U12; 0 U22]
+    OXI_TRY(trsm_right_upper(h, s, trans, unit, rows, hh, T, ldt, X, ldx));
+    // X2 -= X1 * U12, U12 = op(T)[0:hh, hh:n]
+    const double* U12 = trans ? T + hh : T + hh * ldt;  // op(T)[0][hh]: trans -> T[hh][0]
+    OXI_TRY(oxi_launch_dgemm(h, s, 0, trans, rows, n - hh, hh, -1.0, X, ldx, U12, ldt, 1.0, X + hh * ldx, ldx));
+    return trsm_right_upper(h, s, trans, unit, rows, n - hh, T + hh + hh * ldt, ldt, X + hh * ldx, ldx);
+}
+
+// Right-side solve with op(T) lower: X <- X * op(T)^-1 (backward over columns).
+static int trsm_right_lower(oxi_handle_t h, hipStream_t s, int trans, int unit, i64 rows, i64 n, const double* T,
+                            i64 ldt, double* X, i64 ldx);
+
+int oxi_dtrsm_left(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 nrhs, const double* A,
+                   i64 lda, double* B, i64 ldb) {
+    if (m <= 0 || nrhs <= 0) return OXI_OK;
+    const bool lower_op = (!upper) != (trans != 0);  // op(A) lower?
+    if (m <= TRL) {
+        dim3 grid(unsigned(cdiv(nrhs, 4)));
+        if (lower_op) {
+            if (unit) hipLaunchKernelGGL((trsm_left_leaf<true, true>), grid, dim3(256), 0, s, B, ldb, nrhs, int(m), A, lda, trans);
+            else hipLaunchKernelGGL((trsm_left_leaf<true, false>), grid, dim3(256), 0, s, B, ldb, nrhs, int(m), A, lda, trans);
+        } else {
+            if (unit) hipLaunchKernelGGL((trsm_left_leaf<false, true>), grid, dim3(256), 0, s, B, ldb, nrhs, int(m), A, lda, trans);
+            else hipLaunchKernelGGL((trsm_left_leaf<false, false>), grid, dim3(256), 0, s, B, ldb, nrhs, int(m), A, lda, trans);
+        }
+        return launch_check(h, "trsm_left_leaf");
+    }
+    i64 hh = m / 2;
+    if (hh > TRL) hh = (hh / TRL) * TRL;
+    // op(A) = [A11 A12; A21 A22] in op coordinates; op(A)[i][j] = trans ? A[j][i] : A[i][j]
+    const double* A11 = A;
+    const double* A22 = A + hh + hh * lda;
+    if (lower_op) {
+        OXI_TRY(oxi_dtrsm_left(h, s, upper, trans, unit, hh, nrhs, A11, lda, B, ldb));
+        // B2 -= op(A)[hh:, :hh] * B1 ; op(A)[hh][0] = trans ? A[0][hh] : A[hh][0]
+        const double* A21 = trans ? A + hh * lda : A + hh;
+        OXI_TRY(oxi_launch_dgemm(h, s, trans, 0, m - hh, nrhs, hh, -1.0, A21, lda, B, ldb, 1.0, B + hh, ldb));
+        return oxi_dtrsm_left(h, s, upper, trans, unit, m - hh, nrhs, A22, lda, B + hh, ldb);
+    }
+    OXI_TRY(oxi_dtrsm_left(h, s, upper, trans, unit, m - hh, nrhs, A22, lda, B + hh, ldb));
+    // B1 -= op(A)[:hh, hh:] * B2 ; op(A)[0][hh] = trans ? A[hh][0] : A[0][hh]
+    const double* A12 = trans ? A + hh : A + hh * lda;
+    OXI_TRY(oxi_launch_dgemm(h, s, trans, 0, hh, nrhs, m - hh, -1.0, A12, lda, B + hh, ldb, 1.0, B, ldb));
+    return oxi_dtrsm_left(h, s, upper, trans, unit, hh, nrhs, A11, lda, B, ldb);
+}
+
+// Right-side solve with op(T) lower: X <- X * op(T)^-1.  op(T) = [L11 0; L21 L22]:
+// X2 = B2 * L22^-1 first, then B1 -= X2 * L21, then X1 = B1 * L11^-1.
+static int trsm_right_lower(oxi_handle_t h, hipStream_t s, int trans, int unit, i64 rows, i64 n, const double* T,
+                            i64 ldt, double* X, i64 ldx) {
+    if (rows <= 0 || n <= 0) return OXI_OK;
+    if (n <= TRL) {
+        dim3 grid(unsigned(cdiv(rows, 256)));
+        if (unit) hipLaunchKernelGGL((trsm_right_leaf<true, true>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
+        else hipLaunchKernelGGL((trsm_right_leaf<true, false>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
+        return launch_check(h, "trsm_right_leaf");
+    }
+    i64 hh = n / 2;
+    if (hh > TRL) hh = (hh / TRL) * TRL;
+    OXI_TRY(trsm_right_lower(h, s, trans, unit, rows, n - hh, T + hh + hh * ldt, ldt, X + hh * ldx, ldx));
+    const double* L21 = trans ? T + hh * ldt : T + hh;  // op(T)[hh][0]
+    OXI_TRY(oxi_launch_dgemm(h, s, 0, trans, rows, hh, n - hh, -1.0, X + hh * ldx, ldx, L21, ldt, 1.0, X, ldx));
+    return trsm_right_lower(h, s, trans, unit, rows, hh, T, ldt, X, ldx);
+}
+
+int oxi_dtrsm_right(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 n, const double* A,
+                    i64 lda, double* B, i64 ldb) {
+    const bool upper_op = (upper != 0) != (trans != 0);
+    if (upper_op) return trsm_right_upper(h, s, trans, unit, m, n, A, lda, B, ldb);
+    return trsm_right_lower(h, s, trans, unit, m, n, A, lda, B, ldb);
+}
+
+
+static int trans_kind(char t) {
+    if (t == 'N' || t == 'n') return 0;
+    if (t == 'T' || t == 't' || t == 'C' || t == 'c') return 1;
+    return -1;
+}
+
+// First zero on the diagonal of an n x n triangle (1-based), synchronising.
+int oxi_first_zero_diag(oxi_handle_t h, hipStream_t s, const double* T, i64 ldt, i64 n, i64* where) {
+    void* ws;
+    OXI_TRY(oxi_workspace(h, 256, &ws));
+    i64* d = (i64*)ws;
+    hipLaunchKernelGGL(set_i64_kernel, dim3(1), dim3(1), 0, s, d, i64(0x7fffffffffffffffLL));
+    hipLaunchKernelGGL(diag_zero_kernel, dim3(unsigned(std::min<i64>(cdiv(n, 256), 1024))), dim3(256), 0, s, T, ldt, n, d);
+    OXI_TRY(launch_check(h, "diag check"));
+    i64 v = 0;
+    OXI_HIP(h, hipMemcpyAsync(&v, d, sizeof(i64), hipMemcpyDeviceToHost, s));
+    OXI_HIP(h, hipStreamSynchronize(s));
+    *where = (v == i64(0x7fffffffffffffffLL)) ? 0 : v;
+    return OXI_OK;
+}
+
+extern "C" int oxi_dtrsm(oxi_handle_t h, char side, char uplo, char trans, char diag, int64_t m, int64_t n,
+                         double alpha, const double* A, int64_t lda, double* B, int64_t ldb, int64_t* singular_index) {
+    if (!h) return -1;
+    const bool left = side == 'L' || side == 'l';
+    if (!left && !(side == 'R' || side == 'r')) return -2;
+    const bool upper = uplo == 'U' || uplo == 'u';
+    if (!upper && !(uplo == 'L' || uplo == 'l')) return -3;
+    const int tk = trans_kind(trans);
+    if (tk < 0) return -4;
+    const bool unit = diag == 'U' || diag == 'u';
+    if (!unit && !(diag == 'N' || diag == 'n')) return -5;
+    if (m < 0) return -6;
+    if (n < 0) return -7;
+    const i64 na = left ? m : n;
+    if (lda < std::max<i64>(1, na)) return -10;
+    if (ldb < std::max<i64>(1, m)) return -12;
+    if (singular_index) *singular_index = 0;
+    std::lock_guard<std::recursive_mutex> lk(h->guard);
+    if (m == 0 || n == 0) return OXI_OK;
+    (void)hipSetDevice(h->device);
+    hipStream_t s = h->stream;
+    if (alpha == 0.0) return oxi_launch_dscale(s, m, n, 0.0, B, ldb);
+    if (!unit) {
+        i64 z = 0;
+        OXI_TRY(oxi_first_zero_diag(h, s, A, lda, na, &z));
+        if (z) {
+            if (singular_index) *singular_index = z;
+            return oxi_set_error(h, OXI_ERR_SINGULAR, "trsm: zero diagonal at %lld", (long long)z);
+        }
+    }
+    if (alpha != 1.0) OXI_TRY(oxi_launch_dscale(s, m, n, alpha, B, ldb));
+    if (left) return oxi_dtrsm_left(h, s, upper, tk, unit, m, n, A, lda, B, ldb);
+    return oxi_dtrsm_right(h, s, upper, tk, unit, m, n, A, lda, B, ldb);
+}
+

@@ -1,0 +1,19 @@
+"""oxiblas_mi355x -- MI355X (gfx950) implementation of OxiBLAS.jl's level-3
+hot path: gemm!, the blocked LU with partial pivoting and its solves.
+
+Every computation runs in liboxiblas_mi355x.so (hand-written HIP kernels for
+CDNA4) through its C ABI; this package is the host-side mirror of the
+reference's API used by tests and benchmarks.
+"""
+from ._lib import load, handle, Handle, OxiError, LIB_PATH, SIGNATURES  # noqa: F401
+from .device import DMatrix, as_dmatrix  # noqa: F401
+from .linalg import (  # noqa: F401
+    gemm_, trsm_, lu, lu_, lu_factor, LU, ldiv, ldiv_, solve, solve_transposed,
+    transpose, adjoint, SingularException, DimensionMismatch,
+)
+
+__all__ = [
+    "load", "handle", "Handle", "OxiError", "DMatrix", "as_dmatrix",
+    "gemm_", "trsm_", "lu", "lu_", "lu_factor", "LU", "ldiv", "ldiv_", "solve", "solve_transposed",
+    "transpose", "adjoint", "SingularException", "DimensionMismatch",
+]

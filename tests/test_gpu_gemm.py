@@ -1,0 +1,177 @@
+"""FP64 GEMM on the GPU (oxi_dgemm through the C ABI) against the oracle.
+
+Follows test/blas/gemm.jl: every N/T/C combination over its shapes, the
+no-read rules, aliasing snapshots and views with guard regions; plus exact
+integer inputs (bit-identical results), the golden fixtures and the full-size
+shapes of the benchmark (8192^3 and the 16384 x 16384 x 256 trailing update)
+checked entry-wise on a random sample.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(np.float64).eps
+SHAPES = ((0, 0, 0), (0, 3, 7), (4, 0, 5), (3, 4, 0), (1, 1, 1), (2, 3, 5),
+          (7, 9, 11), (8, 6, 4), (17, 19, 23), (33, 13, 35), (65, 67, 49))  # test/blas/gemm.jl:2-3
+
+
+def op(a, t):
+    return a if t == "N" else a.T
+
+
+def run(ox, C0, A, B, al, be, ta, tb):
+    Cd = ox.DMatrix.from_numpy(C0)
+    ox.gemm_(Cd, ox.DMatrix.from_numpy(A), ox.DMatrix.from_numpy(B), al, be, ta, tb)
+    return Cd.numpy()
+
+
+@pytest.mark.parametrize("ta", "NTC")
+@pytest.mark.parametrize("tb", "NTC")
+def test_shapes_against_oracle(gpu, rng, ta, tb):
+    for (m, n, k) in SHAPES + ((129, 130, 257), (300, 129, 512), (128, 128, 16), (257, 1, 300), (1, 300, 77)):
+        A = rng.standard_normal((m, k) if ta == "N" else (k, m))
+        B = rng.standard_normal((k, n) if tb == "N" else (n, k))
+        C0 = np.asfortranarray(rng.standard_normal((m, n)))
+        got = run(gpu, C0, A, B, 0.7, -0.4, ta, tb)
+        ref = C0.copy(order="F")
+        O.dgemm(ref, A, B, 0.7, -0.4, ta, tb)
+        # the reference's tolerance (test/blas/gemm.jl:13) and the normwise bound
+        np.testing.assert_allclose(got, ref, rtol=3e-12, atol=3e-12 * max(1, k))
+        if m and n:
+            bound = 2 * (k + 2) * EPS * (0.7 * np.linalg.norm(A) * np.linalg.norm(B) + 0.4 * np.linalg.norm(C0))
+            assert np.linalg.norm(got - ref) <= bound
+
+
+def test_integer_inputs_bit_exact(gpu, rng):
+    for (m, n, k) in ((65, 67, 49), (600, 300, 500), (5, 7, 900), (129, 257, 130)):
+        for ta in "NT":
+            for tb in "NT":
+                A = rng.integers(-8, 9, (m, k) if ta == "N" else (k, m)).astype(float)
+                B = rng.integers(-8, 9, (k, n) if tb == "N" else (n, k)).astype(float)
+                C0 = np.asfortranarray(rng.integers(-8, 9, (m, n)).astype(float))
+                got = run(gpu, C0, A, B, 2.0, -3.0, ta, tb)
+                assert np.array_equal(got, 2.0 * (op(A, ta) @ op(B, tb)) - 3.0 * C0)
+
+
+def test_no_read_rules(gpu, rng):
+    ox = gpu
+    A = rng.standard_normal((17, 13)); B = rng.standard_normal((13, 19))
+    got = run(ox, np.full((17, 19), np.nan), A, B, 1.0, 0.0, "N", "N")
+    np.testing.assert_allclose(got, A @ B, rtol=3e-12, atol=3e-12 * 13)     # beta == 0 never reads C
+    for beta in (0.0, 1.0, 0.5):
+        got = run(ox, np.full((9, 7), 2.0), np.full((9, 3), np.nan), np.full((3, 7), np.nan), 0.0, beta, "N", "N")
+        assert np.array_equal(got, np.full((9, 7), 2.0 * beta))              # alpha == 0 never reads A, B
+    got = run(ox, np.full((5, 7), 4.0), np.zeros((5, 0)), np.zeros((0, 7)), np.nan, 0.5, "N", "N")
+    assert np.array_equal(got, np.full((5, 7), 2.0))                         # k == 0
+
+
+def test_aliasing_snapshots(gpu, rng):
+    ox = gpu
+    # C is A, C is B (test/blas/gemm.jl:56-62)
+    A = rng.standard_normal((17, 17)); B = rng.standard_normal((17, 17))
+    Ad = ox.DMatrix.from_numpy(A); Bd = ox.DMatrix.from_numpy(B)
+    ox.gemm_(Ad, Ad, Bd, 1.0, 0.3)
+    np.testing.assert_allclose(Ad.numpy(), A @ B + 0.3 * A, rtol=3e-12, atol=3e-12 * 17)
+    A2 = Ad.numpy()
+    ox.gemm_(Bd, Ad, Bd)
+    np.testing.assert_allclose(Bd.numpy(), A2 @ B, rtol=3e-12, atol=3e-12 * 17)
+    # three overlapping views of one matrix (:63-66)
+    X = rng.standard_normal((20, 20))
+    Xd = ox.DMatrix.from_numpy(X)
+    C = Xd.view(slice(1, 18), slice(1, 18)); Av = Xd.view(slice(0, 17), slice(0, 17)); Bv = Xd.view(slice(2, 19), slice(2, 19))
+    expected = X[0:17, 0:17] @ X[2:19, 2:19] + X[1:18, 1:18]
+    ox.gemm_(C, Av, Bv, 1.0, 1.0)
+    np.testing.assert_allclose(Xd.numpy()[1:18, 1:18], expected, rtol=3e-12, atol=3e-12 * 17)
+
+
+def test_views_and_guard_regions(gpu, rng):
+    ox = gpu
+    # unit row stride with padded leading dimension; unaligned first row (test/blas/gemm.jl:28-33)
+    A = rng.standard_normal((17, 15)); B = rng.standard_normal((15, 19))
+    storage = np.full((23, 22), 88.0)
+    Sd = ox.DMatrix.from_numpy(storage)
+    C = Sd.view(slice(1, 18), slice(2, 21))
+    ox.gemm_(C, ox.DMatrix.from_numpy(A), ox.DMatrix.from_numpy(B))
+    S = Sd.numpy()
+    np.testing.assert_allclose(S[1:18, 2:21], A @ B, rtol=3e-12, atol=3e-12 * 15)
+    mask = np.ones_like(S, dtype=bool); mask[1:18, 2:21] = False
+    assert np.all(S[mask] == 88.0)
+    # operands with a leading dimension larger than their rows
+    Ab = ox.DMatrix.from_numpy(A, ld=40); Bb = ox.DMatrix.from_numpy(B, ld=33)
+    Cd = ox.DMatrix.from_numpy(np.zeros((17, 19)), ld=21)
+    ox.gemm_(Cd, Ab, Bb)
+    np.testing.assert_allclose(Cd.numpy(), A @ B, rtol=3e-12, atol=3e-12 * 15)
+
+
+def test_argument_validation(gpu):
+    ox = gpu
+    C = ox.DMatrix.from_numpy(np.zeros((64, 64))); A = ox.DMatrix.from_numpy(np.ones((64, 64)))
+    with pytest.raises(ox.DimensionMismatch):
+        ox.gemm_(C, A, ox.DMatrix.from_numpy(np.zeros((63, 64))))
+    with pytest.raises(ValueError):
+        ox.gemm_(C, A, A, transA="bad")
+    assert np.all(C.numpy() == 0)
+
+
+def test_golden_fixtures(gpu):
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "gemm_cases.npz"), allow_pickle=False)
+    for i in range(int(z["count"])):
+        m, n, k, ta, tb = (int(v) for v in z[f"c{i}_meta"])
+        al, be = z[f"c{i}_ab"]
+        got = run(gpu, z[f"c{i}_C0"], z[f"c{i}_A"], z[f"c{i}_B"], al, be, chr(ta), chr(tb))
+        np.testing.assert_allclose(got, z[f"c{i}_C"], rtol=3e-12, atol=3e-12 * max(1, k))
+
+
+def test_run_to_run_bitwise(gpu, rng):
+    A = rng.standard_normal((700, 900)); B = rng.standard_normal((900, 650)); C0 = rng.standard_normal((700, 650))
+    r1 = run(gpu, C0, A, B, -1.0, 1.0, "N", "N")
+    r2 = run(gpu, C0, A, B, -1.0, 1.0, "N", "N")
+    assert np.array_equal(r1, r2)
+
+
+def sampled_check(rows_a, cols_b, c0, got, al, be):
+    """Entry-wise check on a random sample against exact dot products:
+    |C_ij - ref_ij| <= (k+2) eps (|alpha| sum|a||b| + |beta c|).
+    rows_a[s] = row i_s of op(A), cols_b[s] = column j_s of op(B)."""
+    prod = (rows_a.astype(np.longdouble) * cols_b.astype(np.longdouble)).sum(axis=1)
+    ref = al * prod.astype(np.float64) + be * c0
+    k = rows_a.shape[1]
+    tol = (k + 2) * EPS * (abs(al) * np.sum(np.abs(rows_a) * np.abs(cols_b), axis=1) + np.abs(be * c0))
+    bad = np.abs(got - ref) > tol
+    assert not bad.any(), (np.flatnonzero(bad)[:5], got[bad][:5], ref[bad][:5])
+
+
+@pytest.mark.parametrize("shape", [(8192, 8192, 8192), (16384, 16384, 256)])
+def test_full_size_shapes(gpu, shape):
+    """The benchmark shapes, alpha=1, beta=-1 (the update form of the GEMM
+    workload), all four transpose variants, data generated on the GPU and
+    300 random entries checked against exact dot products."""
+    ox = gpu
+    import torch
+    m, n, k = shape
+    g = torch.Generator(device="cuda").manual_seed(0x0B1A5 + k)
+    C0 = torch.randn(m * n, dtype=torch.float64, device="cuda", generator=g)
+    rng = np.random.default_rng(k)
+    ii = torch.from_numpy(rng.integers(0, m, 300)).cuda(); jj = torch.from_numpy(rng.integers(0, n, 300)).cuda()
+    c0 = C0.view(n, m)[jj, ii].cpu().numpy()
+    for ta in "NT":
+        for tb in "NT":
+            A = torch.randn(m * k, dtype=torch.float64, device="cuda", generator=g)   # storage of A (or A^T)
+            B = torch.randn(k * n, dtype=torch.float64, device="cuda", generator=g)
+            Ad = ox.DMatrix(A, m, k) if ta == "N" else ox.DMatrix(A, k, m)
+            Bd = ox.DMatrix(B, k, n) if tb == "N" else ox.DMatrix(B, n, k)
+            Cd = ox.DMatrix(C0.clone(), m, n)
+            ox.gemm_(Cd, Ad, Bd, 1.0, -1.0, ta, tb)
+            torch.cuda.synchronize()
+            # column-major storage viewed as [col][row]
+            Am = A.view(k, m) if ta == "N" else A.view(m, k)     # Am[p, i] = A[i, p] ('N') ; Am[i, p] = A[p, i] ('T')
+            rows_a = (Am[:, ii].T if ta == "N" else Am[ii, :]).cpu().numpy()
+            Bm = B.view(n, k) if tb == "N" else B.view(k, n)     # Bm[j, p] = B[p, j] ('N') ; Bm[p, j] = B[j, p] ('T')
+            cols_b = (Bm[jj, :] if tb == "N" else Bm[:, jj].T).cpu().numpy()
+            got = Cd.buf.view(n, m)[jj, ii].cpu().numpy()
+            sampled_check(rows_a, cols_b, c0, got, 1.0, -1.0)
+            del A, B, Cd

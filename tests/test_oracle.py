@@ -1,0 +1,278 @@
+"""Pins the CPU oracle (oracle/oxo.cpp) before it is trusted as the checker.
+
+The reference ships no golden vectors; its own tests compare against
+LinearAlgebra/LAPACK computed at test time (test/testsetup.jl:1-3).  The same
+comparisons pin the oracle here: numpy/OpenBLAS products, LAPACK dgetrf pivots
+(test/lapack/lapack_dense.jl:12-14), scipy triangular solves, the normwise
+bounds of the reference's tests, exact integer arithmetic and the reference's
+no-read / singular / convergence contracts.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import oracle as O
+
+EPS = np.finfo(np.float64).eps
+GEMM_SHAPES = ((0, 0, 0), (0, 3, 7), (4, 0, 5), (3, 4, 0), (1, 1, 1), (2, 3, 5),
+               (7, 9, 11), (8, 6, 4), (17, 19, 23), (33, 13, 35), (65, 67, 49))  # test/blas/gemm.jl:2-3
+
+
+def op(a, t):
+    return a if t == "N" else a.T
+
+
+def bwd(E, A):
+    """_bwd of test/lapack/lapack_dense.jl:4."""
+    return np.linalg.norm(E) / (max(np.linalg.norm(A), np.finfo(float).tiny) * max(1, *A.shape) * EPS)
+
+
+# ---------------------------------------------------------------------------
+# gemm
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("ta", "NTC")
+@pytest.mark.parametrize("tb", "NTC")
+def test_gemm_shapes_vs_numpy(rng, ta, tb):
+    for (m, n, k) in GEMM_SHAPES:
+        A = rng.standard_normal((m, k) if ta == "N" else (k, m))
+        B = rng.standard_normal((k, n) if tb == "N" else (n, k))
+        C = np.asfortranarray(rng.standard_normal((m, n)))
+        old = C.copy()
+        expected = 0.7 * (op(A, ta) @ op(B, tb)) - 0.4 * old
+        O.dgemm(C, A, B, 0.7, -0.4, ta, tb)
+        # the reference's tolerance (test/blas/gemm.jl:13, tol(Float64)=3e-12)
+        np.testing.assert_allclose(C, expected, rtol=3e-12, atol=3e-12 * max(1, k))
+        # normwise bound k*eps*|A||B| (+ the beta*C rounding)
+        if m and n:
+            err = np.linalg.norm(C - expected)
+            bound = (k + 2) * EPS * (0.7 * np.linalg.norm(A) * np.linalg.norm(B) + 0.4 * np.linalg.norm(old))
+            assert err <= bound
+
+
+def test_gemm_integer_inputs_bit_exact(rng):
+    # small integers: every product and partial sum is exact, so any blocking
+    # gives the exact result -- a wrong index or transposition cannot hide
+    for (m, n, k) in ((65, 67, 49), (600, 300, 500), (5, 7, 900), (16, 6, 448), (17, 7, 449)):
+        for ta in "NT":
+            for tb in "NT":
+                A = rng.integers(-8, 9, (m, k) if ta == "N" else (k, m)).astype(float)
+                B = rng.integers(-8, 9, (k, n) if tb == "N" else (n, k)).astype(float)
+                C = np.asfortranarray(rng.integers(-8, 9, (m, n)).astype(float))
+                expected = 2.0 * (op(A, ta) @ op(B, tb)) - 3.0 * C
+                O.dgemm(C, A, B, 2.0, -3.0, ta, tb, threads=3)
+                assert np.array_equal(C, expected)
+
+
+def test_gemm_no_read_rules(rng):
+    # beta == 0 never reads C (test/blas/gemm.jl:50-52)
+    A = rng.standard_normal((17, 13)); B = rng.standard_normal((13, 19))
+    C = np.asfortranarray(np.full((17, 19), np.nan))
+    O.dgemm(C, A, B, 1.0, 0.0)
+    np.testing.assert_allclose(C, A @ B, rtol=3e-12, atol=3e-12 * 13)
+    # alpha == 0 never reads A or B (:45-49)
+    for beta in (0.0, 1.0, 0.5):
+        C = np.asfortranarray(np.full((9, 7), 2.0))
+        O.dgemm(C, np.full((9, 3), np.nan), np.full((3, 7), np.nan), 0.0, beta)
+        assert np.array_equal(C, np.full((9, 7), 2.0 * beta))
+    # k == 0 scales C only, even with alpha NaN (:53-55)
+    C = np.asfortranarray(np.full((5, 7), 4.0))
+    O.dgemm(C, np.zeros((5, 0)), np.zeros((0, 7)), np.nan, 0.5)
+    assert np.array_equal(C, np.full((5, 7), 2.0))
+
+
+def test_gemm_threaded_equals_sequential(rng):
+    # beta in {0, 1}: the worker grid changes no element's arithmetic
+    A = rng.standard_normal((700, 900)); B = rng.standard_normal((900, 650))
+    for beta in (0.0, 1.0):
+        C0 = np.asfortranarray(rng.standard_normal((700, 650)))
+        C1 = C0.copy(order="F")
+        O.dgemm(C0, A, B, -1.0, beta, threads=1)
+        O.dgemm(C1, A, B, -1.0, beta, threads=4)
+        assert np.array_equal(C0, C1)
+
+
+def test_gemm_invalid_arguments():
+    C = np.asfortranarray(np.zeros((4, 4)))
+    with pytest.raises(ValueError):
+        O.dgemm(C, np.zeros((4, 4)), np.zeros((4, 4)), transA="X")
+
+
+# ---------------------------------------------------------------------------
+# triangular solves
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [0, 1, 17, 35, 65])  # test/blas/level3_blocked.jl:21-38
+def test_trsm_all_variants_vs_scipy(rng, n):
+    for side in "LR":
+        for uplo in "UL":
+            for trans in "NTC":
+                for unit in (False, True):
+                    A = rng.standard_normal((n, n)) / max(n, 1) + 2 * np.eye(n)  # well-conditioned, also unit
+                    B = np.asfortranarray(rng.standard_normal((n, 5) if side == "L" else (5, n)))
+                    X = B.copy(order="F")
+                    rc = O.dtrsm(X, A, 1.5, side, uplo, trans, unit)
+                    assert rc == 0
+                    T = np.triu(A) if uplo == "U" else np.tril(A)
+                    if unit:
+                        np.fill_diagonal(T, 1.0)
+                    opT = T if trans == "N" else T.T
+                    if side == "L":
+                        ref = 1.5 * np.linalg.solve(opT, B) if n else B
+                        back = opT @ X
+                    else:
+                        ref = 1.5 * np.linalg.solve(opT.T, B.T).T if n else B
+                        back = X @ opT
+                    if n:
+                        # normwise backward error of the solve, and the forward
+                        # error against numpy within the conditioning
+                        res = np.linalg.norm(back - 1.5 * B)
+                        assert res <= 10 * n * EPS * np.linalg.norm(T) * np.linalg.norm(X)
+                        np.testing.assert_allclose(X, ref, rtol=1e-9, atol=1e-10)
+
+
+def test_trsm_singular_reported_before_write(rng):
+    A = np.triu(rng.standard_normal((20, 20))) + 5 * np.eye(20)
+    A[7, 7] = 0.0
+    B = np.asfortranarray(rng.standard_normal((20, 3)))
+    old = B.copy(order="F")
+    rc = O.dtrsm(B, A, 1.0, "L", "U", "N", False)
+    assert rc == 8 and np.array_equal(B, old)   # SingularException(8), B untouched
+
+
+def test_trsv_vs_scipy(rng):
+    for n in (1, 3, 4, 5, 17, 64, 101):
+        for uplo in "UL":
+            for trans in "NT":
+                for unit in (False, True):
+                    A = rng.standard_normal((n, n)) + 2 * n * np.eye(n)
+                    x = rng.standard_normal(n)
+                    y = x.copy()
+                    assert O.dtrsv(y, A, uplo, trans, unit) == 0
+                    ref = sl.solve_triangular(A, x, lower=(uplo == "L"), trans=0 if trans == "N" else 1,
+                                              unit_diagonal=unit)
+                    np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# LU
+# ---------------------------------------------------------------------------
+LU_CASES = ((1, 1, 32), (5, 5, 1), (17, 17, 16), (33, 20, 16), (20, 33, 16), (64, 64, 32), (130, 97, 32),
+            (97, 130, 64), (257, 257, 32), (257, 257, 64))  # test/lapack/lapack_dense.jl:9
+
+
+def reconstruct(F, ipiv, m, n):
+    k = min(m, n)
+    L = np.tril(F[:, :k], -1) + np.eye(m, k)
+    U = np.triu(F[:k, :])
+    return L @ U, O.perm_from_ipiv(ipiv, m)
+
+
+@pytest.mark.parametrize("m,n,bs", LU_CASES)
+def test_getrf_matches_lapack_pivots_and_reconstructs(rng, m, n, bs):
+    A = rng.standard_normal((m, n))
+    F, ipiv, info = O.dgetrf(A, bs)
+    assert info == 0
+    LU, p = reconstruct(F, ipiv, m, n)
+    assert bwd(LU - A[p - 1], A) <= 30
+    # the same partial-pivot choices as LAPACK getrf (lapack_dense.jl:12-14)
+    _, lapack_piv = sl.lu_factor(A, check_finite=False)
+    assert np.array_equal(ipiv, lapack_piv[:min(m, n)] + 1)
+
+
+def test_getrf_blocked_shapes(rng):
+    # test/lapack/factorizations.jl:2-11
+    for (m, n) in ((0, 0), (1, 1), (9, 9), (13, 7), (7, 13), (67, 65), (65, 67)):
+        for block in (1, 7, 32):
+            A = rng.standard_normal((m, n))
+            F, ipiv, info = O.dgetrf(A, block)
+            if m and n:
+                LU, p = reconstruct(F, ipiv, m, n)
+                np.testing.assert_allclose(LU, A[p - 1], rtol=3e-11, atol=3e-12 * max(1, n))
+
+
+def test_getrf_singular_and_hard_inputs(rng):
+    A = rng.standard_normal((40, 40)); A[:, 22] = 0
+    F, ipiv, info = O.dgetrf(A, 128, check=False)
+    assert info == 23          # lapack_dense.jl:24-27
+    LU, p = reconstruct(F, ipiv, 40, 40)
+    assert bwd(LU - A[p - 1], A) <= 30
+    F, ipiv, info = O.dgetrf(np.zeros((3, 3)), check=False)
+    assert info == 1           # test/lapack/factorizations.jl:13-15
+    G = rng.standard_normal((80, 80)) @ np.diag(10.0 ** np.linspace(0, -12, 80))
+    F, ipiv, info = O.dgetrf(G)
+    LU, p = reconstruct(F, ipiv, 80, 80)
+    assert bwd(LU - G[p - 1], G) <= 30
+    N = rng.standard_normal((60, 60)); N[:, -1] = N[:, 0] + EPS * rng.standard_normal(60)
+    F, ipiv, info = O.dgetrf(N, check=False)
+    LU, p = reconstruct(F, ipiv, 60, 60)
+    assert bwd(LU - N[p - 1], N) <= 30
+
+
+def test_getrf_threaded_bitwise_equal(rng):
+    # lapack_dense.jl:33-38
+    A = rng.standard_normal((600, 600))
+    F1, p1, i1 = O.dgetrf(A, threads=1)
+    F2, p2, i2 = O.dgetrf(A, threads=4)
+    assert np.array_equal(F1, F2) and np.array_equal(p1, p2)
+    A[:, 299] = 0
+    F1, p1, i1 = O.dgetrf(A, check=False, threads=1)
+    F2, p2, i2 = O.dgetrf(A, check=False, threads=4)
+    assert i1 == i2 == 300 and np.array_equal(F1, F2)
+
+
+def test_getrs_solves(rng):
+    for n in (1, 9, 64, 129):
+        A = rng.standard_normal((n, n))
+        F, ipiv, info = O.dgetrf(A, 32)
+        c = np.linalg.cond(A)
+        b = rng.standard_normal(n); B = rng.standard_normal((n, 3))
+        assert bwd(A @ O.dgetrs(F, ipiv, b) - b, A) <= 30 * c
+        assert bwd(A @ O.dgetrs(F, ipiv, B) - B, A) <= 30 * c
+        assert bwd(A.T @ O.dgetrs(F, ipiv, b, "T") - b, A) <= 30 * c
+        assert bwd(A.T @ O.dgetrs(F, ipiv, B, "C") - B, A) <= 30 * c
+        # the matrix and vector forms agree (test/api/linalg_api_solve.jl:12-20)
+        np.testing.assert_allclose(O.dgetrs(F, ipiv, b), O.dgetrs(F, ipiv, b.reshape(-1, 1))[:, 0], rtol=1e-10,
+                                   atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# equilibration, residual, mixed solve
+# ---------------------------------------------------------------------------
+def test_equilibrate_radix_properties(rng):
+    A = rng.standard_normal((50, 40)) * np.logspace(-30, 30, 50)[:, None]
+    B, r, c = O.dequilibrate(A, radix=True)
+    # exact powers of two, and the scaling is exact
+    assert np.all(np.frexp(r)[0] == 0.5) and np.all(np.frexp(c)[0] == 0.5)
+    assert np.array_equal(B, A / r[:, None] / c[None, :])
+    rowscaled = A / r[:, None]
+    assert np.all(np.abs(rowscaled).max(1) < 2 ** 0.5 + 1e-12) and np.all(np.abs(rowscaled).max(1) >= 2 ** -0.5)
+    assert np.all(np.abs(B).max(0) < 2 ** 0.5 + 1e-12) and np.all(np.abs(B).max(0) >= 2 ** -0.5)
+    # a zero row keeps scale 1
+    Z = A.copy(); Z[3] = 0
+    _, r2, _ = O.dequilibrate(Z)
+    assert r2[3] == 1.0
+
+
+def test_refinement_residual(rng):
+    A = rng.standard_normal((30, 30)); X = rng.standard_normal((30, 2)); B = rng.standard_normal((30, 2))
+    R, e = O.refinement_residual(A, X, B)
+    np.testing.assert_allclose(R, B - A @ X, rtol=1e-12, atol=1e-12)
+    den = np.abs(A) @ np.abs(X) + np.abs(B)
+    assert e == pytest.approx(np.max(np.abs(R) / den), rel=1e-12)
+    # an exact solution of an integer system has zero residual and zero error
+    Ai = np.diag(np.arange(1.0, 6.0)); xi = np.arange(1.0, 6.0); bi = Ai @ xi
+    R, e = O.refinement_residual(Ai, xi, bi)
+    assert np.all(R == 0) and e == 0.0
+
+
+def test_mixed_solve_contract(rng):
+    # test/lapack/factorizations.jl:57-63
+    for n in (1, 17, 67):
+        A = rng.standard_normal((n, n)) + n * np.eye(n); b = rng.standard_normal(n)
+        r = O.mixed_solve(A, b)
+        assert r["converged"] and r["iterations"] <= 10 and not r["fallback"]
+        np.testing.assert_allclose(A @ r["x"], b, rtol=2e-12, atol=2e-12)
+    # singular after rounding to Float32: falls back to the working precision
+    A = np.array([[1.0, 1.0], [1.0, 1.0 + 1e-12]]); b = np.array([1.0, 2.0])
+    r = O.mixed_solve(A, b)
+    assert r["fallback"] and r["reason"] == "working_factor_fallback"
+    np.testing.assert_allclose(A @ r["x"], b, rtol=1e-9)
