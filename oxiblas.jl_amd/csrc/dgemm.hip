@@ -8,8 +8,8 @@
 // while the matrix cores consume the current one.
 //
 // Work decomposition: a 256-thread workgroup owns a 128x128 tile of C; its
-// four waves each own 64x64 = 4x4 blocks of 16x16 (64 FP64 accumulators per
-// lane), each block computed as four v_mfma_f64_4x4x4_4b with A broadcast.  The epilogue applies alpha and beta as the
+// four waves each own 64x64 = 4x4 blocks of v_mfma_f64_16x16x4 (64 FP64
+// accumulators per lane).  The epilogue applies alpha and beta as the
 // reference's full tiles do: v = alpha*acc; C = beta == 0 ? v : fma(beta, C, v)
 // (src/blas/simd.jl:190-196); beta == 0 never reads C.
 #include "oxi_internal.hpp"
@@ -124,20 +124,11 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
             for (int i = 0; i < 4; ++i) a[i] = as[kr * LDS_A + wm * 64 + i * 16 + lcol];
 #pragma unroll
             for (int j = 0; j < 4; ++j) b[j] = bs[kr * LDS_B + wn * 64 + j * 16 + lcol];
-            // v_mfma_f64_4x4x4_4b with A broadcast (CBSZ=2, ABID=r): the four
-            // 4x4 blocks share the A block held by lane group r, so one
-            // instruction computes rows 4r..4r+3 of the 16x16 block for k=4
-            // with the same A/B register layout as v_mfma_f64_16x16x4, at the
-            // full FP64 matrix rate on gfx950.
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    acc[i][j][0] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[i], b[j], acc[i][j][0], 2, 0, 0);
-                    acc[i][j][1] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[i], b[j], acc[i][j][1], 2, 1, 0);
-                    acc[i][j][2] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[i], b[j], acc[i][j][2], 2, 2, 0);
-                    acc[i][j][3] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[i], b[j], acc[i][j][3], 2, 3, 0);
-                }
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         if (s + 1 < nslab) {
             store_slab<A_OC, LDS_A>(As[cur ^ 1], ra, t);
@@ -146,9 +137,8 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
         __syncthreads();
     }
 
-    // Accumulator layout (measured on gfx950, the same for v_mfma_f64_16x16x4
-    // and for the four broadcast 4x4x4_4b): lane l, item r holds row
-    // 4*r + l/16, column l%16 of the 16x16 block.
+    // Accumulator layout of v_mfma_f64_16x16x4 (measured on gfx950): lane l,
+    // item r holds row 4*r + l/16, column l%16 of the 16x16 block.
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll

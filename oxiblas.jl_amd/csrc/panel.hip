@@ -105,15 +105,15 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
             if (lr[s] == lrow && lrow < a.mrows) {
                 rec[0] = lkey; rec[1] = __longlong_as_double(lrow); rec[2] = v[s][j];
 #pragma unroll
-                for (int c = 0; c < OXI_LEAF_W; ++c) if (c < w) rec[3 + c] = v[s][c];
+                for (int c = 0; c < OXI_LEAF_W; ++c) rec[3 + c] = v[s][c];
             }
             if (p == 0 && lr[s] == j) {
                 if (global_x) {
 #pragma unroll
-                    for (int c = 0; c < OXI_LEAF_W; ++c) if (c < w) drow[c] = v[s][c];
+                    for (int c = 0; c < OXI_LEAF_W; ++c) drow[c] = v[s][c];
                 } else {
 #pragma unroll
-                    for (int c = 0; c < OXI_LEAF_W; ++c) if (c < w) dg_sh[c] = v[s][c];
+                    for (int c = 0; c < OXI_LEAF_W; ++c) dg_sh[c] = v[s][c];
                 }
             }
         }
@@ -153,12 +153,12 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
                 }
                 __syncthreads();
                 const double* rw = a.pub + (size_t(j) * a.P + win_wg) * OXI_PUB_STRIDE;
-                if (t < w) { u_sh[t] = rw[3 + t]; dg_sh[t] = drow[t]; }
+                if (t < OXI_LEAF_W) { u_sh[t] = rw[3 + t]; dg_sh[t] = drow[t]; }
             }
             __syncthreads();
         } else {
             if (t == 0) win_row = lrow;
-            if (t < w) u_sh[t] = lds_rec[3 + t];
+            if (t < OXI_LEAF_W) u_sh[t] = lds_rec[3 + t];
             __syncthreads();
         }
         if (!dead) {
@@ -170,10 +170,10 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
                 for (int s = 0; s < RPT; ++s) {
                     if (lr[s] == j) {
 #pragma unroll
-                        for (int c = 0; c < OXI_LEAF_W; ++c) if (c < w) v[s][c] = u_sh[c];
+                        for (int c = 0; c < OXI_LEAF_W; ++c) v[s][c] = u_sh[c];
                     } else if (lr[s] == piv) {
 #pragma unroll
-                        for (int c = 0; c < OXI_LEAF_W; ++c) if (c < w) v[s][c] = dg_sh[c];
+                        for (int c = 0; c < OXI_LEAF_W; ++c) v[s][c] = dg_sh[c];
                     }
                 }
             }
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
                         v[s][j] = l;
 #pragma unroll
                         for (int c = j + 1; c < OXI_LEAF_W; ++c)
-                            if (c < w) v[s][c] = __builtin_fma(-l, u_sh[c], v[s][c]);
+                            v[s][c] = __builtin_fma(-l, u_sh[c], v[s][c]);
                     }
                 }
             }
