@@ -7,10 +7,14 @@ resident in HBM.  value = (2/3 n^3 + 2 n^2) / seconds per step, in GFLOP/s.
 
 Also reported (extra keys): the trailing-update GEMM roofline measured with
 HIP events on the stream it runs on, the standalone GEMM shapes, the 64-RHS
-solve, the oracle's (and scipy-OpenBLAS's) LU-solve on the host cores, and a
-parity check of the timed factorization.
+solve, the mixed-precision LU + refinement against the FP64 LU-solve
+(A = U diag(sigma) V^T with Haar U, V, generated on the GPU and not timed;
+speedup = FP64 LU-solve time / mixed solve time, both reaching backward error
+<= 16 eps), the oracle's (and scipy-OpenBLAS's) LU-solve on the host cores, and
+a parity check of the timed factorization.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 16384] [--nb 0]
+                  [--no-cpu] [--no-extra] [--no-mixed] [--kappa 1e6]
 """
 import argparse
 import json
@@ -46,6 +50,54 @@ def gemm_standalone(ox, torch, m, n, k, reps=5):
     ms = e0.elapsed_time(e1) / reps
     del A, B, C
     return 2.0 * m * n * k / (ms * 1e-3) / 1e12, ms
+
+
+def haar_conditioned(torch, n, kappa, seed):
+    """A = U diag(sigma) V^T, sigma_i = kappa^(-(i-1)/(n-1)); U, V the Q factors
+    of Gaussian matrices with the sign of diag(R) fixed (test data only)."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+
+    def q():
+        Q, R = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g))
+        return Q * torch.sign(torch.diagonal(R))
+
+    s = kappa ** (-torch.arange(n, dtype=torch.float64, device="cuda") / max(n - 1, 1))
+    U = q()
+    A = (U * s) @ q().T
+    del U
+    return A
+
+
+def mixed_extra(ox, torch, n, kappa, t_fp64, seed=0x0B1A5):
+    """Mixed LU + refinement on a kappa-conditioned system, per low type and
+    solver: time, speedup over the FP64 LU-solve, iterations, errors."""
+    At = haar_conditioned(torch, n, kappa, seed)
+    g = torch.Generator(device="cuda").manual_seed(seed + 1)
+    xt = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    b = At @ xt
+    A = ox.DMatrix(At.T.contiguous().reshape(-1), n, n)   # column-major A
+    bd = ox.DMatrix(b.clone(), n, 1)
+    out = {}
+    for low, solver in (("float32", "ir"), ("float16", "gmres"), ("bfloat16", "gmres")):
+        r = ox.mixed_solve(A, bd, lowtype=low, solver=solver, maxiter=50, gmres_restart=50)   # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = ox.mixed_solve(A, bd, lowtype=low, solver=solver, maxiter=50, gmres_restart=50)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        x = r.x.buf[:n]
+        fwd = (torch.linalg.norm(x - xt) / torch.linalg.norm(xt)).item()
+        key = f"mixed_{low}_{solver}"
+        out[key] = {"ms": round(t * 1e3, 2), "speedup_vs_fp64_lu_solve": round(t_fp64 / t, 3),
+                    "converged": r.converged, "backward_error": r.backward_error,
+                    "backward_error_over_n": r.backward_error / n, "forward_error": fwd,
+                    "iterations": r.iterations, "gmres_iterations": r.inner_iterations, "fallback": r.fallback,
+                    "factor_ms": round(r.factor_ms, 2), "refine_ms": round(r.refine_ms, 2)}
+        log(f"mixed {low}/{solver} n={n} kappa={kappa:g}: {t * 1e3:.1f} ms ({t_fp64 / t:.2f}x), "
+            f"berr {r.backward_error:.2e}, fwd {fwd:.2e}, it {r.iterations}/{r.inner_iterations}, "
+            f"factor {r.factor_ms:.1f} ms refine {r.refine_ms:.1f} ms")
+    del A, bd, At, b
+    return out
 
 
 def cpu_baseline(n=4096, reps=3):
@@ -90,6 +142,8 @@ def main():
     ap.add_argument("--nb", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--kappa", type=float, default=1e6)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -161,6 +215,10 @@ def main():
         extra["getrs_64rhs_gflops"] = round(2.0 * n * n * 64 / ts / 1e9, 1)
         del F, B64
 
+    mixed = None
+    if not args.no_mixed:
+        mixed = mixed_extra(ox, torch, n, args.kappa, dt)
+
     cpu = None
     cpu_ob = None
     if not args.no_cpu:
@@ -192,6 +250,10 @@ def main():
         "cpu_openblas": cpu_ob,
         "parity": {"backward_error_over_n_eps": berr, "bound": 30, "ok": bool(berr <= 30)},
     }
+    if mixed:
+        best = max(mixed.values(), key=lambda v: v["speedup_vs_fp64_lu_solve"] if v["converged"] else 0)
+        out["mixed"] = {"kappa": args.kappa, "n": n, "best_speedup_vs_fp64_lu_solve":
+                        best["speedup_vs_fp64_lu_solve"] if best["converged"] else None, "runs": mixed}
     out.update(extra)
     print(json.dumps(out), flush=True)
 

@@ -105,6 +105,88 @@ int oxi_dgetrf(oxi_handle_t handle, int64_t m, int64_t n, double* A, int64_t lda
 int oxi_dgetrs(oxi_handle_t handle, char trans, int64_t n, int64_t nrhs, const double* LU, int64_t lda,
                const int64_t* ipiv, double* B, int64_t ldb);
 
+/* ---- adjoints ------------------------------------------------------------ */
+/* Pullback of X = A \ B given the factors of oxi_dgetrf and the cotangent
+ * Xbar: Bbar = A^-T Xbar (the transposed solve ldiv!(transpose(F), .),
+ * src/api/linalg_api.jl:176-184, src/lapack/expert_refinement.jl:2-23) and,
+ * when Abar is not NULL, Abar = -Bbar X^T (FP64 MFMA, rank nrhs).  The
+ * reference has no AD rules; this is the ChainRules rrule of `\` built from
+ * its transposed solve and gemm!.  The gemm! pullback (Abar = Cbar op(B)^T,
+ * Bbar = op(A)^T Cbar) is two oxi_dgemm calls.  A zero on U's diagonal returns
+ * OXI_ERR_SINGULAR. */
+int oxi_dgesv_adjoint(oxi_handle_t handle, int64_t n, int64_t nrhs, const double* LU, int64_t lda,
+                      const int64_t* ipiv, const double* X, int64_t ldx, const double* Xbar, int64_t ldxbar,
+                      double* Bbar, int64_t ldbbar, double* Abar, int64_t ldabar);
+
+/* ---- mixed precision ---------------------------------------------------- */
+/* Precision of the trailing products of the FP32 factorization. */
+enum { OXI_PREC_BF16 = 0, OXI_PREC_FP16 = 1, OXI_PREC_FP8 = 2, OXI_PREC_FP32 = 3 };
+
+/* gemm! for Float32 (src/blas/gemm.jl:625-703), FP32 MFMA. */
+int oxi_sgemm(oxi_handle_t handle, char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha,
+              const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc);
+
+/* C (FP32) <- C + alpha * round(A) * round(B): the low-precision trailing
+ * product on its own; A (m x k) and B (k x n) are FP32 and rounded to `prec`
+ * (FP8 e4m3 with one power-of-two scale per 256-row block of A and per
+ * 256-column block of B), products accumulated in FP32.  No reference
+ * counterpart: the building block of the mixed form below. */
+int oxi_gemm_lowp_update(oxi_handle_t handle, int prec, int64_t m, int64_t n, int64_t k, float alpha,
+                         const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc);
+
+/* FP32 blocked LU with partial pivoting whose trailing updates run in `prec`
+ * (the low-precision factor of mixed_solve, src/lapack/extended_lapack.jl:183-198,
+ * with BF16/FP16/FP8 trailing products on top of the reference's Float32
+ * factor).  Same conventions as oxi_dgetrf. */
+int oxi_sgetrf_lowp(oxi_handle_t handle, int prec, int64_t m, int64_t n, float* A, int64_t lda, int64_t* ipiv,
+                    int64_t nb, int64_t* info);
+
+/* FP32 solve with the factors of oxi_sgetrf_lowp (as oxi_dgetrs). */
+int oxi_sgetrs(oxi_handle_t handle, char trans, int64_t n, int64_t nrhs, const float* LU, int64_t lda,
+               const int64_t* ipiv, float* B, int64_t ldb);
+
+typedef struct {
+    int64_t iterations;          /* refinement steps (residual evaluations - 1) */
+    int64_t inner_iterations;    /* GMRES iterations over all refinement steps */
+    int32_t converged;           /* backward_error <= rtol */
+    int32_t fallback;            /* 1: the low-precision factor failed, FP64 factor used */
+    int32_t reason;              /* 0 converged, 1 iteration limit, 2 non-finite, 3 working-precision fallback */
+    int32_t solver;              /* 0 IR, 1 GMRES-IR (the one used) */
+    double backward_error;       /* componentwise max_i |r_i| / (|A||x| + |b|)_i, as _refinement_residual! */
+    double normwise_backward_error; /* max|r| / (||A||_inf max|x| + max|b|) over all columns */
+    double residual_norm;        /* nrm2 of the last residual B - A X (the reference's `residual`) */
+    double factor_ms;            /* factorization time (device) */
+    double refine_ms;            /* first solve + refinement time (device) */
+} oxi_ir_info;
+
+/* mixed_solve(A, B; maxiter, rtol)  (src/lapack/factorizations.jl:419-427,
+ * src/lapack/extended_lapack.jl:114-226): radix equilibration, a low-precision
+ * LU (oxi_sgetrf_lowp with `prec`), and refinement in FP64 on the GPU until the
+ * componentwise backward error is <= rtol: solver 0 = classical iterative
+ * refinement (as the reference), 1 = GMRES-IR (left-preconditioned GMRES on
+ * the correction equation, one cycle of at most gmres_restart iterations per
+ * refinement step and right-hand side, the preconditioner being the low
+ * factors applied in FP64 arithmetic; for large condition numbers),
+ * 2 = automatic (GMRES-IR once classical refinement stops halving the error);
+ * gmres_restart 0 picks 30.  The residual,
+ * the correction accumulation and X stay FP64.  A (n x n) is not modified.
+ * A factorization that fails in low precision falls back to FP64
+ * (extended_lapack.jl:195-211).  Returns OXI_ERR_NOT_FINITE for non-finite
+ * input (:182). */
+int oxi_dgesv_mixed(oxi_handle_t handle, int prec, int solver, int64_t n, int64_t nrhs, const double* A,
+                    int64_t lda, const double* B, int64_t ldb, double* X, int64_t ldx, int32_t maxiter, double rtol,
+                    int32_t gmres_restart, oxi_ir_info* info);
+
+/* equilibrate(A; radix)  (src/lapack/dense_utilities.jl:191-212): row scales
+ * rows[i] = max_j |A[i,j]| (1 for a zero row), then column scales of A ./ rows,
+ * and E = (A ./ rows) ./ cols (E may be A).  radix != 0 rounds each scale to
+ * the power of two 2^round(log2(v)) (ties to even), clamped to the normal
+ * range, so that the scaling is exact.  A non-finite entry returns
+ * OXI_ERR_NOT_FINITE before E is written.  rows, cols: device arrays of m and
+ * n doubles. */
+int oxi_dequilibrate(oxi_handle_t handle, int64_t m, int64_t n, const double* A, int64_t lda, double* E,
+                     int64_t lde, double* rows, double* cols, int radix);
+
 /* ---- profiling ---------------------------------------------------------- */
 /* When enabled, oxi_dgetrf brackets every trailing-update GEMM launch with
  * HIP events on the stream it runs on and accumulates its time and flops. */

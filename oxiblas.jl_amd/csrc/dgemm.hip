@@ -8,13 +8,12 @@
 // while the matrix cores consume the current one.
 //
 // Work decomposition: a 256-thread workgroup owns a 128x128 tile of C; its
-// four waves each own 64x64 = 4x4 blocks of v_mfma_f64_16x16x4 (64 FP64
+// four waves each own 64x64, computed with v_mfma_f64_4x4x4_4b (the FP64
+// matrix instruction that reaches the full rate on gfx950; 64 FP64
 // accumulators per lane).  The epilogue applies alpha and beta as the
 // reference's full tiles do: v = alpha*acc; C = beta == 0 ? v : fma(beta, C, v)
 // (src/blas/simd.jl:190-196); beta == 0 never reads C.
 #include "oxi_internal.hpp"
-
-typedef double d4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
@@ -93,11 +92,18 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
     // op(A) rows contiguous when A is 'N'; op(B) columns contiguous when B is 'T'
     constexpr bool A_OC = !TA, B_OC = TB;
 
-    d4 acc[4][4];
+    // v_mfma_f64_4x4x4_4b runs four independent 4x4x4 products, one per lane
+    // group g = (lane % 16) / 4 (measured on gfx950): A_g[i][k] at lane
+    // 16k + 4g + i, B_g[k][j] at lane 16k + 4g + j, D_g[i][j] at lane
+    // 16i + 4g + j.  Each lane group owns a 32x32 sub-tile of the wave's
+    // 64x64 tile: rows 8q + 4(g&1) + i and columns 8p + 4(g>>1) + j
+    // (q, p = 0..7), interleaved by 4 so that the groups' LDS reads of one k
+    // row fall on consecutive addresses.
+    double acc[8][8];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 8; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < 8; ++j) acc[i][j] = 0.0;
 
     double ra[LOADS], rb[LOADS];
     const int nslab = int((K + BK - 1) / BK);
@@ -107,7 +113,9 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
     store_slab<B_OC, LDS_B>(Bs[0], rb, t);
     __syncthreads();
 
-    const int lrow = lane >> 4, lcol = lane & 15;
+    const int kq = lane >> 4, grp = (lane >> 2) & 3, li = lane & 3;
+    const int roff = wm * 64 + 4 * (grp & 1) + li;   // + 8q
+    const int coff = wn * 64 + 4 * (grp >> 1) + li;  // + 8p
     for (int s = 0; s < nslab; ++s) {
         const int cur = s & 1;
         if (s + 1 < nslab) {
@@ -118,17 +126,17 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
         const double* bs = Bs[cur];
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
-            const int kr = ks * 4 + lrow;
-            double a[4], b[4];
+            const int kr = ks * 4 + kq;
+            double a[8], b[8];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = as[kr * LDS_A + wm * 64 + i * 16 + lcol];
+            for (int q = 0; q < 8; ++q) a[q] = as[kr * LDS_A + roff + 8 * q];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = bs[kr * LDS_B + wn * 64 + j * 16 + lcol];
+            for (int p = 0; p < 8; ++p) b[p] = bs[kr * LDS_B + coff + 8 * p];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int q = 0; q < 8; ++q)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+                for (int p = 0; p < 8; ++p)
+                    acc[q][p] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[q], b[p], acc[q][p], 0, 0, 0);
         }
         if (s + 1 < nslab) {
             store_slab<A_OC, LDS_A>(As[cur ^ 1], ra, t);
@@ -137,24 +145,23 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
         __syncthreads();
     }
 
-    // Accumulator layout of v_mfma_f64_16x16x4 (measured on gfx950): lane l,
-    // item r holds row 4*r + l/16, column l%16 of the 16x16 block.
+    // D_g[i][j] at lane 16i + 4g + j: row 8q + 4(g&1) + i, column 8p + 4(g>>1) + j
+    const int erow = wm * 64 + 4 * (grp & 1) + kq;
+    const int ecol = wn * 64 + 4 * (grp >> 1) + li;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int p = 0; p < 8; ++p) {
+        const i64 col = n0 + ecol + 8 * p;
+        if (col >= N) continue;
+        double* cc = C + col * ldc;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const i64 col = n0 + wn * 64 + j * 16 + lcol;
-            if (col >= N) continue;
-            double* cc = C + col * ldc;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const i64 row = m0 + wm * 64 + i * 16 + r * 4 + lrow;
-                if (row < M) {
-                    const double v = alpha * acc[i][j][r];
-                    cc[row] = BETA0 ? v : __builtin_fma(beta, cc[row], v);
-                }
+        for (int q = 0; q < 8; ++q) {
+            const i64 row = m0 + erow + 8 * q;
+            if (row < M) {
+                const double v = alpha * acc[q][p];
+                cc[row] = BETA0 ? v : __builtin_fma(beta, cc[row], v);
             }
         }
+    }
 }
 
 __global__ void dscale_kernel(i64 m, i64 n, double beta, double* __restrict__ C, i64 ldc) {

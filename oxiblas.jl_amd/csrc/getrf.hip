@@ -42,9 +42,10 @@ namespace {
 constexpr int TT = 32;  // transpose tile
 
 // out (n x m, ldo) <- in^T, in is m x n (ldi)
-__global__ __launch_bounds__(256) void transpose_kernel(const double* __restrict__ in, i64 ldi, double* __restrict__ out,
+template <typename T>
+__global__ __launch_bounds__(256) void transpose_kernel(const T* __restrict__ in, i64 ldi, T* __restrict__ out,
                                                         i64 ldo, i64 m, i64 n) {
-    __shared__ double tile[TT][TT + 1];
+    __shared__ T tile[TT][TT + 1];
     const i64 i0 = i64(blockIdx.x) * TT, j0 = i64(blockIdx.y) * TT;
     const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;  // 32 x 8
     for (int y = ty; y < TT; y += 8) {
@@ -60,8 +61,9 @@ __global__ __launch_bounds__(256) void transpose_kernel(const double* __restrict
 
 // In-place transpose of the leading n x n block: tile (bi, bj) with bi < bj is
 // swapped with tile (bj, bi); diagonal tiles are transposed in LDS.
-__global__ __launch_bounds__(256) void transpose_square_kernel(double* __restrict__ A, i64 lda, i64 n, int tiles) {
-    __shared__ double t1[TT][TT + 1], t2[TT][TT + 1];
+template <typename T>
+__global__ __launch_bounds__(256) void transpose_square_kernel(T* __restrict__ A, i64 lda, i64 n, int tiles) {
+    __shared__ T t1[TT][TT + 1], t2[TT][TT + 1];
     // linear block id -> (bi, bj) with bi <= bj
     i64 id = blockIdx.x;
     int bi = 0;
@@ -148,9 +150,11 @@ __global__ __launch_bounds__(256) void perm_pairs_kernel(const i64* __restrict__
 // [c0, c1) (W column = A row, W row = A column).  A workgroup stages 16 W rows
 // x all pairs through LDS so the reads complete before any write.
 constexpr int PAIR_ROWS = 16;
-__global__ __launch_bounds__(256) void apply_pairs_w_kernel(double* __restrict__ Wm, i64 ldw, i64 c0, i64 c1,
+template <typename T>
+__global__ __launch_bounds__(256) void apply_pairs_w_kernel(T* __restrict__ Wm, i64 ldw, i64 c0, i64 c1,
                                                             const int* __restrict__ pairs, i64 roff) {
-    extern __shared__ double buf[];  // [npairs][PAIR_ROWS]
+    extern __shared__ __attribute__((aligned(16))) unsigned char buf_raw[];  // [npairs][PAIR_ROWS]
+    T* buf = (T*)buf_raw;
     const int np = pairs[0];
     if (np == 0) return;
     const i64 cbase = c0 + i64(blockIdx.x) * PAIR_ROWS;
@@ -160,7 +164,7 @@ __global__ __launch_bounds__(256) void apply_pairs_w_kernel(double* __restrict__
     const bool ok = c < c1;
     for (int q = q0; q < np; q += qs) {
         const i64 src = pairs[2 + 2 * q] + roff;
-        buf[q * PAIR_ROWS + r] = ok ? Wm[c + src * ldw] : 0.0;
+        buf[q * PAIR_ROWS + r] = ok ? Wm[c + src * ldw] : T(0);
     }
     __syncthreads();
     if (ok)
@@ -197,22 +201,10 @@ __global__ void zero_i32_kernel(int* p, int n) {
 }  // namespace
 
 // ===========================================================================
-// Host-side building blocks
+// Host-side driver, templated on the working precision (FP64 for lu, FP32 for
+// the mixed-precision factorization)
 // ===========================================================================
 namespace {
-
-struct GetrfCtx {
-    oxi_handle_t h;
-    hipStream_t s;
-    double* Wm; i64 ldw;
-    i64 m, n;
-    i64* ipiv;
-    i64* dinfo;
-    unsigned* bar; unsigned bar_base;
-    double* pub;
-    int* pairs;
-    int* flags;
-};
 
 int launch_check(oxi_handle_t h, const char* what) {
     hipError_t e = hipGetLastError();
@@ -220,54 +212,93 @@ int launch_check(oxi_handle_t h, const char* what) {
     return OXI_OK;
 }
 
+template <typename T>
+struct GetrfCtx {
+    oxi_handle_t h;
+    hipStream_t s;
+    T* Wm; i64 ldw;
+    i64 m, n;
+    i64* ipiv;
+    i64* dinfo;
+    unsigned* bar; unsigned bar_base;
+    double* pub;
+    int* pairs;
+    int* flags;
+    int prec;          // FP32 only: -1 FP32 trailing update, 0 BF16, 1 FP16, 2 FP8 e4m3
+    void* lowp_ws; size_t lowp_bytes;
+};
+
+template <typename T>
+int gemm_nn(oxi_handle_t h, hipStream_t s, i64 m, i64 n, i64 k, T alpha, const T* A, i64 lda, const T* B, i64 ldb,
+            T beta, T* C, i64 ldc);
+template <>
+int gemm_nn<double>(oxi_handle_t h, hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda,
+                    const double* B, i64 ldb, double beta, double* C, i64 ldc) {
+    return oxi_launch_dgemm(h, s, 0, 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+template <>
+int gemm_nn<float>(oxi_handle_t h, hipStream_t s, i64 m, i64 n, i64 k, float alpha, const float* A, i64 lda,
+                   const float* B, i64 ldb, float beta, float* C, i64 ldc) {
+    return oxi_launch_sgemm(h, s, 0, 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+int trsm_ru(oxi_handle_t h, hipStream_t s, i64 rows, i64 n, const double* T, i64 ldt, double* X, i64 ldx) {
+    return oxi_dtrsm_right(h, s, 1, 0, 1, rows, n, T, ldt, X, ldx);
+}
+int trsm_ru(oxi_handle_t h, hipStream_t s, i64 rows, i64 n, const float* T, i64 ldt, float* X, i64 ldx) {
+    return oxi_strsm_right(h, s, 1, 0, 1, rows, n, T, ldt, X, ldx);
+}
+
 // Interchanges k1..k2 (0-based steps, inclusive; pivots are global 1-based rows)
 // applied to A columns [c0, c1) = W rows [c0, c1).
-int swap_w(GetrfCtx& g, i64 k1, i64 k2, i64 c0, i64 c1, int inverse) {
+template <typename T>
+int swap_w(GetrfCtx<T>& g, i64 k1, i64 k2, i64 c0, i64 c1, int inverse) {
     if (k2 < k1 || c1 <= c0) return OXI_OK;
     const int cnt = int(k2 - k1 + 1);
     if (cnt > PERM_MAX_STEPS) return oxi_set_error(g.h, OXI_ERR_UNSUPPORTED, "laswp block of %d steps", cnt);
     hipLaunchKernelGGL(perm_pairs_kernel, dim3(1), dim3(256), 0, g.s, g.ipiv, i64(0), k1, cnt, inverse, g.pairs);
     OXI_TRY(launch_check(g.h, "perm_pairs"));
-    const size_t shm = size_t(2 * cnt) * PAIR_ROWS * sizeof(double);
-    hipLaunchKernelGGL(apply_pairs_w_kernel, dim3(unsigned(cdiv(c1 - c0, PAIR_ROWS))), dim3(256), shm, g.s, g.Wm, g.ldw,
-                       c0, c1, g.pairs, i64(0));
+    const size_t shm = size_t(2 * cnt) * PAIR_ROWS * sizeof(T);
+    hipLaunchKernelGGL(apply_pairs_w_kernel<T>, dim3(unsigned(cdiv(c1 - c0, PAIR_ROWS))), dim3(256), shm, g.s, g.Wm,
+                       g.ldw, c0, c1, g.pairs, i64(0));
     return launch_check(g.h, "apply_pairs_w");
 }
 
-int leaf(GetrfCtx& g, i64 c0, int w) {
+template <typename T>
+int leaf(GetrfCtx<T>& g, i64 c0, int w) {
     const i64 mrows = g.m - c0;
-    // rows per workgroup: 256 (one per thread) up to 2048 rows, then 512
-    // (two per thread, the most that stays in registers without spilling)
-    const int kind = mrows > 2048 ? 1 : 0;
-    const i64 R = kind == 0 ? 256 : 512;
+    // rows per workgroup: 256 (one per thread) up to 2048 rows, then 512 (two
+    // per thread: the most FP64 rows that stay in registers without spilling);
+    // FP32 rows are half the size and go to 1024 per workgroup beyond 8192 rows
+    int kind = mrows > 2048 ? 1 : 0;
+    if (sizeof(T) == 4 && mrows > 8192) kind = 2;
+    const i64 R = kind == 0 ? 256 : kind == 1 ? 512 : 1024;
     const i64 P = cdiv(mrows, R);
     if (P > std::min(g.h->num_cus, 256))
         return oxi_set_error(g.h, OXI_ERR_UNSUPPORTED, "panel of %lld rows exceeds the leaf kernel capacity",
                              (long long)mrows);
-    LeafArgs a{g.Wm, g.ldw, c0, w, mrows, g.ipiv, g.dinfo, g.bar, g.bar_base, g.pub, int(P), g.flags};
-    OXI_TRY(oxi_launch_lu_leaf(g.h, g.s, a, kind));
+    LeafArgs a{(void*)g.Wm, g.ldw, c0, w, mrows, g.ipiv, g.dinfo, g.bar, g.bar_base, g.pub, int(P), g.flags};
+    if (sizeof(T) == 8) OXI_TRY(oxi_launch_lu_leaf(g.h, g.s, a, kind));
+    else OXI_TRY(oxi_launch_lu_leaf32(g.h, g.s, a, kind));
     if (P > 1) g.bar_base += unsigned(P) * unsigned(w);
     return OXI_OK;
 }
 
-}  // namespace
-
 // getrf2 recursion on the panel A[c0:m, c0:c0+np] in W coordinates
 // (src/lapack/factorizations.jl:114-152).
-static int panel(GetrfCtx& g, i64 c0, i64 np) {
+template <typename T>
+int panel(GetrfCtx<T>& g, i64 c0, i64 np) {
     if (np <= OXI_LEAF_W) return leaf(g, c0, int(np));
     const i64 mp = g.m - c0;
-    i64 h = np / 2;
+    const i64 h = np / 2;
     OXI_TRY(panel(g, c0, h));
     // left half's interchanges on the right half columns
     OXI_TRY(swap_w(g, c0, c0 + h - 1, c0 + h, c0 + np, 0));
     // A12 <- L11^-1 A12 : W[c0+h : c0+np, c0 : c0+h] <- W[..] * (L11^T)^-1, L11^T = W[c0:c0+h, c0:c0+h] upper unit
-    OXI_TRY(oxi_dtrsm_right(g.h, g.s, 1, 0, 1, np - h, h, g.Wm + c0 + c0 * g.ldw, g.ldw, g.Wm + (c0 + h) + c0 * g.ldw,
-                             g.ldw));
+    OXI_TRY(trsm_ru(g.h, g.s, np - h, h, g.Wm + c0 + c0 * g.ldw, g.ldw, g.Wm + (c0 + h) + c0 * g.ldw, g.ldw));
     if (mp > h) {
         // A22 -= A21 * A12  <=>  W[c0+h:c0+np, c0+h:m] -= W[c0+h:c0+np, c0:c0+h] * W[c0:c0+h, c0+h:m]
-        OXI_TRY(oxi_launch_dgemm(g.h, g.s, 0, 0, np - h, mp - h, h, -1.0, g.Wm + (c0 + h) + c0 * g.ldw, g.ldw,
-                                 g.Wm + c0 + (c0 + h) * g.ldw, g.ldw, 1.0, g.Wm + (c0 + h) + (c0 + h) * g.ldw, g.ldw));
+        OXI_TRY(gemm_nn<T>(g.h, g.s, np - h, mp - h, h, T(-1), g.Wm + (c0 + h) + c0 * g.ldw, g.ldw,
+                           g.Wm + c0 + (c0 + h) * g.ldw, g.ldw, T(1), g.Wm + (c0 + h) + (c0 + h) * g.ldw, g.ldw));
         OXI_TRY(panel(g, c0 + h, np - h));
         // right half's interchanges on the left half columns
         OXI_TRY(swap_w(g, c0 + h, c0 + std::min(mp, np) - 1, c0, c0 + h, 0));
@@ -275,11 +306,120 @@ static int panel(GetrfCtx& g, i64 c0, i64 np) {
     return OXI_OK;
 }
 
-static int choose_nb(i64 m, i64 n) {
+// Trailing update A22 -= L21*U12, i.e. W[j1:n, j1:m] -= W[j1:n, j0:j1] * W[j0:j1, j1:m].
+int trailing(GetrfCtx<double>& g, i64 j0, i64 j1) {
+    return oxi_launch_dgemm(g.h, g.s, 0, 0, g.n - j1, g.m - j1, j1 - j0, -1.0, g.Wm + j1 + j0 * g.ldw, g.ldw,
+                            g.Wm + j0 + j1 * g.ldw, g.ldw, 1.0, g.Wm + j1 + j1 * g.ldw, g.ldw);
+}
+int trailing(GetrfCtx<float>& g, i64 j0, i64 j1) {
+    if (g.prec < 0)
+        return oxi_launch_sgemm(g.h, g.s, 0, 0, g.n - j1, g.m - j1, j1 - j0, -1.0f, g.Wm + j1 + j0 * g.ldw, g.ldw,
+                                g.Wm + j0 + j1 * g.ldw, g.ldw, 1.0f, g.Wm + j1 + j1 * g.ldw, g.ldw);
+    return oxi_lowp_update(g.h, g.s, g.prec, g.n - j1, g.m - j1, j1 - j0, -1.0f, g.Wm + j1 + j0 * g.ldw, g.ldw,
+                           g.Wm + j0 + j1 * g.ldw, g.ldw, g.Wm + j1 + j1 * g.ldw, g.ldw, g.lowp_ws, g.lowp_bytes);
+}
+
+int choose_nb(i64 m, i64 n) {
     const i64 q = std::min(m, n);
     if (q >= 8192) return 256;
     if (q >= 1024) return 128;
     return 64;
+}
+
+template <typename T>
+int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64* info, int prec) {
+    (void)hipSetDevice(h->device);
+    hipStream_t s = h->stream;
+    const i64 q = std::min(m, n);
+    if (nb == 0) nb = choose_nb(m, n);
+    if (nb > PERM_MAX_STEPS) nb = PERM_MAX_STEPS;
+
+    // workspace: [W if rectangular][pub][pairs][bar][low-precision operands]
+    const bool square = (m == n);
+    const i64 ldw = square ? lda : n;
+    const size_t wbytes = square ? 0 : size_t(n) * size_t(m) * sizeof(T);
+    const size_t pub_doubles = size_t(OXI_LEAF_W) * (size_t(256) + 1) * OXI_PUB_STRIDE + size_t(OXI_LEAF_W) * OXI_PUB_STRIDE;
+    const size_t off_pub = (wbytes + 255) & ~size_t(255);
+    const size_t off_pairs = off_pub + ((pub_doubles * 8 + 255) & ~size_t(255));
+    const size_t off_bar = off_pairs + ((size_t(4 * PERM_MAX_STEPS + 4) * 4 + 255) & ~size_t(255));
+    const size_t off_lowp = off_bar + 256;
+    const size_t lowp_bytes = (sizeof(T) == 4 && prec >= 0) ? oxi_lowp_update_ws(prec, n, m, nb) : 0;
+    const size_t total = off_lowp + lowp_bytes;
+    void* ws;
+    OXI_TRY(oxi_workspace(h, total, &ws));
+    char* base = (char*)ws;
+    GetrfCtx<T> g;
+    g.h = h; g.s = s;
+    g.Wm = square ? A : (T*)base; g.ldw = ldw;
+    g.m = m; g.n = n; g.ipiv = ipiv; g.dinfo = h->dev_info;
+    g.pub = (double*)(base + off_pub);
+    g.pairs = (int*)(base + off_pairs);
+    g.bar = (unsigned*)(base + off_bar); g.bar_base = 0;
+    g.flags = h->dev_flags;
+    g.prec = prec;
+    g.lowp_ws = base + off_lowp; g.lowp_bytes = lowp_bytes;
+
+    static bool attr_set = false;
+    if (!attr_set) {
+        // the pair-staging kernel may use up to 2 * 512 rows * 16 doubles of LDS
+        OXI_HIP(h, hipFuncSetAttribute((const void*)apply_pairs_w_kernel<double>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        OXI_HIP(h, hipFuncSetAttribute((const void*)apply_pairs_w_kernel<float>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, g.bar, 64);
+    hipLaunchKernelGGL(zero_i64_kernel, dim3(1), dim3(64), 0, s, g.dinfo, 1);
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, s, g.flags, 1);
+    OXI_TRY(launch_check(h, "zero"));
+
+    // A -> W
+    const int tiles = int(cdiv(n, TT));
+    const i64 nblk = i64(tiles) * (tiles + 1) / 2;
+    if (square)
+        hipLaunchKernelGGL(transpose_square_kernel<T>, dim3(unsigned(nblk)), dim3(256), 0, s, A, lda, n, tiles);
+    else
+        hipLaunchKernelGGL(transpose_kernel<T>, dim3(unsigned(cdiv(m, TT)), unsigned(cdiv(n, TT))), dim3(256), 0, s, A,
+                           lda, g.Wm, ldw, m, n);
+    OXI_TRY(launch_check(h, "transpose"));
+
+    for (i64 j0 = 0; j0 < q; j0 += nb) {
+        const i64 jb = std::min<i64>(nb, q - j0);
+        const i64 j1 = j0 + jb;
+        OXI_TRY(panel(g, j0, jb));
+        OXI_TRY(swap_w(g, j0, j1 - 1, 0, j0, 0));
+        OXI_TRY(swap_w(g, j0, j1 - 1, j1, n, 0));
+        if (j1 >= n) continue;
+        // U12 <- L11^-1 A12 : W[j1:n, j0:j1] <- W[j1:n, j0:j1] * (W[j0:j1, j0:j1])^-1 (upper unit)
+        OXI_TRY(trsm_ru(h, s, n - j1, jb, g.Wm + j0 + j0 * ldw, ldw, g.Wm + j1 + j0 * ldw, ldw));
+        if (j1 >= m) continue;
+        hipEvent_t pe = nullptr;
+        OXI_TRY(oxi_prof_begin(h, s, &pe));
+        OXI_TRY(trailing(g, j0, j1));
+        OXI_TRY(oxi_prof_end(h, s, pe, 2.0 * double(n - j1) * double(m - j1) * double(jb)));
+    }
+
+    // W -> A
+    if (square)
+        hipLaunchKernelGGL(transpose_square_kernel<T>, dim3(unsigned(nblk)), dim3(256), 0, s, A, lda, n, tiles);
+    else
+        hipLaunchKernelGGL(transpose_kernel<T>, dim3(unsigned(cdiv(n, TT)), unsigned(cdiv(m, TT))), dim3(256), 0, s,
+                           g.Wm, ldw, A, lda, n, m);
+    OXI_TRY(launch_check(h, "transpose back"));
+    i64 dinfo = 0;
+    int flag = 0;
+    OXI_HIP(h, hipMemcpyAsync(&dinfo, g.dinfo, sizeof(i64), hipMemcpyDeviceToHost, s));
+    OXI_HIP(h, hipMemcpyAsync(&flag, g.flags, sizeof(int), hipMemcpyDeviceToHost, s));
+    OXI_HIP(h, hipStreamSynchronize(s));
+    if (flag) return oxi_set_error(h, OXI_ERR_TIMEOUT, "getrf: a panel barrier timed out");
+    *info = dinfo;
+    return OXI_OK;
+}
+
+}  // namespace
+
+int oxi_getrf32(oxi_handle_t h, i64 m, i64 n, float* A, i64 lda, i64* ipiv, i64 nb, i64* info, int prec) {
+    return getrf_t<float>(h, m, n, A, lda, ipiv, nb, info, prec);
 }
 
 extern "C" int oxi_dgetrf(oxi_handle_t h, int64_t m, int64_t n, double* A, int64_t lda, int64_t* ipiv, int64_t nb,
@@ -295,97 +435,26 @@ extern "C" int oxi_dgetrf(oxi_handle_t h, int64_t m, int64_t n, double* A, int64
     if (m == 0 || n == 0) return OXI_OK;
     if (!A) return -4;
     if (!ipiv) return -6;
-    (void)hipSetDevice(h->device);
-    hipStream_t s = h->stream;
-    const i64 q = std::min(m, n);
-    if (nb == 0) nb = choose_nb(m, n);
-    if (nb > PERM_MAX_STEPS) nb = PERM_MAX_STEPS;
-
-    // workspace: [W if rectangular][pub][pairs][bar]
-    const bool square = (m == n);
-    const i64 ldw = square ? lda : n;
-    const size_t wbytes = square ? 0 : size_t(n) * size_t(m) * 8;
-    const size_t pub_doubles = size_t(OXI_LEAF_W) * (size_t(256) + 1) * OXI_PUB_STRIDE + size_t(OXI_LEAF_W) * OXI_PUB_STRIDE;
-    const size_t off_pub = (wbytes + 255) & ~size_t(255);
-    const size_t off_pairs = off_pub + ((pub_doubles * 8 + 255) & ~size_t(255));
-    const size_t off_bar = off_pairs + ((size_t(4 * PERM_MAX_STEPS + 4) * 4 + 255) & ~size_t(255));
-    const size_t total = off_bar + 256;
-    void* ws;
-    OXI_TRY(oxi_workspace(h, total, &ws));
-    char* base = (char*)ws;
-    GetrfCtx g;
-    g.h = h; g.s = s;
-    g.Wm = square ? A : (double*)base; g.ldw = ldw;
-    g.m = m; g.n = n; g.ipiv = ipiv; g.dinfo = h->dev_info;
-    g.pub = (double*)(base + off_pub);
-    g.pairs = (int*)(base + off_pairs);
-    g.bar = (unsigned*)(base + off_bar); g.bar_base = 0;
-    g.flags = h->dev_flags;
-
-    static bool attr_set = false;
-    if (!attr_set) {
-        // the pair-staging kernel may use up to 2 * OXI max steps * 16 doubles of LDS
-        OXI_HIP(h, hipFuncSetAttribute((const void*)apply_pairs_w_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_set = true;
-    }
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, g.bar, 64);
-    hipLaunchKernelGGL(zero_i64_kernel, dim3(1), dim3(64), 0, s, g.dinfo, 1);
-    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, s, g.flags, 1);
-    OXI_TRY(launch_check(h, "zero"));
-
-    // A -> W
-    if (square) {
-        const int tiles = int(cdiv(n, TT));
-        const i64 nblk = i64(tiles) * (tiles + 1) / 2;
-        hipLaunchKernelGGL(transpose_square_kernel, dim3(unsigned(nblk)), dim3(256), 0, s, A, lda, n, tiles);
-    } else {
-        hipLaunchKernelGGL(transpose_kernel, dim3(unsigned(cdiv(m, TT)), unsigned(cdiv(n, TT))), dim3(256), 0, s, A,
-                           lda, g.Wm, ldw, m, n);
-    }
-    OXI_TRY(launch_check(h, "transpose"));
-
-    for (i64 j0 = 0; j0 < q; j0 += nb) {
-        const i64 jb = std::min<i64>(nb, q - j0);
-        const i64 j1 = j0 + jb;
-        OXI_TRY(panel(g, j0, jb));
-        OXI_TRY(swap_w(g, j0, j1 - 1, 0, j0, 0));
-        OXI_TRY(swap_w(g, j0, j1 - 1, j1, n, 0));
-        if (j1 >= n) continue;
-        // U12 <- L11^-1 A12 : W[j1:n, j0:j1] <- W[j1:n, j0:j1] * (W[j0:j1, j0:j1])^-1 (upper unit)
-        OXI_TRY(oxi_dtrsm_right(h, s, 1, 0, 1, n - j1, jb, g.Wm + j0 + j0 * ldw, ldw, g.Wm + j1 + j0 * ldw, ldw));
-        if (j1 >= m) continue;
-        // A22 -= L21 U12 : W[j1:n, j1:m] -= W[j1:n, j0:j1] * W[j0:j1, j1:m]
-        hipEvent_t pe = nullptr;
-        OXI_TRY(oxi_prof_begin(h, s, &pe));
-        OXI_TRY(oxi_launch_dgemm(h, s, 0, 0, n - j1, m - j1, jb, -1.0, g.Wm + j1 + j0 * ldw, ldw, g.Wm + j0 + j1 * ldw,
-                                 ldw, 1.0, g.Wm + j1 + j1 * ldw, ldw));
-        OXI_TRY(oxi_prof_end(h, s, pe, 2.0 * double(n - j1) * double(m - j1) * double(jb)));
-    }
-
-    // W -> A
-    if (square) {
-        const int tiles = int(cdiv(n, TT));
-        const i64 nblk = i64(tiles) * (tiles + 1) / 2;
-        hipLaunchKernelGGL(transpose_square_kernel, dim3(unsigned(nblk)), dim3(256), 0, s, A, lda, n, tiles);
-    } else {
-        hipLaunchKernelGGL(transpose_kernel, dim3(unsigned(cdiv(n, TT)), unsigned(cdiv(m, TT))), dim3(256), 0, s, g.Wm,
-                           ldw, A, lda, n, m);
-    }
-    OXI_TRY(launch_check(h, "transpose back"));
-    i64 dinfo = 0;
-    int flag = 0;
-    OXI_HIP(h, hipMemcpyAsync(&dinfo, g.dinfo, sizeof(i64), hipMemcpyDeviceToHost, s));
-    OXI_HIP(h, hipMemcpyAsync(&flag, g.flags, sizeof(int), hipMemcpyDeviceToHost, s));
-    OXI_HIP(h, hipStreamSynchronize(s));
-    if (flag) return oxi_set_error(h, OXI_ERR_TIMEOUT, "getrf: a panel barrier timed out");
-    *info = dinfo;
-    return OXI_OK;
+    return getrf_t<double>(h, m, n, A, lda, ipiv, nb, info, -1);
 }
 
-// ===========================================================================
-// laswp, trsm and getrs entry points (column-major operands)
-// ===========================================================================
+extern "C" int oxi_sgetrf_lowp(oxi_handle_t h, int prec, int64_t m, int64_t n, float* A, int64_t lda, int64_t* ipiv,
+                               int64_t nb, int64_t* info) {
+    if (!h) return -1;
+    if (prec < OXI_PREC_BF16 || prec > OXI_PREC_FP32) return -2;
+    if (m < 0) return -3;
+    if (n < 0) return -4;
+    if (lda < std::max<i64>(1, m)) return -6;
+    if (nb < 0) return -8;
+    if (!info) return -9;
+    std::lock_guard<std::recursive_mutex> lk(h->guard);
+    *info = 0;
+    if (m == 0 || n == 0) return OXI_OK;
+    if (!A) return -5;
+    if (!ipiv) return -7;
+    return getrf_t<float>(h, m, n, A, lda, ipiv, nb, info, prec == OXI_PREC_FP32 ? -1 : prec);
+}
+
 extern "C" int oxi_dlaswp(oxi_handle_t h, int64_t ncols, double* A, int64_t lda, int64_t k1, int64_t k2,
                           const int64_t* ipiv, int inverse) {
     if (!h) return -1;

@@ -33,6 +33,24 @@ int oxi_workspace(oxi_handle_t h, size_t bytes, void** out) {
     return OXI_OK;
 }
 
+int oxi_workspace2(oxi_handle_t h, size_t bytes, void** out) {
+    if (bytes <= h->ws2_bytes) { *out = h->ws2; return OXI_OK; }
+    size_t want = ((bytes + (size_t(64) << 20) - 1) >> 26) << 26;
+    if (h->ws2) {
+        OXI_HIP(h, hipStreamSynchronize(h->stream));
+        OXI_HIP(h, hipStreamSynchronize(h->side));
+        OXI_HIP(h, hipFree(h->ws2));
+        h->ws2 = nullptr; h->ws2_bytes = 0;
+    }
+    if (hipMalloc(&h->ws2, want) != hipSuccess) {
+        h->ws2 = nullptr;
+        return oxi_set_error(h, OXI_ERR_ALLOC, "workspace allocation of %zu bytes failed", want);
+    }
+    h->ws2_bytes = want;
+    *out = h->ws2;
+    return OXI_OK;
+}
+
 extern "C" {
 
 const char* oxi_version(void) { return "oxiblas_mi355x 0.1.0 (gfx950)"; }
@@ -72,6 +90,7 @@ int oxi_destroy(oxi_handle_t h) {
     if (h->side) (void)hipStreamSynchronize(h->side);
     (void)hipDeviceSynchronize();
     if (h->ws) (void)hipFree(h->ws);
+    if (h->ws2) (void)hipFree(h->ws2);
     if (h->dev_flags) (void)hipFree(h->dev_flags);
     if (h->dev_info) (void)hipFree(h->dev_info);
     if (h->ev_a) (void)hipEventDestroy(h->ev_a);

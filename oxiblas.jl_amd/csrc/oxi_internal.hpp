@@ -22,8 +22,10 @@ struct oxi_handle_s {
     hipStream_t stream = nullptr;       // caller's stream (null = default stream)
     hipStream_t side = nullptr;         // internal stream for look-ahead work
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
-    void* ws = nullptr;                 // device workspace
+    void* ws = nullptr;                 // device workspace (kernels' scratch)
     size_t ws_bytes = 0;
+    void* ws2 = nullptr;                // second workspace for drivers that call the others
+    size_t ws2_bytes = 0;
     int* dev_flags = nullptr;           // [0]: barrier timeout flag, [1..]: reserved
     i64* dev_info = nullptr;            // device-side info (first zero pivot)
     std::string last_error;
@@ -57,6 +59,9 @@ int oxi_set_error(oxi_handle_t h, int code, const char* fmt, ...);
 // Grow the handle workspace to at least `bytes` (synchronises the streams
 // before freeing the old block).
 int oxi_workspace(oxi_handle_t h, size_t bytes, void** out);
+// Second grow-only workspace, for drivers (mixed solve) whose buffers must
+// survive the calls they make into the routines that use the first one.
+int oxi_workspace2(oxi_handle_t h, size_t bytes, void** out);
 
 // ---------------------------------------------------------------------------
 // Internal launchers (stream-explicit, no argument validation).
@@ -71,9 +76,41 @@ int oxi_dtrsm_left(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit
                    const double* A, i64 lda, double* B, i64 ldb);
 int oxi_dtrsm_right(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 n,
                     const double* A, i64 lda, double* B, i64 ldb);
+int oxi_strsm_left(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 nrhs,
+                   const float* A, i64 lda, float* B, i64 ldb);
+int oxi_strsm_right(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 n,
+                    const float* A, i64 lda, float* B, i64 ldb);
+// FP32 GEMM (sgemm.hip)
+int oxi_launch_sgemm(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n, i64 k, float alpha,
+                     const float* A, i64 lda, const float* B, i64 ldb, float beta, float* C, i64 ldc);
+// Low-precision trailing update (lowp_gemm.hip): prec 0 BF16, 1 FP16, 2 FP8 e4m3
+int oxi_lowp_update(oxi_handle_t h, hipStream_t s, int prec, i64 M, i64 N, i64 K, float alpha, const float* XA,
+                    i64 ldxa, const float* XB, i64 ldxb, float* C, i64 ldc, void* ws, size_t ws_bytes);
+size_t oxi_lowp_update_ws(int prec, i64 M, i64 N, i64 K);
+// 1-based permutation from LAPACK interchanges (host, synchronises s)
+int oxi_perm_from_ipiv(oxi_handle_t h, hipStream_t s, const i64* ipiv, i64 n, std::vector<i64>& perm);
+// getrs with a device permutation and an n x nrhs workspace Y (no synchronisation)
+int oxi_getrs_perm_f64(oxi_handle_t h, hipStream_t s, int trans, i64 n, i64 nrhs, const double* LU, i64 lda,
+                       const i64* dperm, double* B, i64 ldb, double* Y, void* ctrl);
+int oxi_getrs_perm_f32(oxi_handle_t h, hipStream_t s, int trans, i64 n, i64 nrhs, const float* LU, i64 lda,
+                       const i64* dperm, float* B, i64 ldb, float* Y, void* ctrl);
+// FP32 getrf with a trailing update in prec (-1 FP32, 0 BF16, 1 FP16, 2 FP8)
+int oxi_getrf32(oxi_handle_t h, i64 m, i64 n, float* A, i64 lda, i64* ipiv, i64 nb, i64* info, int prec);
 
 // First zero on the diagonal of an n x n triangle (1-based, 0 if none);
 // synchronises the stream.
+// Single right-hand-side triangular solve against LU factors (trsv.hip).
+// which: 0 = L unit forward, 1 = U backward, 2 = U^T forward, 3 = L^T unit
+// backward.  ctrl is 16 bytes of device scratch; the kernel's bounded waits
+// record a timeout there, read by oxi_trsv_check (which synchronises).
+int oxi_trsv(oxi_handle_t h, hipStream_t s, int which, i64 n, const double* M, i64 ld, double* x, void* ctrl);
+int oxi_trsv(oxi_handle_t h, hipStream_t s, int which, i64 n, const float* M, i64 ld, float* x, void* ctrl);
+int oxi_trsv(oxi_handle_t h, hipStream_t s, int which, i64 n, const float* M, i64 ld, double* x, void* ctrl);
+int oxi_trsv_check(oxi_handle_t h, hipStream_t s, const void* ctrl);
+// LU solves with at most this many right-hand sides go column by column
+// through oxi_trsv (when the caller supplies ctrl), more through trsm.
+constexpr i64 OXI_TRSV_MAX_RHS = 4;
+
 int oxi_first_zero_diag(oxi_handle_t h, hipStream_t s, const double* T, i64 ldt, i64 n, i64* where);
 
 // ---------------------------------------------------------------------------
@@ -83,7 +120,7 @@ constexpr int OXI_LEAF_W = 32;      // maximum leaf width
 constexpr int OXI_PUB_STRIDE = 48;  // doubles per published (column, workgroup) record
 
 struct LeafArgs {
-    double* Wm; i64 ldw;
+    void* Wm; i64 ldw;   // double* or float*
     i64 c0;          // first column (A) of the leaf = its first diagonal row
     int w;           // leaf width <= LEAF_W
     i64 mrows;       // rows c0 .. c0+mrows-1 of A
@@ -98,6 +135,8 @@ struct LeafArgs {
 
 // kind 0: 256 threads x 1 row per workgroup; kind 1: 256 threads x 2 rows.
 int oxi_launch_lu_leaf(oxi_handle_t h, hipStream_t s, const LeafArgs& a, int kind);
+// FP32 leaf; kind 2: 256 threads x 4 rows.
+int oxi_launch_lu_leaf32(oxi_handle_t h, hipStream_t s, const LeafArgs& a, int kind);
 
 // Bracket one trailing-update GEMM launch with profiling events (no-op when
 // profiling is off); resolved by oxi_profile_read.

@@ -1,0 +1,14 @@
+// FP32 instances of the panel leaf (the working precision of the
+// mixed-precision LU, as the reference's mixed_solve factors in Float32:
+// src/lapack/extended_lapack.jl:183,195-198).  FP32 rows take half the
+// registers, so up to four rows per thread stay in registers.
+#include "panel_kernel.hpp"
+
+int oxi_launch_lu_leaf32(oxi_handle_t h, hipStream_t s, const LeafArgs& a, int kind) {
+    if (kind == 0) hipLaunchKernelGGL((lu_leaf_kernel<float, 256, 1>), dim3(unsigned(a.P)), dim3(256), 0, s, a);
+    else if (kind == 1) hipLaunchKernelGGL((lu_leaf_kernel<float, 256, 2>), dim3(unsigned(a.P)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((lu_leaf_kernel<float, 256, 4>), dim3(unsigned(a.P)), dim3(256), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "lu_leaf32 launch: %s", hipGetErrorString(e));
+    return OXI_OK;
+}

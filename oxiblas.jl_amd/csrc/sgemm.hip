@@ -1,0 +1,189 @@
+// FP32 GEMM on CDNA4 matrix cores (v_mfma_f32_16x16x4f32):
+// C <- alpha*op(A)*op(B) + beta*C.  Used by the FP32 working precision of the
+// mixed-precision LU (the panel's recursive updates and the FP32 solves), the
+// Float32 path of gemm! (src/blas/gemm.jl:625-703; GemmConfig kc=896 for
+// 4-byte types, :14).
+//
+// Same decomposition as the FP64 kernel: a 256-thread workgroup owns a
+// 128x128 tile, each wave 64x64 = 4x4 blocks of 16x16, BK=16 slabs staged in
+// LDS and double-buffered.  Measured lane layout of v_mfma_f32_16x16x4f32 on
+// gfx950: A[l%16][l/16] at lane l, B[l/16][l%16], D item r of lane l = row
+// 4*(l/16)+r, column l%16.
+#include "oxi_internal.hpp"
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
+constexpr int LDS_A = BM + 1, LDS_B = BN + 1;
+constexpr int LOADS = BM * BK / NT;
+
+template <bool OUTER_CONTIG>
+__device__ __forceinline__ void load_slab(float (&r)[LOADS], const float* __restrict__ X, i64 ldx, i64 o0, i64 k0,
+                                          i64 outer, i64 K, int t) {
+#pragma unroll
+    for (int i = 0; i < LOADS; ++i) {
+        int o, kk;
+        if (OUTER_CONTIG) { o = t % BM; kk = t / BM + i * (NT / BM); }
+        else { kk = t % BK; o = t / BK + i * (NT / BK); }
+        const i64 go = o0 + o, gk = k0 + kk;
+        r[i] = (go < outer && gk < K) ? (OUTER_CONTIG ? X[go + gk * ldx] : X[gk + go * ldx]) : 0.0f;
+    }
+}
+
+template <bool OUTER_CONTIG>
+__device__ __forceinline__ void store_slab(float* __restrict__ S, int lds, const float (&r)[LOADS], int t) {
+#pragma unroll
+    for (int i = 0; i < LOADS; ++i) {
+        int o, kk;
+        if (OUTER_CONTIG) { o = t % BM; kk = t / BM + i * (NT / BM); }
+        else { kk = t % BK; o = t / BK + i * (NT / BK); }
+        S[kk * lds + o] = r[i];
+    }
+}
+
+__device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int& tm, int& tn) {
+    const int total = tiles_m * tiles_n, nx = 8;
+    int logical = bid;
+    if (total >= nx) {
+        const int xcd = bid % nx, local = bid / nx, per = total / nx, rem = total % nx;
+        logical = xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+    }
+    const int G = 8, group_size = G * tiles_n, group = logical / group_size, first_m = group * G;
+    const int gm = min(tiles_m - first_m, G), in = logical % group_size;
+    tm = first_m + in % gm;
+    tn = in / gm;
+}
+
+template <bool TA, bool TB, bool BETA0>
+__global__ __launch_bounds__(NT) void sgemm_mfma_kernel(i64 M, i64 N, i64 K, float alpha, const float* __restrict__ A,
+                                                        i64 lda, const float* __restrict__ B, i64 ldb, float beta,
+                                                        float* __restrict__ C, i64 ldc, int tiles_m, int tiles_n) {
+    __shared__ float As[2][BK * LDS_A];
+    __shared__ float Bs[2][BK * LDS_B];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    int tm, tn;
+    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const i64 m0 = i64(tm) * BM, n0 = i64(tn) * BN;
+    constexpr bool A_OC = !TA, B_OC = TB;
+
+    f4v acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
+
+    float ra[LOADS], rb[LOADS];
+    const int nslab = int((K + BK - 1) / BK);
+    load_slab<A_OC>(ra, A, lda, m0, 0, M, K, t);
+    load_slab<B_OC>(rb, B, ldb, n0, 0, N, K, t);
+    store_slab<A_OC>(As[0], LDS_A, ra, t);
+    store_slab<B_OC>(Bs[0], LDS_B, rb, t);
+    __syncthreads();
+    const int lrow = lane >> 4, lcol = lane & 15;
+    for (int s = 0; s < nslab; ++s) {
+        const int cur = s & 1;
+        if (s + 1 < nslab) {
+            load_slab<A_OC>(ra, A, lda, m0, i64(s + 1) * BK, M, K, t);
+            load_slab<B_OC>(rb, B, ldb, n0, i64(s + 1) * BK, N, K, t);
+        }
+        const float* as = As[cur];
+        const float* bs = Bs[cur];
+#pragma unroll
+        for (int ks = 0; ks < BK / 4; ++ks) {
+            const int kr = ks * 4 + lrow;
+            float a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = as[kr * LDS_A + wm * 64 + i * 16 + lcol];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = bs[kr * LDS_B + wn * 64 + j * 16 + lcol];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (s + 1 < nslab) {
+            store_slab<A_OC>(As[cur ^ 1], LDS_A, ra, t);
+            store_slab<B_OC>(Bs[cur ^ 1], LDS_B, rb, t);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const i64 col = n0 + wn * 64 + j * 16 + lcol;
+            if (col >= N) continue;
+            float* cc = C + col * ldc;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const i64 row = m0 + wm * 64 + i * 16 + lrow * 4 + r;
+                if (row < M) {
+                    const float v = alpha * acc[i][j][r];
+                    cc[row] = BETA0 ? v : __builtin_fmaf(beta, cc[row], v);
+                }
+            }
+        }
+}
+
+__global__ void sscale_kernel(i64 m, i64 n, float beta, float* __restrict__ C, i64 ldc) {
+    const i64 total = m * n;
+    for (i64 idx = i64(blockIdx.x) * blockDim.x + threadIdx.x; idx < total; idx += i64(gridDim.x) * blockDim.x) {
+        const i64 j = idx / m, i = idx - j * m;
+        float& c = C[i + j * ldc];
+        c = (beta == 0.0f) ? 0.0f : c * beta;
+    }
+}
+
+template <bool TA, bool TB>
+void launch_variant(hipStream_t s, i64 m, i64 n, i64 k, float alpha, const float* A, i64 lda, const float* B, i64 ldb,
+                    float beta, float* C, i64 ldc) {
+    const int tiles_m = int(cdiv(m, BM)), tiles_n = int(cdiv(n, BN));
+    dim3 grid(tiles_m * tiles_n), block(NT);
+    if (beta == 0.0f)
+        hipLaunchKernelGGL((sgemm_mfma_kernel<TA, TB, true>), grid, block, 0, s, m, n, k, alpha, A, lda, B, ldb, beta,
+                           C, ldc, tiles_m, tiles_n);
+    else
+        hipLaunchKernelGGL((sgemm_mfma_kernel<TA, TB, false>), grid, block, 0, s, m, n, k, alpha, A, lda, B, ldb,
+                           beta, C, ldc, tiles_m, tiles_n);
+}
+
+}  // namespace
+
+int oxi_launch_sgemm(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n, i64 k, float alpha, const float* A,
+                     i64 lda, const float* B, i64 ldb, float beta, float* C, i64 ldc) {
+    if (m <= 0 || n <= 0) return OXI_OK;
+    if (alpha == 0.0f || k <= 0) {
+        if (beta == 1.0f) return OXI_OK;
+        const int blocks = int(std::min<i64>(cdiv(m * n, 256), 4096));
+        hipLaunchKernelGGL(sscale_kernel, dim3(blocks), dim3(256), 0, s, m, n, beta, C, ldc);
+    } else if (!ta && !tb) launch_variant<false, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (!ta && tb) launch_variant<false, true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (ta && !tb) launch_variant<true, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else launch_variant<true, true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "sgemm launch: %s", hipGetErrorString(e));
+    return OXI_OK;
+}
+
+extern "C" int oxi_sgemm(oxi_handle_t h, char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha,
+                         const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,
+                         int64_t ldc) {
+    if (!h) return -1;
+    auto kind = [](char t) { return (t == 'N' || t == 'n') ? 0 : (t == 'T' || t == 't' || t == 'C' || t == 'c') ? 1 : -1; };
+    const int ka = kind(transA), kb = kind(transB);
+    if (ka < 0) return -2;
+    if (kb < 0) return -3;
+    if (m < 0) return -4;
+    if (n < 0) return -5;
+    if (k < 0) return -6;
+    if (lda < std::max<i64>(1, ka ? k : m)) return -9;
+    if (ldb < std::max<i64>(1, kb ? n : k)) return -11;
+    if (ldc < std::max<i64>(1, m)) return -14;
+    std::lock_guard<std::recursive_mutex> lk(h->guard);
+    if (m == 0 || n == 0) return OXI_OK;
+    (void)hipSetDevice(h->device);
+    return oxi_launch_sgemm(h, h->stream, ka, kb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}

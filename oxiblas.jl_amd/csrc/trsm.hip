@@ -7,35 +7,38 @@
 #include <algorithm>
 
 namespace {
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 constexpr int TRL = 64;
 
 // Right side: X (rows x n) <- X * op(T)^-1, one thread per row of X.  Element
 // order matches the reference's right leaf (src/blas/level3.jl:319-333):
 // op(T) upper: x_c = (b_c - sum_{p<c} x_p U[p][c]) / U[c][c], c ascending;
 // op(T) lower: x_c = (b_c - sum_{p>c} x_p L[p][c]) / L[c][c], c descending.
-template <bool LOWER, bool UNIT>
-__global__ __launch_bounds__(256) void trsm_right_leaf(double* __restrict__ X, i64 ldx, i64 rows, int n,
-                                                       const double* __restrict__ T, i64 ldt, int trans) {
-    __shared__ double U[TRL * TRL];  // U[p*TRL + c] = op(T)[p][c]
+template <typename TY, bool LOWER, bool UNIT>
+__global__ __launch_bounds__(256) void trsm_right_leaf(TY* __restrict__ X, i64 ldx, i64 rows, int n,
+                                                       const TY* __restrict__ T, i64 ldt, int trans) {
+    __shared__ TY U[TRL * TRL];  // U[p*TRL + c] = op(T)[p][c]
     for (int e = threadIdx.x; e < TRL * TRL; e += blockDim.x) {
         const int pp = e / TRL, c = e % TRL;
-        double val = 0.0;
+        TY val = TY(0);
         if (pp < n && c < n) val = trans ? T[c + i64(pp) * ldt] : T[pp + i64(c) * ldt];
         U[pp * TRL + c] = val;
     }
     __syncthreads();
     const i64 r = i64(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= rows) return;
-    double x[TRL];
+    TY x[TRL];
 #pragma unroll
-    for (int c = 0; c < TRL; ++c) x[c] = c < n ? X[r + c * ldx] : 0.0;
+    for (int c = 0; c < TRL; ++c) x[c] = c < n ? X[r + c * ldx] : TY(0);
     if (!LOWER) {
 #pragma unroll
         for (int c = 0; c < TRL; ++c) {
             if (c < n) {
-                double s = x[c];
+                TY s = x[c];
 #pragma unroll
-                for (int pp = 0; pp < c; ++pp) s = __builtin_fma(-x[pp], U[pp * TRL + c], s);
+                for (int pp = 0; pp < c; ++pp) s = fma_t(-x[pp], U[pp * TRL + c], s);
                 x[c] = UNIT ? s : s / U[c * TRL + c];
             }
         }
@@ -43,10 +46,10 @@ __global__ __launch_bounds__(256) void trsm_right_leaf(double* __restrict__ X, i
 #pragma unroll
         for (int c = TRL - 1; c >= 0; --c) {
             if (c < n) {
-                double s = x[c];
+                TY s = x[c];
 #pragma unroll
                 for (int pp = TRL - 1; pp > c; --pp)
-                    if (pp < n) s = __builtin_fma(-x[pp], U[pp * TRL + c], s);
+                    if (pp < n) s = fma_t(-x[pp], U[pp * TRL + c], s);
                 x[c] = UNIT ? s : s / U[c * TRL + c];
             }
         }
@@ -59,13 +62,13 @@ __global__ __launch_bounds__(256) void trsm_right_leaf(double* __restrict__ X, i
 // row.  LOWER refers to op(T).  Per element: s = b_i; s = fma(-T[i][p], x_p, s)
 // in order of p; s / T[i][i] when not unit (src/blas/level3.jl:309-318,
 // src/blas/triangular_microkernels.jl:31-59).
-template <bool LOWER, bool UNIT>
-__global__ __launch_bounds__(256) void trsm_left_leaf(double* __restrict__ X, i64 ldx, i64 nrhs, int n,
-                                                      const double* __restrict__ T, i64 ldt, int trans) {
-    __shared__ double S[TRL * (TRL + 1)];  // S[p*(TRL+1) + i] = op(T)[i][p]
+template <typename TY, bool LOWER, bool UNIT>
+__global__ __launch_bounds__(256) void trsm_left_leaf(TY* __restrict__ X, i64 ldx, i64 nrhs, int n,
+                                                      const TY* __restrict__ T, i64 ldt, int trans) {
+    __shared__ TY S[TRL * (TRL + 1)];  // S[p*(TRL+1) + i] = op(T)[i][p]
     for (int e = threadIdx.x; e < TRL * TRL; e += blockDim.x) {
         const int pp = e / TRL, i = e % TRL;
-        double val = 0.0;
+        TY val = TY(0);
         if (pp < n && i < n) val = trans ? T[pp + i64(i) * ldt] : T[i + i64(pp) * ldt];
         S[pp * (TRL + 1) + i] = val;
     }
@@ -73,19 +76,19 @@ __global__ __launch_bounds__(256) void trsm_left_leaf(double* __restrict__ X, i6
     const int lane = threadIdx.x & 63;
     const i64 col = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (col >= nrhs) return;
-    double* xc = X + col * ldx;
-    double x = lane < n ? xc[lane] : 0.0;
+    TY* xc = X + col * ldx;
+    TY x = lane < n ? xc[lane] : TY(0);
     if (LOWER) {
         for (int pp = 0; pp < n; ++pp) {
             if (!UNIT && lane == pp) x = x / S[pp * (TRL + 1) + pp];
-            const double xp = __shfl(x, pp);
-            if (lane > pp) x = __builtin_fma(-S[pp * (TRL + 1) + lane], xp, x);
+            const TY xp = __shfl(x, pp);
+            if (lane > pp) x = fma_t(-S[pp * (TRL + 1) + lane], xp, x);
         }
     } else {
         for (int pp = n - 1; pp >= 0; --pp) {
             if (!UNIT && lane == pp) x = x / S[pp * (TRL + 1) + pp];
-            const double xp = __shfl(x, pp);
-            if (lane < pp) x = __builtin_fma(-S[pp * (TRL + 1) + lane], xp, x);
+            const TY xp = __shfl(x, pp);
+            if (lane < pp) x = fma_t(-S[pp * (TRL + 1) + lane], xp, x);
         }
     }
     if (lane < n) xc[lane] = x;
@@ -106,16 +109,31 @@ int launch_check(oxi_handle_t h, const char* what) {
 }
 }  // namespace
 
+template <typename TY>
+int gemm_t(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n, i64 k, TY alpha, const TY* A, i64 lda,
+           const TY* B, i64 ldb, TY beta, TY* C, i64 ldc);
+template <>
+int gemm_t<double>(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n, i64 k, double alpha, const double* A,
+                   i64 lda, const double* B, i64 ldb, double beta, double* C, i64 ldc) {
+    return oxi_launch_dgemm(h, s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+template <>
+int gemm_t<float>(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n, i64 k, float alpha, const float* A,
+                  i64 lda, const float* B, i64 ldb, float beta, float* C, i64 ldc) {
+    return oxi_launch_sgemm(h, s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
 // Right-side triangular solve X <- X * op(T)^-1 where op(T) is upper
 // triangular (T upper and trans == 0, or T lower and trans == 1), recursive
 // halving with GEMM updates (src/blas/level3.jl:358-376).
-static int trsm_right_upper(oxi_handle_t h, hipStream_t s, int trans, int unit, i64 rows, i64 n, const double* T,
-                            i64 ldt, double* X, i64 ldx) {
+template <typename TY>
+static int trsm_right_upper(oxi_handle_t h, hipStream_t s, int trans, int unit, i64 rows, i64 n, const TY* T,
+                            i64 ldt, TY* X, i64 ldx) {
     if (rows <= 0 || n <= 0) return OXI_OK;
     if (n <= TRL) {
         dim3 grid(unsigned(cdiv(rows, 256)));
-        if (unit) hipLaunchKernelGGL((trsm_right_leaf<false, true>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
-        else hipLaunchKernelGGL((trsm_right_leaf<false, false>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
+        if (unit) hipLaunchKernelGGL((trsm_right_leaf<TY, false, true>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
+        else hipLaunchKernelGGL((trsm_right_leaf<TY, false, false>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
         return launch_check(h, "trsm_right_leaf");
     }
     i64 hh = n / 2;
@@ -123,75 +141,96 @@ static int trsm_right_upper(oxi_handle_t h, hipStream_t s, int trans, int unit, 
     // X1 = X[:, :hh], X2 = X[:, hh:]; op(T) = [U11 U12; 0 U22]
     OXI_TRY(trsm_right_upper(h, s, trans, unit, rows, hh, T, ldt, X, ldx));
     // X2 -= X1 * U12, U12 = op(T)[0:hh, hh:n]
-    const double* U12 = trans ? T + hh : T + hh * ldt;  // op(T)[0][hh]: trans -> T[hh][0]
-    OXI_TRY(oxi_launch_dgemm(h, s, 0, trans, rows, n - hh, hh, -1.0, X, ldx, U12, ldt, 1.0, X + hh * ldx, ldx));
+    const TY* U12 = trans ? T + hh : T + hh * ldt;  // op(T)[0][hh]: trans -> T[hh][0]
+    OXI_TRY(gemm_t<TY>(h, s, 0, trans, rows, n - hh, hh, TY(-1), X, ldx, U12, ldt, TY(1), X + hh * ldx, ldx));
     return trsm_right_upper(h, s, trans, unit, rows, n - hh, T + hh + hh * ldt, ldt, X + hh * ldx, ldx);
 }
 
 // Right-side solve with op(T) lower: X <- X * op(T)^-1 (backward over columns).
-static int trsm_right_lower(oxi_handle_t h, hipStream_t s, int trans, int unit, i64 rows, i64 n, const double* T,
-                            i64 ldt, double* X, i64 ldx);
+template <typename TY>
+static int trsm_right_lower(oxi_handle_t h, hipStream_t s, int trans, int unit, i64 rows, i64 n, const TY* T,
+                            i64 ldt, TY* X, i64 ldx);
 
-int oxi_dtrsm_left(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 nrhs, const double* A,
-                   i64 lda, double* B, i64 ldb) {
+template <typename TY>
+static int trsm_left_t(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 nrhs, const TY* A,
+                       i64 lda, TY* B, i64 ldb) {
     if (m <= 0 || nrhs <= 0) return OXI_OK;
     const bool lower_op = (!upper) != (trans != 0);  // op(A) lower?
     if (m <= TRL) {
         dim3 grid(unsigned(cdiv(nrhs, 4)));
         if (lower_op) {
-            if (unit) hipLaunchKernelGGL((trsm_left_leaf<true, true>), grid, dim3(256), 0, s, B, ldb, nrhs, int(m), A, lda, trans);
-            else hipLaunchKernelGGL((trsm_left_leaf<true, false>), grid, dim3(256), 0, s, B, ldb, nrhs, int(m), A, lda, trans);
+            if (unit) hipLaunchKernelGGL((trsm_left_leaf<TY, true, true>), grid, dim3(256), 0, s, B, ldb, nrhs, int(m), A, lda, trans);
+            else hipLaunchKernelGGL((trsm_left_leaf<TY, true, false>), grid, dim3(256), 0, s, B, ldb, nrhs, int(m), A, lda, trans);
         } else {
-            if (unit) hipLaunchKernelGGL((trsm_left_leaf<false, true>), grid, dim3(256), 0, s, B, ldb, nrhs, int(m), A, lda, trans);
-            else hipLaunchKernelGGL((trsm_left_leaf<false, false>), grid, dim3(256), 0, s, B, ldb, nrhs, int(m), A, lda, trans);
+            if (unit) hipLaunchKernelGGL((trsm_left_leaf<TY, false, true>), grid, dim3(256), 0, s, B, ldb, nrhs, int(m), A, lda, trans);
+            else hipLaunchKernelGGL((trsm_left_leaf<TY, false, false>), grid, dim3(256), 0, s, B, ldb, nrhs, int(m), A, lda, trans);
         }
         return launch_check(h, "trsm_left_leaf");
     }
     i64 hh = m / 2;
     if (hh > TRL) hh = (hh / TRL) * TRL;
     // op(A) = [A11 A12; A21 A22] in op coordinates; op(A)[i][j] = trans ? A[j][i] : A[i][j]
-    const double* A11 = A;
-    const double* A22 = A + hh + hh * lda;
+    const TY* A11 = A;
+    const TY* A22 = A + hh + hh * lda;
     if (lower_op) {
-        OXI_TRY(oxi_dtrsm_left(h, s, upper, trans, unit, hh, nrhs, A11, lda, B, ldb));
+        OXI_TRY(trsm_left_t(h, s, upper, trans, unit, hh, nrhs, A11, lda, B, ldb));
         // B2 -= op(A)[hh:, :hh] * B1 ; op(A)[hh][0] = trans ? A[0][hh] : A[hh][0]
-        const double* A21 = trans ? A + hh * lda : A + hh;
-        OXI_TRY(oxi_launch_dgemm(h, s, trans, 0, m - hh, nrhs, hh, -1.0, A21, lda, B, ldb, 1.0, B + hh, ldb));
-        return oxi_dtrsm_left(h, s, upper, trans, unit, m - hh, nrhs, A22, lda, B + hh, ldb);
+        const TY* A21 = trans ? A + hh * lda : A + hh;
+        OXI_TRY(gemm_t<TY>(h, s, trans, 0, m - hh, nrhs, hh, TY(-1), A21, lda, B, ldb, TY(1), B + hh, ldb));
+        return trsm_left_t(h, s, upper, trans, unit, m - hh, nrhs, A22, lda, B + hh, ldb);
     }
-    OXI_TRY(oxi_dtrsm_left(h, s, upper, trans, unit, m - hh, nrhs, A22, lda, B + hh, ldb));
+    OXI_TRY(trsm_left_t(h, s, upper, trans, unit, m - hh, nrhs, A22, lda, B + hh, ldb));
     // B1 -= op(A)[:hh, hh:] * B2 ; op(A)[0][hh] = trans ? A[hh][0] : A[0][hh]
-    const double* A12 = trans ? A + hh : A + hh * lda;
-    OXI_TRY(oxi_launch_dgemm(h, s, trans, 0, hh, nrhs, m - hh, -1.0, A12, lda, B + hh, ldb, 1.0, B, ldb));
-    return oxi_dtrsm_left(h, s, upper, trans, unit, hh, nrhs, A11, lda, B, ldb);
+    const TY* A12 = trans ? A + hh : A + hh * lda;
+    OXI_TRY(gemm_t<TY>(h, s, trans, 0, hh, nrhs, m - hh, TY(-1), A12, lda, B + hh, ldb, TY(1), B, ldb));
+    return trsm_left_t(h, s, upper, trans, unit, hh, nrhs, A11, lda, B, ldb);
 }
 
 // Right-side solve with op(T) lower: X <- X * op(T)^-1.  op(T) = [L11 0; L21 L22]:
 // X2 = B2 * L22^-1 first, then B1 -= X2 * L21, then X1 = B1 * L11^-1.
-static int trsm_right_lower(oxi_handle_t h, hipStream_t s, int trans, int unit, i64 rows, i64 n, const double* T,
-                            i64 ldt, double* X, i64 ldx) {
+template <typename TY>
+static int trsm_right_lower(oxi_handle_t h, hipStream_t s, int trans, int unit, i64 rows, i64 n, const TY* T,
+                            i64 ldt, TY* X, i64 ldx) {
     if (rows <= 0 || n <= 0) return OXI_OK;
     if (n <= TRL) {
         dim3 grid(unsigned(cdiv(rows, 256)));
-        if (unit) hipLaunchKernelGGL((trsm_right_leaf<true, true>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
-        else hipLaunchKernelGGL((trsm_right_leaf<true, false>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
+        if (unit) hipLaunchKernelGGL((trsm_right_leaf<TY, true, true>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
+        else hipLaunchKernelGGL((trsm_right_leaf<TY, true, false>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
         return launch_check(h, "trsm_right_leaf");
     }
     i64 hh = n / 2;
     if (hh > TRL) hh = (hh / TRL) * TRL;
     OXI_TRY(trsm_right_lower(h, s, trans, unit, rows, n - hh, T + hh + hh * ldt, ldt, X + hh * ldx, ldx));
-    const double* L21 = trans ? T + hh * ldt : T + hh;  // op(T)[hh][0]
-    OXI_TRY(oxi_launch_dgemm(h, s, 0, trans, rows, hh, n - hh, -1.0, X + hh * ldx, ldx, L21, ldt, 1.0, X, ldx));
+    const TY* L21 = trans ? T + hh * ldt : T + hh;  // op(T)[hh][0]
+    OXI_TRY(gemm_t<TY>(h, s, 0, trans, rows, hh, n - hh, TY(-1), X + hh * ldx, ldx, L21, ldt, TY(1), X, ldx));
     return trsm_right_lower(h, s, trans, unit, rows, hh, T, ldt, X, ldx);
 }
 
-int oxi_dtrsm_right(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 n, const double* A,
-                    i64 lda, double* B, i64 ldb) {
+template <typename TY>
+static int trsm_right_t(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 n, const TY* A,
+                        i64 lda, TY* B, i64 ldb) {
     const bool upper_op = (upper != 0) != (trans != 0);
     if (upper_op) return trsm_right_upper(h, s, trans, unit, m, n, A, lda, B, ldb);
     return trsm_right_lower(h, s, trans, unit, m, n, A, lda, B, ldb);
 }
 
+
+int oxi_dtrsm_left(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 nrhs, const double* A,
+                   i64 lda, double* B, i64 ldb) {
+    return trsm_left_t<double>(h, s, upper, trans, unit, m, nrhs, A, lda, B, ldb);
+}
+int oxi_dtrsm_right(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 n, const double* A,
+                    i64 lda, double* B, i64 ldb) {
+    return trsm_right_t<double>(h, s, upper, trans, unit, m, n, A, lda, B, ldb);
+}
+int oxi_strsm_left(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 nrhs, const float* A,
+                   i64 lda, float* B, i64 ldb) {
+    return trsm_left_t<float>(h, s, upper, trans, unit, m, nrhs, A, lda, B, ldb);
+}
+int oxi_strsm_right(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 n, const float* A,
+                    i64 lda, float* B, i64 ldb) {
+    return trsm_right_t<float>(h, s, upper, trans, unit, m, n, A, lda, B, ldb);
+}
 
 static int trans_kind(char t) {
     if (t == 'N' || t == 'n') return 0;

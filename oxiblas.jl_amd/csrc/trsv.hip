@@ -1,0 +1,261 @@
+// Triangular solve with one right-hand side against LU factors:
+// trsv!(x, A; uplo, trans, unit)  (src/blas/level2.jl trsv!, the vector branch
+// of the LU solve, src/api/linalg_api.jl:168-184).
+//
+// One launch per triangle.  The n rows are cut into blocks of RB=128 rows in
+// solve order; block b needs x of every earlier block.  Workgroups take block
+// numbers from an atomic ticket, so a workgroup only ever waits on blocks
+// owned by workgroups that are already running (no dependence on dispatch
+// order, no deadlock).  A workgroup
+//   1. stages its RB x RB diagonal block in LDS and inverts its diagonal,
+//   2. streams the off-diagonal part of its block row, chunk by chunk, as the
+//      earlier blocks publish their x (one agent-scope `ready` counter that
+//      only grows: block b stores b+1 after its x is written),
+//   3. reduces the partial sums, and one wave solves the diagonal block by
+//      column sweep (lane i owns rows i and i+64; x_k broadcast by readlane),
+//   4. writes its x and releases `ready`.
+// The matrix is read once (HBM-bound: n^2/2 elements per triangle); the
+// dependence chain is the n column-sweep steps plus one publish per block.
+//
+// The matrix type TM and the arithmetic/vector type TV may differ: FP32
+// factors solved in FP64 arithmetic is the preconditioner of GMRES-IR.
+// Solve order and memory access: `which` 0 = L (unit) forward, 1 = U backward,
+// 2 = U^T forward, 3 = L^T (unit) backward; backward solves use logical index
+// i' = n-1-i, so the same code runs both directions with coalesced access.
+#include "oxi_internal.hpp"
+
+namespace {
+
+constexpr int RB = 128, NT = 512;
+constexpr unsigned SPIN_LIMIT = 1u << 20;
+
+struct TrsvCtrl { unsigned ticket, ready, err, pad; };
+
+template <typename TM> struct Vec;
+template <> struct Vec<double> { typedef double type __attribute__((ext_vector_type(2))); static constexpr int N = 2; };
+template <> struct Vec<float> { typedef float type __attribute__((ext_vector_type(4))); static constexpr int N = 4; };
+
+__device__ __forceinline__ double bcast(double v, int l) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane(unsigned(u), l);
+    const unsigned hi = __builtin_amdgcn_readlane(unsigned(u >> 32), l);
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+__device__ __forceinline__ float bcast(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(unsigned, v), l));
+}
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+__global__ void trsv_ctrl_reset(TrsvCtrl* c) {
+    if (threadIdx.x == 0) { c->ticket = 0; c->ready = 0; c->err = 0; c->pad = 0; }
+}
+
+template <typename TM, typename TV, bool TRANS, bool UNIT, bool BWD, bool VECOK>
+__global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M, i64 ld, i64 n, TV* x,
+                                                        TrsvCtrl* ctrl) {
+    constexpr int VEC = VECOK ? Vec<TM>::N : 1;
+    constexpr int LPC = RB / VEC;              // no-trans: lanes per column
+    constexpr int CG = NT / LPC;               // no-trans: column groups
+    constexpr int NRED = TRANS ? 1 : CG;
+    constexpr int LPR = 8;                     // trans: lanes per row
+    constexpr int RPT = RB / (NT / LPR);       // trans: rows per thread
+    constexpr int ITS = RB / (LPR * VEC);      // trans: iterations per chunk
+    __shared__ TM D[RB * (RB + 1)];            // D[k][i] = T(i, k), logical in-block indices
+    __shared__ TV red[NRED][RB];
+    __shared__ TV rinv[RB];
+    __shared__ int s_b, s_ready;
+
+    const int t = threadIdx.x;
+    if (t == 0) s_b = int(atomicAdd(&ctrl->ticket, 1u));
+    __syncthreads();
+    const int b = s_b;
+    const i64 lo = i64(b) * RB;
+    const int rows = int(min(i64(RB), n - lo));
+    const i64 p0 = BWD ? n - lo - rows : lo;   // first physical row of the block
+
+    // 1. diagonal block -> LDS (physical offsets pr, pc; stored by logical index)
+    for (int e = t; e < RB * RB; e += NT) {
+        const int pr = e % RB, pc = e / RB;
+        if (pr < rows && pc < rows) {
+            const TM v = M[(p0 + pr) + (p0 + pc) * ld];
+            const int lr = BWD ? rows - 1 - pr : pr, lc = BWD ? rows - 1 - pc : pc;
+            const int i = TRANS ? lc : lr, k = TRANS ? lr : lc;
+            D[k * (RB + 1) + i] = v;
+        }
+    }
+    __syncthreads();
+    if (!UNIT)
+        for (int k = t; k < rows; k += NT) rinv[k] = TV(1) / TV(D[k * (RB + 1) + k]);
+
+    // 2. off-diagonal block row, chunk by chunk as earlier blocks publish
+    TV acc[TRANS ? RPT : VEC];
+#pragma unroll
+    for (int r = 0; r < (TRANS ? RPT : VEC); ++r) acc[r] = TV(0);
+    int avail = 0;
+    for (int c = 0; c < b; ++c) {
+        if (c >= avail) {
+            if (t == 0) {
+                unsigned v, spins = 0;
+                while ((v = __hip_atomic_load(&ctrl->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) <=
+                       unsigned(c)) {
+                    if (__hip_atomic_load(&ctrl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                        ++spins > SPIN_LIMIT) {
+                        __hip_atomic_store(&ctrl->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        v = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                s_ready = int(v);
+            }
+            __syncthreads();
+            avail = s_ready;
+            __syncthreads();
+            if (avail <= c) return;            // timed out: uniform exit
+        }
+        const i64 q0 = BWD ? n - i64(c + 1) * RB : i64(c) * RB;   // chunk = full block
+        if constexpr (!TRANS) {
+            // T = M: block rows (contiguous) x chunk columns
+            const int pr = (t % LPC) * VEC, cg = t / LPC;
+            if (pr < rows) {
+#pragma unroll 4
+                for (int q = cg; q < RB; q += CG) {
+                    const i64 col = q0 + q;
+                    const TV xv = x[col];
+                    const TM* src = M + (p0 + pr) + col * ld;
+                    if constexpr (VECOK) {
+                        const typename Vec<TM>::type m = *reinterpret_cast<const typename Vec<TM>::type*>(src);
+#pragma unroll
+                        for (int v = 0; v < VEC; ++v) acc[v] = fma_t(TV(m[v]), xv, acc[v]);
+                    } else {
+                        acc[0] = fma_t(TV(src[0]), xv, acc[0]);
+                    }
+                }
+            }
+        } else {
+            // T = M^T: block rows are columns of M, chunk indices are rows of M (contiguous)
+            const int u = t % LPR, rg = t / LPR;
+#pragma unroll 2
+            for (int it = 0; it < ITS; ++it) {
+                const int jj = (it * LPR + u) * VEC;
+                TV xv[VEC];
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) xv[v] = x[q0 + jj + v];
+#pragma unroll
+                for (int r = 0; r < RPT; ++r) {
+                    const int pr = rg + r * (NT / LPR);
+                    if (pr < rows) {
+                        const TM* src = M + (q0 + jj) + (p0 + pr) * ld;
+                        if constexpr (VECOK) {
+                            const typename Vec<TM>::type m = *reinterpret_cast<const typename Vec<TM>::type*>(src);
+#pragma unroll
+                            for (int v = 0; v < VEC; ++v) acc[r] = fma_t(TV(m[v]), xv[v], acc[r]);
+                        } else {
+                            acc[r] = fma_t(TV(src[0]), xv[0], acc[r]);
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // 3. partial sums -> LDS (fixed order: deterministic)
+    if constexpr (!TRANS) {
+        const int pr = (t % LPC) * VEC, cg = t / LPC;
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) red[cg][pr + v] = acc[v];
+    } else {
+        const int u = t % LPR, rg = t / LPR;
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            TV a = acc[r];
+#pragma unroll
+            for (int o = 1; o < LPR; o <<= 1) a += __shfl_xor(a, o);
+            if (u == 0) red[0][rg + r * (NT / LPR)] = a;
+        }
+    }
+    __syncthreads();
+    if (t >= 64) return;
+
+    // 4. diagonal block by column sweep in wave 0
+    TV s[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int i = t + 64 * h;
+        s[h] = TV(0);
+        if (i < rows) {
+            const int pr = BWD ? rows - 1 - i : i;
+            TV sum = red[0][pr];
+#pragma unroll
+            for (int g = 1; g < NRED; ++g) sum += red[g][pr];
+            s[h] = x[p0 + pr] - sum;
+        }
+    }
+    for (int k = 0; k < rows; ++k) {
+        const TV sk = k < 64 ? s[0] : s[1];
+        const TV xk = bcast(UNIT ? sk : sk * rinv[k], k & 63);
+        const TV d0 = TV(D[k * (RB + 1) + t]), d1 = TV(D[k * (RB + 1) + 64 + t]);
+        if (t > k) s[0] = fma_t(-d0, xk, s[0]);
+        if (t + 64 > k) s[1] = fma_t(-d1, xk, s[1]);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int i = t + 64 * h;
+        if (i < rows) x[p0 + (BWD ? rows - 1 - i : i)] = UNIT ? s[h] : s[h] * rinv[i];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (t == 0) __hip_atomic_store(&ctrl->ready, unsigned(b + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename TM, typename TV, bool TRANS, bool UNIT, bool BWD>
+void launch_dir(hipStream_t s, i64 n, const TM* M, i64 ld, TV* x, TrsvCtrl* c, bool vecok) {
+    const unsigned grid = unsigned(cdiv(n, RB));
+    if (vecok)
+        hipLaunchKernelGGL((trsv_chain_kernel<TM, TV, TRANS, UNIT, BWD, true>), dim3(grid), dim3(NT), 0, s, M, ld, n,
+                           x, c);
+    else
+        hipLaunchKernelGGL((trsv_chain_kernel<TM, TV, TRANS, UNIT, BWD, false>), dim3(grid), dim3(NT), 0, s, M, ld,
+                           n, x, c);
+}
+
+template <typename TM, typename TV>
+int trsv_t(oxi_handle_t h, hipStream_t s, int which, i64 n, const TM* M, i64 ld, TV* x, void* ctrl) {
+    if (n <= 0) return OXI_OK;
+    if (n > i64(RB) * 0x7fffffff) return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "trsv: n too large");
+    TrsvCtrl* c = static_cast<TrsvCtrl*>(ctrl);
+    constexpr int VEC = Vec<TM>::N;
+    const bool vecok = (reinterpret_cast<uintptr_t>(M) % 16 == 0) && (ld % VEC == 0) && (n % VEC == 0);
+    hipLaunchKernelGGL(trsv_ctrl_reset, dim3(1), dim3(64), 0, s, c);
+    switch (which) {
+        case 0: launch_dir<TM, TV, false, true, false>(s, n, M, ld, x, c, vecok); break;
+        case 1: launch_dir<TM, TV, false, false, true>(s, n, M, ld, x, c, vecok); break;
+        case 2: launch_dir<TM, TV, true, false, false>(s, n, M, ld, x, c, vecok); break;
+        case 3: launch_dir<TM, TV, true, true, true>(s, n, M, ld, x, c, vecok); break;
+        default: return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "trsv: bad triangle %d", which);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "trsv launch: %s", hipGetErrorString(e));
+    return OXI_OK;
+}
+
+}  // namespace
+
+int oxi_trsv(oxi_handle_t h, hipStream_t s, int which, i64 n, const double* M, i64 ld, double* x, void* ctrl) {
+    return trsv_t<double, double>(h, s, which, n, M, ld, x, ctrl);
+}
+int oxi_trsv(oxi_handle_t h, hipStream_t s, int which, i64 n, const float* M, i64 ld, float* x, void* ctrl) {
+    return trsv_t<float, float>(h, s, which, n, M, ld, x, ctrl);
+}
+int oxi_trsv(oxi_handle_t h, hipStream_t s, int which, i64 n, const float* M, i64 ld, double* x, void* ctrl) {
+    return trsv_t<float, double>(h, s, which, n, M, ld, x, ctrl);
+}
+
+int oxi_trsv_check(oxi_handle_t h, hipStream_t s, const void* ctrl) {
+    unsigned err = 0;
+    OXI_HIP(h, hipMemcpyAsync(&err, &static_cast<const TrsvCtrl*>(ctrl)->err, sizeof(unsigned),
+                              hipMemcpyDeviceToHost, s));
+    OXI_HIP(h, hipStreamSynchronize(s));
+    if (err) return oxi_set_error(h, OXI_ERR_TIMEOUT, "trsv: block chain timed out");
+    return OXI_OK;
+}

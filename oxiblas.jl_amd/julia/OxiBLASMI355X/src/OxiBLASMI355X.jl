@@ -14,7 +14,7 @@ import LinearAlgebra: ldiv!, issuccess, Factorization, SingularException
 import ChainRulesCore
 import ChainRulesCore: rrule, NoTangent, unthunk, @thunk
 
-export DeviceMatrix, gemm!, lu, lu!, ldiv, solve, solve_transposed
+export DeviceMatrix, gemm!, lu, lu!, ldiv, solve, solve_transposed, mixed_solve, equilibrate
 
 const LIB = get(ENV, "OXIBLAS_MI355X_LIB",
                 normpath(joinpath(@__DIR__, "..", "..", "..", "liboxiblas_mi355x.so")))
@@ -289,6 +289,29 @@ function rrule(::typeof(\), F::LU, B::StridedVecOrMat{Float64})
     X, ldiv_pullback
 end
 
+# x = A \ B on a plain matrix: one factorization kept in HBM, one fused
+# pullback call (transposed solve + rank-nrhs GEMM) through oxi_dgesv_adjoint.
+function rrule(::typeof(\), A::StridedMatrix{Float64}, B::StridedVecOrMat{Float64})
+    F = lu(A)
+    n, k = size(A, 1), size(B, 2)
+    Xd = DeviceMatrix(Matrix{Float64}(reshape(B, n, k)))
+    _getrs!(F, Xd, 'N')
+    X = Array(Xd)
+    function ldiv_pullback(X̄)
+        X̄d = DeviceMatrix(Matrix{Float64}(reshape(unthunk(X̄), n, k)))
+        B̄d = DeviceMatrix(n, k); Ād = DeviceMatrix(n, n)
+        h = handle()
+        rc = ccall((:oxi_dgesv_adjoint, LIB), Cint,
+                   (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Int64, Ptr{Int64}, Ptr{Float64}, Int64, Ptr{Float64},
+                    Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64),
+                   h, n, k, F.factors.ptr, F.factors.ld, getfield(F, :ipiv), Xd.ptr, Xd.ld, X̄d.ptr, X̄d.ld,
+                   B̄d.ptr, B̄d.ld, Ād.ptr, Ād.ld)
+        _check(h, rc, "ldiv pullback")
+        (NoTangent(), Array(Ād), reshape(Array(B̄d), size(B)))
+    end
+    reshape(X, size(B)), ldiv_pullback
+end
+
 function rrule(::typeof(*), A::StridedMatrix{Float64}, B::StridedMatrix{Float64})
     C = zeros(size(A, 1), size(B, 2)); gemm!(C, A, B)
     function mul_pullback(C̄)
@@ -298,6 +321,84 @@ function rrule(::typeof(*), A::StridedMatrix{Float64}, B::StridedMatrix{Float64}
         (NoTangent(), Ā, B̄)
     end
     C, mul_pullback
+end
+
+# ---------------------------------------------------------------------------
+# mixed precision (src/lapack/extended_lapack.jl:175-226) and equilibration
+# (src/lapack/dense_utilities.jl:191-212)
+# ---------------------------------------------------------------------------
+struct IRInfo
+    iterations::Int64
+    inner_iterations::Int64
+    converged::Int32
+    fallback::Int32
+    reason::Int32
+    solver::Int32
+    backward_error::Float64
+    normwise_backward_error::Float64
+    residual_norm::Float64
+    factor_ms::Float64
+    refine_ms::Float64
+end
+IRInfo() = IRInfo(0, 0, 0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0.0)
+
+# low type -> OXI_PREC_*: Float32 (the reference's default), Float16, and the
+# names of types Base lacks
+function _prec(lowtype)
+    lowtype === Float32 && return Cint(3)
+    lowtype === Float16 && return Cint(1)
+    name = lowercase(string(lowtype))
+    occursin("bfloat16", name) && return Cint(0)
+    (occursin("float8", name) || occursin("e4m3", name)) && return Cint(2)
+    throw(ArgumentError("low precision type must be Float32, Float16, BFloat16 or Float8_E4M3FN"))
+end
+const _REASONS = (:converged, :iteration_limit, :nonfinite_iteration, :working_factor_fallback)
+
+"""
+    mixed_solve(A, b; maxiter=20, rtol=16eps(), lowtype=Float32, solver=:ir, gmres_restart=0)
+
+Radix equilibration, a low-precision LU on the GPU (FP32 panel; trailing
+products in `lowtype`) and FP64 refinement until the componentwise backward
+error is <= rtol; `solver=:gmres` runs GMRES-IR, `:auto` switches to it when
+classical refinement stalls.  Returns the reference's NamedTuple.
+"""
+function mixed_solve(A::StridedMatrix{Float64}, b::StridedVecOrMat{Float64}; maxiter::Integer=20,
+                     rtol::Real=16eps(Float64), lowtype=Float32, solver::Symbol=:ir, gmres_restart::Integer=0,
+                     factorization::Symbol=:lu)
+    factorization === :lu || throw(ArgumentError("the GPU mixed solve implements factorization=:lu"))
+    n = size(A, 1); size(A, 2) == n || throw(DimensionMismatch("mixed factorization dimensions differ"))
+    B = reshape(b, size(b, 1), :); k = size(B, 2)
+    Ad = DeviceMatrix(Matrix{Float64}(A)); Bd = DeviceMatrix(Matrix{Float64}(B)); Xd = DeviceMatrix(n, k)
+    info = Ref(IRInfo())
+    h = handle()
+    rc = ccall((:oxi_dgesv_mixed, LIB), Cint,
+               (Ptr{Cvoid}, Cint, Cint, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64,
+                Int32, Float64, Int32, Ref{IRInfo}),
+               h, _prec(lowtype), Cint(solver === :ir ? 0 : solver === :gmres ? 1 : 2), n, k, Ad.ptr, Ad.ld, Bd.ptr,
+               Bd.ld, Xd.ptr, Xd.ld, Int32(maxiter), Float64(rtol), Int32(gmres_restart), info)
+    rc == 6 && throw(ArgumentError("finite mixed input required"))
+    _check(h, rc, "mixed_solve")
+    r = info[]
+    X = Array(Xd)
+    (; x=ndims(b) == 1 ? vec(X) : X, converged=r.converged != 0, iterations=Int(r.iterations),
+     residual=r.residual_norm, backward_error=r.backward_error, metric=:componentwise_backward_error,
+     low_type=lowtype, fallback=r.fallback != 0, factorization=:lu, finite=r.reason != 2,
+     reason=_REASONS[r.reason + 1], gmres_iterations=Int(r.inner_iterations))
+end
+
+"""
+    equilibrate(A; radix=false) -> (; matrix, row_scale, column_scale)
+"""
+function equilibrate(A::StridedMatrix{Float64}; radix::Bool=false)
+    m, n = size(A)
+    Ad = DeviceMatrix(Matrix{Float64}(A)); Ed = DeviceMatrix(m, n)
+    r = DeviceMatrix(max(m, 1), 1); c = DeviceMatrix(max(n, 1), 1)
+    h = handle()
+    rc = ccall((:oxi_dequilibrate, LIB), Cint,
+               (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}, Cint),
+               h, m, n, Ad.ptr, Ad.ld, Ed.ptr, Ed.ld, r.ptr, c.ptr, Cint(radix))
+    _check(h, rc, "equilibrate")
+    (; matrix=Array(Ed), row_scale=vec(Array(r))[1:m], column_scale=vec(Array(c))[1:n])
 end
 
 end # module

@@ -1,5 +1,7 @@
 """oxiblas_mi355x -- MI355X (gfx950) implementation of OxiBLAS.jl's level-3
-hot path: gemm!, the blocked LU with partial pivoting and its solves.
+hot path: gemm!, the blocked LU with partial pivoting and its solves, the
+mixed-precision LU with FP64 refinement (IR / GMRES-IR) and the adjoints of
+X = A \\ B and gemm!.
 
 Every computation runs in liboxiblas_mi355x.so (hand-written HIP kernels for
 CDNA4) through its C ABI; this package is the host-side mirror of the
@@ -11,9 +13,15 @@ from .linalg import (  # noqa: F401
     gemm_, trsm_, lu, lu_, lu_factor, LU, ldiv, ldiv_, solve, solve_transposed,
     transpose, adjoint, SingularException, DimensionMismatch,
 )
+from .mixed import (  # noqa: F401
+    mixed_solve, equilibrate, sgemm_, gemm_lowp_update_, lu_lowp, LUFactor32, MixedSolveResult, precision_code,
+)
+from .adjoint import ldiv_pullback, rrule_ldiv, gemm_pullback, rrule_mul  # noqa: F401
 
 __all__ = [
     "load", "handle", "Handle", "OxiError", "DMatrix", "as_dmatrix",
     "gemm_", "trsm_", "lu", "lu_", "lu_factor", "LU", "ldiv", "ldiv_", "solve", "solve_transposed",
     "transpose", "adjoint", "SingularException", "DimensionMismatch",
+    "mixed_solve", "equilibrate", "sgemm_", "gemm_lowp_update_", "lu_lowp", "LUFactor32", "MixedSolveResult",
+    "precision_code", "ldiv_pullback", "rrule_ldiv", "gemm_pullback", "rrule_mul",
 ]

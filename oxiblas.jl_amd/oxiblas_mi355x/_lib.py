@@ -23,6 +23,29 @@ _i64 = C.c_int64
 _vp = C.c_void_p
 _ch = C.c_char
 _dbl = C.c_double
+_flt = C.c_float
+
+OXI_PREC_BF16 = 0
+OXI_PREC_FP16 = 1
+OXI_PREC_FP8 = 2
+OXI_PREC_FP32 = 3
+
+
+class IRInfo(C.Structure):
+    """oxi_ir_info: what mixed_solve reports (src/lapack/extended_lapack.jl:219-225)."""
+    _fields_ = [
+        ("iterations", C.c_int64),
+        ("inner_iterations", C.c_int64),
+        ("converged", C.c_int32),
+        ("fallback", C.c_int32),
+        ("reason", C.c_int32),
+        ("solver", C.c_int32),
+        ("backward_error", C.c_double),
+        ("normwise_backward_error", C.c_double),
+        ("residual_norm", C.c_double),
+        ("factor_ms", C.c_double),
+        ("refine_ms", C.c_double),
+    ]
 
 # name -> (restype, argtypes); every symbol the header declares
 SIGNATURES = {
@@ -40,6 +63,14 @@ SIGNATURES = {
     "oxi_dlaswp": (C.c_int, [_vp, _i64, _vp, _i64, _i64, _i64, _vp, C.c_int]),
     "oxi_dgetrf": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, C.POINTER(_i64)]),
     "oxi_dgetrs": (C.c_int, [_vp, _ch, _i64, _i64, _vp, _i64, _vp, _vp, _i64]),
+    "oxi_dgesv_adjoint": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64]),
+    "oxi_sgemm": (C.c_int, [_vp, _ch, _ch, _i64, _i64, _i64, _flt, _vp, _i64, _vp, _i64, _flt, _vp, _i64]),
+    "oxi_gemm_lowp_update": (C.c_int, [_vp, C.c_int, _i64, _i64, _i64, _flt, _vp, _i64, _vp, _i64, _vp, _i64]),
+    "oxi_sgetrf_lowp": (C.c_int, [_vp, C.c_int, _i64, _i64, _vp, _i64, _vp, _i64, C.POINTER(_i64)]),
+    "oxi_sgetrs": (C.c_int, [_vp, _ch, _i64, _i64, _vp, _i64, _vp, _vp, _i64]),
+    "oxi_dgesv_mixed": (C.c_int, [_vp, C.c_int, C.c_int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, C.c_int32,
+                                  _dbl, C.c_int32, C.POINTER(IRInfo)]),
+    "oxi_dequilibrate": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, C.c_int]),
     "oxi_profile_enable": (C.c_int, [_vp, C.c_int]),
     "oxi_profile_read": (C.c_int, [_vp, C.POINTER(_dbl), C.POINTER(_dbl), C.POINTER(_i64)]),
 }

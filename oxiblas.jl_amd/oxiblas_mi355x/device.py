@@ -91,6 +91,18 @@ class DMatrix:
                 out.buf[: self.m * self.n].view(self.n, self.m).copy_(full.view(self.n, self.ld)[:, : self.m])
         return out
 
+    def strided(self):
+        """The (n, m) torch view whose row j is column j of this matrix."""
+        return self.buf.as_strided((self.n, self.m), (self.ld, 1), self.buf.storage_offset() + self.offset)
+
+    def copy_from(self, src):
+        """Device copy of src (same shape and dtype) into this matrix."""
+        if src.shape != self.shape:
+            raise ValueError("copy_from: shapes differ")
+        if self.m and self.n:
+            self.strided().copy_(src.strided())
+        return self
+
     @property
     def ptr(self):
         return self.buf.data_ptr() + self.offset * self.buf.element_size()

@@ -318,12 +318,14 @@ def ldiv_(*args):
         F, B = args
         return F.ldiv_(B)
     Y, F, B = args
-    X = F.solve(B)
     if isinstance(Y, DMatrix):
-        Xd = as_dmatrix(X)
-        gemm_(Y, Xd, DMatrix.from_numpy(np.eye(Xd.n)), 1.0, 0.0)  # device copy through the ABI
-        return Y
-    Y[...] = X
+        Bd = as_dmatrix(B)
+        if Y.shape != Bd.shape:
+            raise DimensionMismatch("ldiv!: Y and B differ in size")
+        if Y is not B:
+            Y.copy_from(Bd)
+        return F.ldiv_(Y)
+    Y[...] = F.solve(B)
     return Y
 
 

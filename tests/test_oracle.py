@@ -252,6 +252,45 @@ def test_equilibrate_radix_properties(rng):
     assert r2[3] == 1.0
 
 
+def exact_radix_scale(v):
+    """2^round(log2 v) with log2 correctly rounded to double (80-digit Decimal)
+    and ties to even, clamped to [exponent(floatmin), exponent(floatmax)]:
+    the closed form of equilibrate(A; radix=true)'s scale."""
+    from decimal import Decimal, localcontext
+    import math
+    with localcontext() as ctx:
+        ctx.prec = 80
+        L = float(Decimal(v).ln() / Decimal(2).ln())
+    return math.ldexp(1.0, min(1023, max(-1022, round(L))))
+
+
+def radix_boundary_values():
+    """Values whose log2 sits at or next to a half-integer (sqrt(2) 2^k and its
+    neighbours, where round-half-even decides), powers of two, subnormals."""
+    s = 1.4142135623730951
+    mant = (s, np.nextafter(s, 0.0), np.nextafter(s, 2.0), np.nextafter(np.nextafter(s, 0.0), 0.0), 1.0, 1.5,
+            np.nextafter(2.0, 0.0))
+    ks = list(range(-40, 41)) + [-1030, -1022, -1000, -700, 500, 1000, 1020, 1023]
+    vals = [m * 2.0 ** k for k in ks for m in mant if np.isfinite(m * 2.0 ** k) and m * 2.0 ** k > 0]
+    return np.array(vals + [5e-324, 1e-310, 2.2250738585072014e-308, 1.7976931348623157e308])
+
+
+def test_equilibrate_radix_rounding_is_exact():
+    v = radix_boundary_values()
+    _, r, c = O.dequilibrate(v.reshape(-1, 1), radix=True)
+    expect = np.array([exact_radix_scale(x) for x in v])
+    bad = np.nonzero(r != expect)[0]
+    assert bad.size == 0, [(v[i], r[i], expect[i]) for i in bad[:5]]
+    # a plausible mistake -- rounding log2 half away from zero -- is caught
+    from decimal import Decimal, localcontext, ROUND_HALF_UP
+    with localcontext() as ctx:
+        ctx.prec = 80
+        half_up = [int((Decimal(float(Decimal(x).ln() / Decimal(2).ln()))).quantize(Decimal(1), ROUND_HALF_UP))
+                   for x in v if x > 1e-300]
+    assert any(2.0 ** e != exact_radix_scale(x) for e, x in zip(half_up, [x for x in v if x > 1e-300]))
+    assert c[0] == exact_radix_scale(np.max(v / r))
+
+
 def test_refinement_residual(rng):
     A = rng.standard_normal((30, 30)); X = rng.standard_normal((30, 2)); B = rng.standard_normal((30, 2))
     R, e = O.refinement_residual(A, X, B)
