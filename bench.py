@@ -78,11 +78,11 @@ def mixed_extra(ox, torch, n, kappa, t_fp64, seed=0x0B1A5):
     A = ox.DMatrix(At.T.contiguous().reshape(-1), n, n)   # column-major A
     bd = ox.DMatrix(b.clone(), n, 1)
     out = {}
-    for low, solver in (("float32", "ir"), ("float16", "gmres"), ("bfloat16", "gmres")):
-        r = ox.mixed_solve(A, bd, lowtype=low, solver=solver, maxiter=50, gmres_restart=50)   # warm
+    for low, solver in (("float32", "ir"), ("float32", "gmres"), ("float16", "gmres")):
+        r = ox.mixed_solve(A, bd, lowtype=low, solver=solver, maxiter=20, gmres_restart=30)   # warm
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        r = ox.mixed_solve(A, bd, lowtype=low, solver=solver, maxiter=50, gmres_restart=50)
+        r = ox.mixed_solve(A, bd, lowtype=low, solver=solver, maxiter=20, gmres_restart=30)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         x = r.x.buf[:n]
@@ -144,6 +144,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--kappa", type=float, default=1e6)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (5 times): the known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -151,6 +154,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe:
+        nel = args.copy_probe * (1 << 20) // 8
+        src = torch.randn(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": nel * 8, "copy_probe_bytes_written": nel * 8, "launches": 5}))
+        return
     import oxiblas_mi355x as ox
     ox.load()
     n = args.n

@@ -30,6 +30,18 @@ constexpr int LOADS = BM * BK / NT;    // doubles per thread per operand per sla
 template <bool OUTER_CONTIG>
 __device__ __forceinline__ void load_slab(double (&r)[LOADS], const double* __restrict__ X, i64 ldx, i64 o0, i64 k0,
                                           i64 outer, i64 K, int t) {
+    if (o0 + BM <= outer && k0 + BK <= K) {
+        // interior slab: no bounds checks (the common case)
+#pragma unroll
+        for (int i = 0; i < LOADS; ++i) {
+            int o, kk;
+            if (OUTER_CONTIG) { o = t % BM; kk = t / BM + i * (NT / BM); }
+            else { kk = t % BK; o = t / BK + i * (NT / BK); }
+            const i64 go = o0 + o, gk = k0 + kk;
+            r[i] = OUTER_CONTIG ? X[go + gk * ldx] : X[gk + go * ldx];
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < LOADS; ++i) {
         int o, kk;

@@ -33,6 +33,7 @@
 
 #include <vector>
 #include <algorithm>
+#include <cstdlib>
 
 namespace {
 
@@ -92,29 +93,116 @@ __global__ __launch_bounds__(256) void transpose_square_kernel(T* __restrict__ A
 
 // ---------------------------------------------------------------------------
 // Row interchanges.
-// perm_pairs_kernel: one workgroup simulates the interchanges k1..k2 (0-based
-// steps of ipiv, 1-based rows) on the affected rows and emits (dst, src) pairs
-// of the net permutation: new[dst] = old[src].  Rows k1..k2 occupy slots
-// 0..cnt-1; pivot rows beyond k2 get extra slots through an LDS hash table.
+// perm_pairs_kernel: one workgroup turns the interchanges k1..k2 (0-based
+// steps of ipiv, 1-based rows) into (dst, src) pairs of the net permutation,
+// new[dst] = old[src].
+//
+// LU pivots satisfy p_s >= k1+s, and then the net permutation has a closed
+// form that needs no sequential replay: position k1+s is final after step s
+// and holds what position p_s held just before step s.  A position q >= k1+s
+// changes before step s only at steps u < s with p_u = q, where it receives
+// val(u), the content of position k1+u just before step u.  So with
+//   prev(t)  = the latest u < t with p_u = k1+t,
+//   same(s)  = the latest u < s with p_u = p_s,
+//   val(t)   = prev(t) >= 0 ? val(prev(t)) : k1+t     (pointer jumping),
+// position k1+s ends with same(s) >= 0 ? val(same(s)) : p_s, and a pivot row
+// q beyond k2 ends with val(u) for the last step u with p_u = q.  Other
+// interchange lists (and the reverse order) are replayed in order by one
+// thread in LDS.
 // ---------------------------------------------------------------------------
 constexpr int PERM_MAX_STEPS = 512;
 constexpr int HASH = 2048;
 
 __global__ __launch_bounds__(256) void perm_pairs_kernel(const i64* __restrict__ ipiv, i64 poff, i64 k1, int cnt,
                                                          int inverse, int* __restrict__ pairs /*[2*2*cnt+1]*/) {
-    __shared__ int slot_row[2 * PERM_MAX_STEPS];
-    __shared__ int slot_src[2 * PERM_MAX_STEPS];
+    __shared__ int pv[PERM_MAX_STEPS];     // 0-based pivot row of each step
+    __shared__ int prv[PERM_MAX_STEPS];    // prev(t)
+    __shared__ int sm[PERM_MAX_STEPS];     // same(s)
+    __shared__ int val[PERM_MAX_STEPS], nxt[PERM_MAX_STEPS];
+    __shared__ int islast[PERM_MAX_STEPS];
+    __shared__ int slot_row[2 * PERM_MAX_STEPS], slot_src[2 * PERM_MAX_STEPS];
     __shared__ int hkey[HASH], hval[HASH];
-    __shared__ int nslots;
-    const int t = threadIdx.x;
-    for (int i = t; i < HASH; i += blockDim.x) hkey[i] = -1;
-    for (int i = t; i < cnt; i += blockDim.x) { slot_row[i] = int(k1 + i); slot_src[i] = int(k1 + i); }
-    if (t == 0) nslots = cnt;
+    __shared__ int np, ordered, unresolved;
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int K1 = int(k1);
+    if (t == 0) { np = 0; ordered = 1; unresolved = 0; }
+    for (int i = t; i < cnt; i += nt) pv[i] = int(ipiv[k1 + i] - 1 - poff);
+    __syncthreads();
+    for (int i = t; i < cnt; i += nt) {
+        if (pv[i] < K1 + i) ordered = 0;
+        prv[i] = -1; islast[i] = 1;
+    }
+    __syncthreads();
+
+    if (!inverse && ordered) {
+        // prev(t): every u with p_u = k1+t has u <= t
+        for (int u = t; u < cnt; u += nt) {
+            const int w = pv[u] - K1;
+            if (w > u && w < cnt) atomicMax(&prv[w], u);
+        }
+        // same(s): previous occurrence of the same pivot row (broadcast reads)
+        for (int s = t; s < cnt; s += nt) {
+            const int ps = pv[s];
+            int b = -1;
+            for (int u = 0; u < s; ++u) b = (pv[u] == ps) ? u : b;
+            sm[s] = b;
+        }
+        __syncthreads();
+        for (int i = t; i < cnt; i += nt) {
+            val[i] = prv[i] < 0 ? K1 + i : -1;
+            nxt[i] = prv[i];
+            if (sm[i] >= 0) islast[sm[i]] = 0;
+        }
+        // val by pointer jumping (chains point to earlier steps)
+        for (int round = 0; round < 12; ++round) {
+            __syncthreads();
+            if (t == 0) unresolved = 0;
+            int nv[PERM_MAX_STEPS / 256], nn[PERM_MAX_STEPS / 256];
+            for (int k = 0, i = t; i < cnt; i += nt, ++k) {
+                nv[k] = val[i]; nn[k] = nxt[i];
+                if (nv[k] < 0) {
+                    const int r = nn[k];
+                    if (val[r] >= 0) { nv[k] = val[r]; nn[k] = -1; }
+                    else nn[k] = nxt[r];
+                }
+            }
+            __syncthreads();
+            for (int k = 0, i = t; i < cnt; i += nt, ++k) {
+                val[i] = nv[k]; nxt[i] = nn[k];
+                if (nv[k] < 0) unresolved = 1;
+            }
+            __syncthreads();
+            if (!unresolved) break;
+        }
+    }
+    if (!inverse && ordered && !unresolved) {
+        // pairs: the panel positions, then pivot rows beyond k2
+        for (int s = t; s < cnt; s += nt) {
+            const int src = sm[s] >= 0 ? val[sm[s]] : pv[s];
+            if (src != K1 + s) {
+                const int q = atomicAdd(&np, 1);
+                pairs[1 + 2 * q] = K1 + s; pairs[2 + 2 * q] = src;
+            }
+            if (pv[s] >= K1 + cnt && islast[s] && val[s] != pv[s]) {
+                const int q = atomicAdd(&np, 1);
+                pairs[1 + 2 * q] = pv[s]; pairs[2 + 2 * q] = val[s];
+            }
+        }
+        __syncthreads();
+        if (t == 0) pairs[0] = np;
+        return;
+    }
+
+    // general replay, also the safety net should a chain stay unresolved:
+    // rows k1..k2 occupy slots 0..cnt-1; other rows get slots
+    // through an LDS hash table
+    for (int i = t; i < HASH; i += nt) hkey[i] = -1;
+    for (int i = t; i < cnt; i += nt) { slot_row[i] = K1 + i; slot_src[i] = K1 + i; }
     __syncthreads();
     if (t == 0) {
         int ns = cnt;
         auto find = [&](int row) -> int {
-            if (row >= k1 && row < k1 + cnt) return int(row - k1);
+            if (row >= K1 && row < K1 + cnt) return row - K1;
             unsigned hsh = (unsigned(row) * 2654435761u) & (HASH - 1);
             while (true) {
                 if (hkey[hsh] == row) return hval[hsh];
@@ -128,21 +216,16 @@ __global__ __launch_bounds__(256) void perm_pairs_kernel(const i64* __restrict__
         };
         for (int s = 0; s < cnt; ++s) {
             const int step = inverse ? cnt - 1 - s : s;
-            const int j = int(k1 + step);
-            const int p = int(ipiv[k1 + step] - 1 - poff);
+            const int j = K1 + step;
+            const int p = pv[step];
             if (p == j) continue;
             const int a = find(j), b = find(p);
             const int tmp = slot_src[a]; slot_src[a] = slot_src[b]; slot_src[b] = tmp;
         }
-        nslots = ns;
-    }
-    __syncthreads();
-    // compact the pairs that move
-    if (t == 0) {
-        int np = 0;
-        for (int s = 0; s < nslots; ++s)
-            if (slot_src[s] != slot_row[s]) { pairs[1 + 2 * np] = slot_row[s]; pairs[2 + 2 * np] = slot_src[s]; ++np; }
-        pairs[0] = np;
+        int n2 = 0;
+        for (int q = 0; q < ns; ++q)
+            if (slot_src[q] != slot_row[q]) { pairs[1 + 2 * n2] = slot_row[q]; pairs[2 + 2 * n2] = slot_src[q]; ++n2; }
+        pairs[0] = n2;
     }
 }
 
@@ -306,17 +389,18 @@ int panel(GetrfCtx<T>& g, i64 c0, i64 np) {
     return OXI_OK;
 }
 
-// Trailing update A22 -= L21*U12, i.e. W[j1:n, j1:m] -= W[j1:n, j0:j1] * W[j0:j1, j1:m].
-int trailing(GetrfCtx<double>& g, i64 j0, i64 j1) {
-    return oxi_launch_dgemm(g.h, g.s, 0, 0, g.n - j1, g.m - j1, j1 - j0, -1.0, g.Wm + j1 + j0 * g.ldw, g.ldw,
-                            g.Wm + j0 + j1 * g.ldw, g.ldw, 1.0, g.Wm + j1 + j1 * g.ldw, g.ldw);
+// Trailing update of A columns [r0, r1) (W rows) with panel [j0, j1):
+// W[r0:r1, j1:m] -= W[r0:r1, j0:j1] * W[j0:j1, j1:m]  (A22 -= L21 * U12).
+int trailing_rows(GetrfCtx<double>& g, i64 j0, i64 j1, i64 r0, i64 r1) {
+    return oxi_launch_dgemm(g.h, g.s, 0, 0, r1 - r0, g.m - j1, j1 - j0, -1.0, g.Wm + r0 + j0 * g.ldw, g.ldw,
+                            g.Wm + j0 + j1 * g.ldw, g.ldw, 1.0, g.Wm + r0 + j1 * g.ldw, g.ldw);
 }
-int trailing(GetrfCtx<float>& g, i64 j0, i64 j1) {
+int trailing_rows(GetrfCtx<float>& g, i64 j0, i64 j1, i64 r0, i64 r1) {
     if (g.prec < 0)
-        return oxi_launch_sgemm(g.h, g.s, 0, 0, g.n - j1, g.m - j1, j1 - j0, -1.0f, g.Wm + j1 + j0 * g.ldw, g.ldw,
-                                g.Wm + j0 + j1 * g.ldw, g.ldw, 1.0f, g.Wm + j1 + j1 * g.ldw, g.ldw);
-    return oxi_lowp_update(g.h, g.s, g.prec, g.n - j1, g.m - j1, j1 - j0, -1.0f, g.Wm + j1 + j0 * g.ldw, g.ldw,
-                           g.Wm + j0 + j1 * g.ldw, g.ldw, g.Wm + j1 + j1 * g.ldw, g.ldw, g.lowp_ws, g.lowp_bytes);
+        return oxi_launch_sgemm(g.h, g.s, 0, 0, r1 - r0, g.m - j1, j1 - j0, -1.0f, g.Wm + r0 + j0 * g.ldw, g.ldw,
+                                g.Wm + j0 + j1 * g.ldw, g.ldw, 1.0f, g.Wm + r0 + j1 * g.ldw, g.ldw);
+    return oxi_lowp_update(g.h, g.s, g.prec, r1 - r0, g.m - j1, j1 - j0, -1.0f, g.Wm + r0 + j0 * g.ldw, g.ldw,
+                           g.Wm + j0 + j1 * g.ldw, g.ldw, g.Wm + r0 + j1 * g.ldw, g.ldw, g.lowp_ws, g.lowp_bytes);
 }
 
 int choose_nb(i64 m, i64 n) {
@@ -334,14 +418,16 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     if (nb == 0) nb = choose_nb(m, n);
     if (nb > PERM_MAX_STEPS) nb = PERM_MAX_STEPS;
 
-    // workspace: [W if rectangular][pub][pairs][bar][low-precision operands]
+    // workspace: [W if rectangular][pub][pairs main][pairs side][bar][low-precision operands]
     const bool square = (m == n);
     const i64 ldw = square ? lda : n;
     const size_t wbytes = square ? 0 : size_t(n) * size_t(m) * sizeof(T);
     const size_t pub_doubles = size_t(OXI_LEAF_W) * (size_t(256) + 1) * OXI_PUB_STRIDE + size_t(OXI_LEAF_W) * OXI_PUB_STRIDE;
+    const size_t pairs_bytes = (size_t(4 * PERM_MAX_STEPS + 4) * 4 + 255) & ~size_t(255);
     const size_t off_pub = (wbytes + 255) & ~size_t(255);
     const size_t off_pairs = off_pub + ((pub_doubles * 8 + 255) & ~size_t(255));
-    const size_t off_bar = off_pairs + ((size_t(4 * PERM_MAX_STEPS + 4) * 4 + 255) & ~size_t(255));
+    const size_t off_pairs2 = off_pairs + pairs_bytes;
+    const size_t off_bar = off_pairs2 + pairs_bytes;
     const size_t off_lowp = off_bar + 256;
     const size_t lowp_bytes = (sizeof(T) == 4 && prec >= 0) ? oxi_lowp_update_ws(prec, n, m, nb) : 0;
     const size_t total = off_lowp + lowp_bytes;
@@ -383,21 +469,60 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
                            lda, g.Wm, ldw, m, n);
     OXI_TRY(launch_check(h, "transpose"));
 
+    // Right-looking blocked LU with one panel of look-ahead: after panel j's
+    // interchanges reach the columns to its right, the next panel's columns
+    // are updated first and panel j+1 is factored on the side stream while
+    // the main stream updates the rest of the trailing matrix.  The
+    // interchanges of each panel on the columns to its left touch the L21
+    // block the concurrent update reads, so they are applied at the end, in
+    // panel order (the same per-column order as applying them at once).
+    static const bool lookahead = !getenv("OXIBLAS_NO_LOOKAHEAD");
+    GetrfCtx<T> gs = g;               // panel context on the side stream
+    gs.s = h->side;
+    gs.pairs = (int*)(base + off_pairs2);
+    bool panel_on_side = false;
+    OXI_TRY(panel(g, 0, std::min<i64>(nb, q)));
+    gs.bar_base = g.bar_base;
     for (i64 j0 = 0; j0 < q; j0 += nb) {
         const i64 jb = std::min<i64>(nb, q - j0);
         const i64 j1 = j0 + jb;
-        OXI_TRY(panel(g, j0, jb));
-        OXI_TRY(swap_w(g, j0, j1 - 1, 0, j0, 0));
+        if (panel_on_side) OXI_HIP(h, hipStreamWaitEvent(s, h->ev_b, 0));
         OXI_TRY(swap_w(g, j0, j1 - 1, j1, n, 0));
         if (j1 >= n) continue;
-        // U12 <- L11^-1 A12 : W[j1:n, j0:j1] <- W[j1:n, j0:j1] * (W[j0:j1, j0:j1])^-1 (upper unit)
-        OXI_TRY(trsm_ru(h, s, n - j1, jb, g.Wm + j0 + j0 * ldw, ldw, g.Wm + j1 + j0 * ldw, ldw));
-        if (j1 >= m) continue;
-        hipEvent_t pe = nullptr;
-        OXI_TRY(oxi_prof_begin(h, s, &pe));
-        OXI_TRY(trailing(g, j0, j1));
-        OXI_TRY(oxi_prof_end(h, s, pe, 2.0 * double(n - j1) * double(m - j1) * double(jb)));
+        const bool next = j1 < q;                         // then j1 < m and j1 < n
+        const i64 j2 = next && lookahead ? std::min<i64>(j1 + nb, q) : j1;
+        // look-ahead block: the next panel's columns [j1, j2)
+        if (j2 > j1) {
+            OXI_TRY(trsm_ru(h, s, j2 - j1, jb, g.Wm + j0 + j0 * ldw, ldw, g.Wm + j1 + j0 * ldw, ldw));
+            hipEvent_t pe = nullptr;
+            OXI_TRY(oxi_prof_begin(h, s, &pe));
+            OXI_TRY(trailing_rows(g, j0, j1, j1, j2));
+            OXI_TRY(oxi_prof_end(h, s, pe, 2.0 * double(j2 - j1) * double(m - j1) * double(jb)));
+            OXI_HIP(h, hipEventRecord(h->ev_a, s));
+            OXI_HIP(h, hipStreamWaitEvent(h->side, h->ev_a, 0));
+            OXI_TRY(panel(gs, j1, j2 - j1));
+            OXI_HIP(h, hipEventRecord(h->ev_b, h->side));
+            panel_on_side = true;
+        }
+        // the rest of the trailing matrix: columns [j2, n)
+        if (j2 < n) {
+            OXI_TRY(trsm_ru(h, s, n - j2, jb, g.Wm + j0 + j0 * ldw, ldw, g.Wm + j2 + j0 * ldw, ldw));
+            if (j1 < m) {
+                hipEvent_t pe = nullptr;
+                OXI_TRY(oxi_prof_begin(h, s, &pe));
+                OXI_TRY(trailing_rows(g, j0, j1, j2, n));
+                OXI_TRY(oxi_prof_end(h, s, pe, 2.0 * double(n - j2) * double(m - j1) * double(jb)));
+            }
+        }
+        if (next && !lookahead) {
+            OXI_TRY(panel(g, j1, std::min<i64>(nb, q - j1)));
+            panel_on_side = false;
+        }
+        g.bar_base = gs.bar_base = std::max(g.bar_base, gs.bar_base);
     }
+    if (panel_on_side) OXI_HIP(h, hipStreamWaitEvent(s, h->ev_b, 0));
+    // deferred interchanges on the columns left of each panel
+    for (i64 j0 = nb; j0 < q; j0 += nb) OXI_TRY(swap_w(g, j0, std::min<i64>(j0 + nb, q) - 1, 0, j0, 0));
 
     // W -> A
     if (square)

@@ -22,10 +22,33 @@ __device__ __forceinline__ bool better(double k1, long long r1, double k2, long 
     return k1 > k2 || (k1 == k2 && r1 < r2);
 }
 
+// Cross-workgroup exchange without agent-scope fences.  A release/acquire
+// pair at agent scope writes back and invalidates the whole L2 of the XCD,
+// which costs microseconds per pivot column while the trailing GEMM keeps
+// the L2 full of dirty lines.  Instead, the exchanged data itself uses
+// device-scope (agent) relaxed stores and loads, which are coherent across
+// XCDs without touching other lines; each wave waits for its stores to be
+// acknowledged (s_waitcnt 0) before the workgroup barrier that precedes the
+// counter increment, and readers load the data only after observing the
+// counter.
+__device__ __forceinline__ void put_agent(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double get_agent(const double* p) {
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Up to this many workgroups, every workgroup reads all published records
+// (key, row and the candidate's 32 values) in one pass after the barrier, so
+// the winner's row needs no second round trip.
+constexpr int LEAF_MERGE_P = 64;
+constexpr int REC_W = 3 + OXI_LEAF_W;   // doubles of a record that are read
+
 template <typename T, int LTT, int RPT>
 __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
     constexpr int LT = LTT;
     __shared__ T u_sh[OXI_LEAF_W], dg_sh[OXI_LEAF_W];
+    __shared__ double rec_sh[LEAF_MERGE_P * REC_W + OXI_LEAF_W];
     __shared__ double wkey[LT / 64];
     __shared__ long long wrow[LT / 64];
     __shared__ long long win_row;
@@ -91,14 +114,20 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
 #pragma unroll
         for (int s = 0; s < RPT; ++s) {
             if (lr[s] == lrow && lrow < a.mrows) {
-                rec[0] = lkey; rec[1] = __longlong_as_double(lrow); rec[2] = double(v[s][j]);
+                if (global_x) {
+                    put_agent(&rec[0], lkey); put_agent(&rec[1], __longlong_as_double(lrow));
 #pragma unroll
-                for (int c = 0; c < OXI_LEAF_W; ++c) rec[3 + c] = double(v[s][c]);
+                    for (int c = 0; c < OXI_LEAF_W; ++c) put_agent(&rec[3 + c], double(v[s][c]));
+                } else {
+                    rec[0] = lkey; rec[1] = __longlong_as_double(lrow);
+#pragma unroll
+                    for (int c = 0; c < OXI_LEAF_W; ++c) rec[3 + c] = double(v[s][c]);
+                }
             }
             if (p == 0 && lr[s] == j) {
                 if (global_x) {
 #pragma unroll
-                    for (int c = 0; c < OXI_LEAF_W; ++c) drow[c] = double(v[s][c]);
+                    for (int c = 0; c < OXI_LEAF_W; ++c) put_agent(&drow[c], double(v[s][c]));
                 } else {
 #pragma unroll
                     for (int c = 0; c < OXI_LEAF_W; ++c) dg_sh[c] = v[s][c];
@@ -106,28 +135,69 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
             }
         }
         if (lrow >= a.mrows && t == 0) {  // no candidate rows in this WG
-            rec[0] = -2.0; rec[1] = __longlong_as_double((long long)(1ll << 62)); rec[2] = 0.0;
+            if (global_x) {
+                put_agent(&rec[0], -2.0); put_agent(&rec[1], __longlong_as_double((long long)(1ll << 62)));
+            } else {
+                rec[0] = -2.0; rec[1] = __longlong_as_double((long long)(1ll << 62));
+            }
+        }
+        if (global_x) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_s_waitcnt(0);      // this wave's record stores are acknowledged
         }
         __syncthreads();
         // ---- device-wide exchange
         if (global_x) {
             if (t == 0) {
                 const unsigned target = a.base + unsigned(a.P) * unsigned(j + 1);
-                __hip_atomic_fetch_add(a.bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(a.bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 long spins = 0;
-                while (__hip_atomic_load(a.bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                while (__hip_atomic_load(a.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
                     if (++spins > (1L << 24)) { dead = 1; __hip_atomic_store(a.flags, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
                     __builtin_amdgcn_s_sleep(1);
                 }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             }
             __syncthreads();
-            if (!dead) {
+            if (!dead && a.P <= LEAF_MERGE_P) {
+                // all records and the diagonal row into LDS in one pass
+                const int nrec = a.P * REC_W;
+                for (int e = t; e < nrec + OXI_LEAF_W; e += LT) {
+                    if (e < nrec) {
+                        const int q = e / REC_W, f = e - q * REC_W;
+                        rec_sh[e] = get_agent(&a.pub[(size_t(j) * a.P + q) * OXI_PUB_STRIDE + f]);
+                    } else {
+                        rec_sh[e] = get_agent(&drow[e - nrec]);
+                    }
+                }
+                __syncthreads();
+                if (wave == 0) {
+                    double k = -3.0; long long r = (1ll << 62); int wg = 0;
+                    for (int q = lane; q < a.P; q += 64) {
+                        const double kq = rec_sh[q * REC_W];
+                        const long long rr = __double_as_longlong(rec_sh[q * REC_W + 1]);
+                        if (better(kq, rr, k, r)) { k = kq; r = rr; wg = q; }
+                    }
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) {
+                        double ok = __shfl_xor(k, off);
+                        long long orr = __shfl_xor(r, off);
+                        int ow = __shfl_xor(wg, off);
+                        if (better(ok, orr, k, r)) { k = ok; r = orr; wg = ow; }
+                    }
+                    if (lane == 0) win_row = r;
+                    if (lane < OXI_LEAF_W) {
+                        u_sh[lane] = T(rec_sh[wg * REC_W + 3 + lane]);
+                        dg_sh[lane] = T(rec_sh[nrec + lane]);
+                    }
+                }
+            } else if (!dead) {
                 // winner over all P records (wave 0)
                 if (wave == 0) {
                     double k = -3.0; long long r = (1ll << 62); int wg = 0;
                     for (int q = lane; q < a.P; q += 64) {
                         const double* rq = a.pub + (size_t(j) * a.P + q) * OXI_PUB_STRIDE;
-                        double kq = rq[0]; long long rr = __double_as_longlong(rq[1]);
+                        double kq = get_agent(&rq[0]); long long rr = __double_as_longlong(get_agent(&rq[1]));
                         if (better(kq, rr, k, r)) { k = kq; r = rr; wg = q; }
                     }
 #pragma unroll
@@ -141,7 +211,7 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
                 }
                 __syncthreads();
                 const double* rw = a.pub + (size_t(j) * a.P + win_wg) * OXI_PUB_STRIDE;
-                if (t < OXI_LEAF_W) { u_sh[t] = T(rw[3 + t]); dg_sh[t] = T(drow[t]); }
+                if (t < OXI_LEAF_W) { u_sh[t] = T(get_agent(&rw[3 + t])); dg_sh[t] = T(get_agent(&drow[t])); }
             }
             __syncthreads();
         } else {

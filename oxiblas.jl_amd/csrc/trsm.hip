@@ -16,46 +16,71 @@ constexpr int TRL = 64;
 // order matches the reference's right leaf (src/blas/level3.jl:319-333):
 // op(T) upper: x_c = (b_c - sum_{p<c} x_p U[p][c]) / U[c][c], c ascending;
 // op(T) lower: x_c = (b_c - sum_{p>c} x_p L[p][c]) / L[c][c], c descending.
-template <typename TY, bool LOWER, bool UNIT>
-__global__ __launch_bounds__(256) void trsm_right_leaf(TY* __restrict__ X, i64 ldx, i64 rows, int n,
-                                                       const TY* __restrict__ T, i64 ldt, int trans) {
-    __shared__ TY U[TRL * TRL];  // U[p*TRL + c] = op(T)[p][c]
-    for (int e = threadIdx.x; e < TRL * TRL; e += blockDim.x) {
-        const int pp = e / TRL, c = e % TRL;
-        TY val = TY(0);
-        if (pp < n && c < n) val = trans ? T[c + i64(pp) * ldt] : T[pp + i64(c) * ldt];
-        U[pp * TRL + c] = val;
-    }
-    __syncthreads();
+// The loops run as a column sweep (once x_c is final it is applied to every
+// later element), which performs each element's operations in exactly that
+// order but keeps up to 63 independent FMAs in flight.  The triangle is the
+// same for every lane, so its entries are read with wave-uniform loads (the
+// scalar path) straight from memory: no LDS staging and no LDS traffic.
+template <typename TY, bool LOWER, bool UNIT, bool TRANS, int NW>
+__global__ __launch_bounds__(64) void trsm_right_leaf(TY* __restrict__ X, i64 ldx, i64 rows, int n_rt,
+                                                      const TY* __restrict__ T, i64 ldt) {
+    // NW > 0: the width is the compile-time NW (no guards); NW == 0: runtime n <= TRL
+    const int n = NW > 0 ? NW : n_rt;
     const i64 r = i64(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= rows) return;
-    TY x[TRL];
+    // op(T)[p][c]
+    auto u = [&](int pp, int c) __attribute__((always_inline)) -> TY {
+        return TRANS ? T[c + i64(pp) * ldt] : T[pp + i64(c) * ldt];
+    };
+    constexpr int W = NW > 0 ? NW : TRL;
+    TY x[W];
 #pragma unroll
-    for (int c = 0; c < TRL; ++c) x[c] = c < n ? X[r + c * ldx] : TY(0);
+    for (int c = 0; c < W; ++c) x[c] = c < n ? X[r + c * ldx] : TY(0);
     if (!LOWER) {
 #pragma unroll
-        for (int c = 0; c < TRL; ++c) {
+        for (int c = 0; c < W; ++c) {
             if (c < n) {
-                TY s = x[c];
+                if (!UNIT) x[c] = x[c] / u(c, c);
+                const TY xc = x[c];
 #pragma unroll
-                for (int pp = 0; pp < c; ++pp) s = fma_t(-x[pp], U[pp * TRL + c], s);
-                x[c] = UNIT ? s : s / U[c * TRL + c];
+                for (int q = c + 1; q < W; ++q)
+                    if (q < n) x[q] = fma_t(-xc, u(c, q), x[q]);
             }
         }
     } else {
 #pragma unroll
-        for (int c = TRL - 1; c >= 0; --c) {
+        for (int c = W - 1; c >= 0; --c) {
             if (c < n) {
-                TY s = x[c];
+                if (!UNIT) x[c] = x[c] / u(c, c);
+                const TY xc = x[c];
 #pragma unroll
-                for (int pp = TRL - 1; pp > c; --pp)
-                    if (pp < n) s = fma_t(-x[pp], U[pp * TRL + c], s);
-                x[c] = UNIT ? s : s / U[c * TRL + c];
+                for (int q = c - 1; q >= 0; --q) x[q] = fma_t(-xc, u(c, q), x[q]);
             }
         }
     }
 #pragma unroll
-    for (int c = 0; c < TRL; ++c) if (c < n) X[r + c * ldx] = x[c];
+    for (int c = 0; c < W; ++c) if (c < n) X[r + c * ldx] = x[c];
+}
+
+template <typename TY, bool LOWER, bool UNIT, bool TRANS>
+void launch_right_leaf_w(hipStream_t s, i64 rows, int n, const TY* T, i64 ldt, TY* X, i64 ldx) {
+    // one wave per block: tall X spreads over many CUs
+    const dim3 grid(unsigned(cdiv(rows, 64))), block(64);
+    if (n == 64) hipLaunchKernelGGL((trsm_right_leaf<TY, LOWER, UNIT, TRANS, 64>), grid, block, 0, s, X, ldx, rows, n, T, ldt);
+    else if (n == 32) hipLaunchKernelGGL((trsm_right_leaf<TY, LOWER, UNIT, TRANS, 32>), grid, block, 0, s, X, ldx, rows, n, T, ldt);
+    else if (n == 16) hipLaunchKernelGGL((trsm_right_leaf<TY, LOWER, UNIT, TRANS, 16>), grid, block, 0, s, X, ldx, rows, n, T, ldt);
+    else hipLaunchKernelGGL((trsm_right_leaf<TY, LOWER, UNIT, TRANS, 0>), grid, block, 0, s, X, ldx, rows, n, T, ldt);
+}
+
+template <typename TY, bool LOWER>
+void launch_right_leaf(hipStream_t s, int unit, int trans, i64 rows, int n, const TY* T, i64 ldt, TY* X, i64 ldx) {
+    if (unit) {
+        if (trans) launch_right_leaf_w<TY, LOWER, true, true>(s, rows, n, T, ldt, X, ldx);
+        else launch_right_leaf_w<TY, LOWER, true, false>(s, rows, n, T, ldt, X, ldx);
+    } else {
+        if (trans) launch_right_leaf_w<TY, LOWER, false, true>(s, rows, n, T, ldt, X, ldx);
+        else launch_right_leaf_w<TY, LOWER, false, false>(s, rows, n, T, ldt, X, ldx);
+    }
 }
 
 // Left side: X (n x nrhs) <- op(T)^-1 X, one wave per right-hand side, lane =
@@ -131,9 +156,7 @@ static int trsm_right_upper(oxi_handle_t h, hipStream_t s, int trans, int unit, 
                             i64 ldt, TY* X, i64 ldx) {
     if (rows <= 0 || n <= 0) return OXI_OK;
     if (n <= TRL) {
-        dim3 grid(unsigned(cdiv(rows, 256)));
-        if (unit) hipLaunchKernelGGL((trsm_right_leaf<TY, false, true>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
-        else hipLaunchKernelGGL((trsm_right_leaf<TY, false, false>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
+        launch_right_leaf<TY, false>(s, unit, trans, rows, int(n), T, ldt, X, ldx);
         return launch_check(h, "trsm_right_leaf");
     }
     i64 hh = n / 2;
@@ -193,9 +216,7 @@ static int trsm_right_lower(oxi_handle_t h, hipStream_t s, int trans, int unit, 
                             i64 ldt, TY* X, i64 ldx) {
     if (rows <= 0 || n <= 0) return OXI_OK;
     if (n <= TRL) {
-        dim3 grid(unsigned(cdiv(rows, 256)));
-        if (unit) hipLaunchKernelGGL((trsm_right_leaf<TY, true, true>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
-        else hipLaunchKernelGGL((trsm_right_leaf<TY, true, false>), grid, dim3(256), 0, s, X, ldx, rows, int(n), T, ldt, trans);
+        launch_right_leaf<TY, true>(s, unit, trans, rows, int(n), T, ldt, X, ldx);
         return launch_check(h, "trsm_right_leaf");
     }
     i64 hh = n / 2;

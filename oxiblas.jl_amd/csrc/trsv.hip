@@ -13,7 +13,7 @@
 //      only grows: block b stores b+1 after its x is written),
 //   3. reduces the partial sums, and one wave solves the diagonal block by
 //      column sweep (lane i owns rows i and i+64; x_k broadcast by readlane),
-//   4. writes its x and releases `ready`.
+//   4. writes its x and then bumps `ready`.
 // The matrix is read once (HBM-bound: n^2/2 elements per triangle); the
 // dependence chain is the n column-sweep steps plus one publish per block.
 //
@@ -46,6 +46,19 @@ __device__ __forceinline__ float bcast(float v, int l) {
 }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// x is exchanged between workgroups with device-scope (agent) relaxed stores
+// and loads, which are coherent across XCDs; the publishing wave waits for
+// its stores to be acknowledged before it bumps `ready`.  (An agent-scope
+// release/acquire pair would write back and invalidate the L2 instead.)
+template <typename V>
+__device__ __forceinline__ void put_agent(V* p, V v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename V>
+__device__ __forceinline__ V get_agent(const V* p) {
+    return __hip_atomic_load(const_cast<V*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __global__ void trsv_ctrl_reset(TrsvCtrl* c) {
     if (threadIdx.x == 0) { c->ticket = 0; c->ready = 0; c->err = 0; c->pad = 0; }
@@ -97,7 +110,7 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
         if (c >= avail) {
             if (t == 0) {
                 unsigned v, spins = 0;
-                while ((v = __hip_atomic_load(&ctrl->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) <=
+                while ((v = __hip_atomic_load(&ctrl->ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) <=
                        unsigned(c)) {
                     if (__hip_atomic_load(&ctrl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
                         ++spins > SPIN_LIMIT) {
@@ -107,6 +120,7 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
                     }
                     __builtin_amdgcn_s_sleep(2);
                 }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 s_ready = int(v);
             }
             __syncthreads();
@@ -122,7 +136,7 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
 #pragma unroll 4
                 for (int q = cg; q < RB; q += CG) {
                     const i64 col = q0 + q;
-                    const TV xv = x[col];
+                    const TV xv = get_agent(&x[col]);
                     const TM* src = M + (p0 + pr) + col * ld;
                     if constexpr (VECOK) {
                         const typename Vec<TM>::type m = *reinterpret_cast<const typename Vec<TM>::type*>(src);
@@ -141,7 +155,7 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
                 const int jj = (it * LPR + u) * VEC;
                 TV xv[VEC];
 #pragma unroll
-                for (int v = 0; v < VEC; ++v) xv[v] = x[q0 + jj + v];
+                for (int v = 0; v < VEC; ++v) xv[v] = get_agent(&x[q0 + jj + v]);
 #pragma unroll
                 for (int r = 0; r < RPT; ++r) {
                     const int pr = rg + r * (NT / LPR);
@@ -202,10 +216,11 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int i = t + 64 * h;
-        if (i < rows) x[p0 + (BWD ? rows - 1 - i : i)] = UNIT ? s[h] : s[h] * rinv[i];
+        if (i < rows) put_agent(&x[p0 + (BWD ? rows - 1 - i : i)], UNIT ? s[h] : s[h] * rinv[i]);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (t == 0) __hip_atomic_store(&ctrl->ready, unsigned(b + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);   // x of this block acknowledged before `ready` moves
+    if (t == 0) __hip_atomic_store(&ctrl->ready, unsigned(b + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename TM, typename TV, bool TRANS, bool UNIT, bool BWD>

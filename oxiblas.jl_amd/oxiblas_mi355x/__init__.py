@@ -16,7 +16,7 @@ from .linalg import (  # noqa: F401
 from .mixed import (  # noqa: F401
     mixed_solve, equilibrate, sgemm_, gemm_lowp_update_, lu_lowp, LUFactor32, MixedSolveResult, precision_code,
 )
-from .adjoint import ldiv_pullback, rrule_ldiv, gemm_pullback, rrule_mul  # noqa: F401
+from .pullbacks import ldiv_pullback, rrule_ldiv, gemm_pullback, rrule_mul  # noqa: F401
 
 __all__ = [
     "load", "handle", "Handle", "OxiError", "DMatrix", "as_dmatrix",
