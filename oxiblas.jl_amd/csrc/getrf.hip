@@ -271,9 +271,6 @@ __global__ __launch_bounds__(256) void apply_pairs_cm_kernel(double* __restrict_
     for (int q = threadIdx.x; q < np; q += blockDim.x) a[pairs[1 + 2 * q] + roff] = buf[q];
 }
 
-__global__ void zero_u32_kernel(unsigned* p, int n) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
-}
 __global__ void zero_i64_kernel(i64* p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
@@ -303,7 +300,7 @@ struct GetrfCtx {
     i64 m, n;
     i64* ipiv;
     i64* dinfo;
-    unsigned* bar; unsigned bar_base;
+    unsigned long long tag0;   // (call sequence) << 32: tags of the leaf exchange records
     double* pub;
     int* pairs;
     int* flags;
@@ -356,13 +353,12 @@ int leaf(GetrfCtx<T>& g, i64 c0, int w) {
     if (sizeof(T) == 4 && mrows > 8192) kind = 2;
     const i64 R = kind == 0 ? 256 : kind == 1 ? 512 : 1024;
     const i64 P = cdiv(mrows, R);
-    if (P > std::min(g.h->num_cus, 256))
+    if (P > std::min<i64>(g.h->num_cus, OXI_LEAF_MAX_P))
         return oxi_set_error(g.h, OXI_ERR_UNSUPPORTED, "panel of %lld rows exceeds the leaf kernel capacity",
                              (long long)mrows);
-    LeafArgs a{(void*)g.Wm, g.ldw, c0, w, mrows, g.ipiv, g.dinfo, g.bar, g.bar_base, g.pub, int(P), g.flags};
+    LeafArgs a{(void*)g.Wm, g.ldw, c0, w, mrows, g.ipiv, g.dinfo, g.tag0, g.pub, int(P), g.flags};
     if (sizeof(T) == 8) OXI_TRY(oxi_launch_lu_leaf(g.h, g.s, a, kind));
     else OXI_TRY(oxi_launch_lu_leaf32(g.h, g.s, a, kind));
-    if (P > 1) g.bar_base += unsigned(P) * unsigned(w);
     return OXI_OK;
 }
 
@@ -418,7 +414,7 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     if (nb == 0) nb = choose_nb(m, n);
     if (nb > PERM_MAX_STEPS) nb = PERM_MAX_STEPS;
 
-    // workspace: [W if rectangular][pub][pairs main][pairs side][bar][low-precision operands]
+    // workspace: [W if rectangular][pub][pairs main][pairs side][low-precision operands]
     const bool square = (m == n);
     const i64 ldw = square ? lda : n;
     const size_t wbytes = square ? 0 : size_t(n) * size_t(m) * sizeof(T);
@@ -427,8 +423,7 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     const size_t off_pub = (wbytes + 255) & ~size_t(255);
     const size_t off_pairs = off_pub + ((pub_doubles * 8 + 255) & ~size_t(255));
     const size_t off_pairs2 = off_pairs + pairs_bytes;
-    const size_t off_bar = off_pairs2 + pairs_bytes;
-    const size_t off_lowp = off_bar + 256;
+    const size_t off_lowp = off_pairs2 + pairs_bytes;
     const size_t lowp_bytes = (sizeof(T) == 4 && prec >= 0) ? oxi_lowp_update_ws(prec, n, m, nb) : 0;
     const size_t total = off_lowp + lowp_bytes;
     void* ws;
@@ -440,7 +435,7 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     g.m = m; g.n = n; g.ipiv = ipiv; g.dinfo = h->dev_info;
     g.pub = (double*)(base + off_pub);
     g.pairs = (int*)(base + off_pairs);
-    g.bar = (unsigned*)(base + off_bar); g.bar_base = 0;
+    g.tag0 = (++h->getrf_seq) << 32;
     g.flags = h->dev_flags;
     g.prec = prec;
     g.lowp_ws = base + off_lowp; g.lowp_bytes = lowp_bytes;
@@ -454,7 +449,6 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr_set = true;
     }
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, g.bar, 64);
     hipLaunchKernelGGL(zero_i64_kernel, dim3(1), dim3(64), 0, s, g.dinfo, 1);
     hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, s, g.flags, 1);
     OXI_TRY(launch_check(h, "zero"));
@@ -482,7 +476,6 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     gs.pairs = (int*)(base + off_pairs2);
     bool panel_on_side = false;
     OXI_TRY(panel(g, 0, std::min<i64>(nb, q)));
-    gs.bar_base = g.bar_base;
     for (i64 j0 = 0; j0 < q; j0 += nb) {
         const i64 jb = std::min<i64>(nb, q - j0);
         const i64 j1 = j0 + jb;
@@ -518,7 +511,6 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
             OXI_TRY(panel(g, j1, std::min<i64>(nb, q - j1)));
             panel_on_side = false;
         }
-        g.bar_base = gs.bar_base = std::max(g.bar_base, gs.bar_base);
     }
     if (panel_on_side) OXI_HIP(h, hipStreamWaitEvent(s, h->ev_b, 0));
     // deferred interchanges on the columns left of each panel

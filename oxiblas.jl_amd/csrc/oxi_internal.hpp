@@ -38,6 +38,7 @@ struct oxi_handle_s {
     double prof_gemm_ms = 0.0, prof_gemm_flops = 0.0;
     i64 prof_gemm_launches = 0;
     int num_cus = 256;
+    unsigned long long getrf_seq = 0;   // getrf calls on this handle (tags the leaf exchange)
 };
 
 int oxi_set_error(oxi_handle_t h, int code, const char* fmt, ...);
@@ -118,6 +119,7 @@ int oxi_first_zero_diag(oxi_handle_t h, hipStream_t s, const double* T, i64 ldt,
 // ---------------------------------------------------------------------------
 constexpr int OXI_LEAF_W = 32;      // maximum leaf width
 constexpr int OXI_PUB_STRIDE = 48;  // doubles per published (column, workgroup) record
+constexpr int OXI_LEAF_MAX_P = 127; // workgroups of one leaf (LDS holds P + 1 records per column)
 
 struct LeafArgs {
     void* Wm; i64 ldw;   // double* or float*
@@ -126,8 +128,7 @@ struct LeafArgs {
     i64 mrows;       // rows c0 .. c0+mrows-1 of A
     i64* ipiv;       // global pivots (1-based rows), entries c0..c0+w-1 written
     i64* info;       // device info (first zero pivot, 1-based), 0 if none
-    unsigned* bar;   // barrier counter (monotonic over one getrf call)
-    unsigned base;   // counter value at launch
+    unsigned long long tag0;  // (getrf call sequence) << 32; column c's records carry tag0 + c
     double* pub;     // published records [w][P][PUB_STRIDE] + diag rows [w][PUB_STRIDE]
     int P;
     int* flags;      // flags[0] = 1 on barrier timeout

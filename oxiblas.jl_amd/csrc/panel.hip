@@ -12,11 +12,11 @@
 // Operands live in the transposed working layout W = A^T (see getrf.hip):
 // the w leaf values of one row of A are contiguous.  Each workgroup owns a
 // contiguous block of rows (one or two per thread, in registers).  Per column
-// every workgroup publishes its candidate (key, row and the whole row) and
-// meets the others at a device-scope counter barrier (release/acquire at
-// agent scope); every workgroup then reads all candidates and picks the same
-// winner.  The spin is bounded: on timeout a flag is raised and every wave
-// leaves, so the grid always drains.  With P == 1 the exchange stays in LDS.
+// every workgroup publishes its candidate (key, row and the whole row) as a
+// self-validating record (tag + checksum, panel_kernel.hpp) and reads all
+// the others; every workgroup picks the same winner.  The wait is bounded:
+// on timeout a flag is raised and every wave leaves, so the grid always
+// drains.  With P == 1 the exchange stays in LDS.
 #include "panel_kernel.hpp"
 
 int oxi_launch_lu_leaf(oxi_handle_t h, hipStream_t s, const LeafArgs& a, int kind) {

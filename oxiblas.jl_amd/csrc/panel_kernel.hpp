@@ -1,6 +1,6 @@
-// Panel leaf kernel template (see panel.hip for the algorithm); included by
-// panel.hip (FP64) and panel32.hip (FP32, the working precision of the
-// mixed-precision LU).
+// Panel leaf kernel template (the algorithm is described in panel.hip);
+// included by panel.hip (FP64) and panel32.hip (FP32, the working precision
+// of the mixed-precision LU).
 #pragma once
 #include "oxi_internal.hpp"
 
@@ -8,6 +8,8 @@
 #include <type_traits>
 
 namespace {
+
+typedef unsigned long long u64;
 
 // Pivot key with the reference's strict '>' scan semantics: NaN in the
 // diagonal position wins (nothing compares greater), NaN elsewhere never wins.
@@ -22,44 +24,68 @@ __device__ __forceinline__ bool better(double k1, long long r1, double k2, long 
     return k1 > k2 || (k1 == k2 && r1 < r2);
 }
 
-// Cross-workgroup exchange without agent-scope fences.  A release/acquire
-// pair at agent scope writes back and invalidates the whole L2 of the XCD,
-// which costs microseconds per pivot column while the trailing GEMM keeps
-// the L2 full of dirty lines.  Instead, the exchanged data itself uses
-// device-scope (agent) relaxed stores and loads, which are coherent across
-// XCDs without touching other lines; each wave waits for its stores to be
-// acknowledged (s_waitcnt 0) before the workgroup barrier that precedes the
-// counter increment, and readers load the data only after observing the
-// counter.
+// ---------------------------------------------------------------------------
+// Cross-workgroup exchange of one pivot column: self-validating records.
+//
+// Every workgroup publishes one record per column: its best candidate (key,
+// row) and that row's w values, plus a tag (getrf call and global column) and
+// a checksum over all other fields keyed by the tag.  Every workgroup then
+// loads all records and accepts a record only when its checksum matches what
+// the fields and the expected tag give; it reloads the others until all are
+// valid.  Nothing orders the writer's stores: a record that is still partly
+// old (an earlier column or call) fails its checksum and is simply read
+// again, so one round trip suffices once the last record has landed.  The
+// data goes through device-scope (agent) relaxed stores and loads, which are
+// coherent across XCDs without the L2 write-back / invalidate that an
+// agent-scope release / acquire pair costs.
+// ---------------------------------------------------------------------------
+constexpr int REC_N = 4 + OXI_LEAF_W;           // key, row, tag, 32 values, checksum
+constexpr int REC_CHK = REC_N - 1;
+constexpr int REC_SLOTS = OXI_LEAF_MAX_P + 1;   // records per column: P candidates + the diagonal row
+
 __device__ __forceinline__ void put_agent(double* p, double v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ double get_agent(const double* p) {
     return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ u64 bits(double v) { return __builtin_bit_cast(u64, v); }
+__device__ __forceinline__ double from_bits(u64 v) { return __builtin_bit_cast(double, v); }
+// field f's share of the checksum (fields 0 .. REC_N-2)
+__device__ __forceinline__ u64 rec_term(u64 b, u64 tag, int f) {
+    return (b ^ (tag * 0x9E3779B97F4A7C15ull + u64(f + 1) * 0xD1B54A32D192ED03ull)) * 0x94D049BB133111EBull;
+}
 
-// Up to this many workgroups, every workgroup reads all published records
-// (key, row and the candidate's 32 values) in one pass after the barrier, so
-// the winner's row needs no second round trip.
-constexpr int LEAF_MERGE_P = 64;
-constexpr int REC_W = 3 + OXI_LEAF_W;   // doubles of a record that are read
+// Publish a record: fields 0..REC_N-2 from vals(f) (field 2 is the tag), then
+// the checksum.
+template <typename F>
+__device__ __forceinline__ void publish(double* rec, u64 tag, F vals) {
+    u64 h = 0;
+#pragma unroll
+    for (int f = 0; f < REC_CHK; ++f) {
+        const double v = f == 2 ? from_bits(tag) : vals(f);
+        h ^= rec_term(bits(v), tag, f);
+        put_agent(&rec[f], v);
+    }
+    put_agent(&rec[REC_CHK], from_bits(h));
+}
 
 template <typename T, int LTT, int RPT>
 __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
-    constexpr int LT = LTT;
-    __shared__ T u_sh[OXI_LEAF_W], dg_sh[OXI_LEAF_W];
-    __shared__ double rec_sh[LEAF_MERGE_P * REC_W + OXI_LEAF_W];
-    __shared__ double wkey[LT / 64];
-    __shared__ long long wrow[LT / 64];
-    __shared__ long long win_row;
-    __shared__ int win_wg, dead;
-    __shared__ double lds_rec[OXI_PUB_STRIDE];  // P == 1 exchange
+    constexpr int LT = LTT, NWV = LT / 64;
+    constexpr int RPW = 16;                           // records one wave validates per pass (4 lanes each)
+    constexpr int PASS = NWV * RPW;
+    constexpr int NPASS_MAX = (REC_SLOTS + PASS - 1) / PASS;
+    __shared__ double wkey[2][NWV];
+    __shared__ long long wrow[2][NWV];
+    __shared__ double rec_sh[2][REC_SLOTS * REC_N];   // by column parity
+    __shared__ int dead;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int p = blockIdx.x;
     const i64 R = i64(LT) * RPT;
     const int w = a.w;
     const bool global_x = a.P > 1;
-    // a barrier of an earlier leaf timed out: the factorization is already
+    // a wait of an earlier leaf timed out: the factorization is already
     // reported as failed, leave at once
     if (__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
 
@@ -77,12 +103,14 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
     if (t == 0) dead = 0;
     __syncthreads();
 
-    // one elimination step per leaf column; the column index is a
-    // compile-time constant so that v[][] stays in registers
+    // One elimination step per leaf column.  The column index is a
+    // compile-time constant so that v[][] stays in registers and each step
+    // touches only the columns right of its pivot.
     auto step = [&](auto jc) __attribute__((always_inline)) {
         constexpr int j = decltype(jc)::value;
-        if (j < w && !dead) {
-        // ---- local candidate
+        if (j >= w || dead) return;
+        const int par = j & 1;
+        // ---- local candidate: waves, then every thread over the waves
         double bk = -2.0; long long br = (1ll << 62);
 #pragma unroll
         for (int s = 0; s < RPT; ++s) {
@@ -97,162 +125,145 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
             long long orr = __shfl_xor(br, off);
             if (better(ok, orr, bk, br)) { bk = ok; br = orr; }
         }
-        if (lane == 0) { wkey[wave] = bk; wrow[wave] = br; }
+        if (lane == 0) { wkey[par][wave] = bk; wrow[par][wave] = br; }
         __syncthreads();
-        if (t == 0) {
-            double k = wkey[0]; long long r = wrow[0];
-            for (int q = 1; q < LT / 64; ++q)
-                if (better(wkey[q], wrow[q], k, r)) { k = wkey[q]; r = wrow[q]; }
-            wkey[0] = k; wrow[0] = r;
-        }
-        __syncthreads();
-        const double lkey = wkey[0];
-        const long long lrow = wrow[0];
-        // ---- publish: record + candidate row; WG 0 also publishes diag row j
-        double* rec = global_x ? a.pub + (size_t(j) * a.P + p) * OXI_PUB_STRIDE : lds_rec;
-        double* drow = a.pub + (size_t(w) * a.P + j) * OXI_PUB_STRIDE;
+        double lkey = wkey[par][0];
+        long long lrow = wrow[par][0];
 #pragma unroll
-        for (int s = 0; s < RPT; ++s) {
-            if (lr[s] == lrow && lrow < a.mrows) {
-                if (global_x) {
-                    put_agent(&rec[0], lkey); put_agent(&rec[1], __longlong_as_double(lrow));
+        for (int q = 1; q < NWV; ++q)
+            if (better(wkey[par][q], wrow[par][q], lkey, lrow)) { lkey = wkey[par][q]; lrow = wrow[par][q]; }
+
+        // ---- exchange: the winner's row (up) and row j (dp), as doubles in LDS
+        double* rs = rec_sh[par];
+        const double* up;
+        const double* dp;
+        long long piv;
+        if (!global_x) {
+            // one workgroup: the candidate is the winner
 #pragma unroll
-                    for (int c = 0; c < OXI_LEAF_W; ++c) put_agent(&rec[3 + c], double(v[s][c]));
-                } else {
-                    rec[0] = lkey; rec[1] = __longlong_as_double(lrow);
+            for (int s = 0; s < RPT; ++s) {
+                if (lr[s] == lrow && lrow < a.mrows)
 #pragma unroll
-                    for (int c = 0; c < OXI_LEAF_W; ++c) rec[3 + c] = double(v[s][c]);
-                }
-            }
-            if (p == 0 && lr[s] == j) {
-                if (global_x) {
+                    for (int c = 0; c < OXI_LEAF_W; ++c) rs[3 + c] = double(v[s][c]);
+                if (lr[s] == j)
 #pragma unroll
-                    for (int c = 0; c < OXI_LEAF_W; ++c) put_agent(&drow[c], double(v[s][c]));
-                } else {
-#pragma unroll
-                    for (int c = 0; c < OXI_LEAF_W; ++c) dg_sh[c] = v[s][c];
-                }
-            }
-        }
-        if (lrow >= a.mrows && t == 0) {  // no candidate rows in this WG
-            if (global_x) {
-                put_agent(&rec[0], -2.0); put_agent(&rec[1], __longlong_as_double((long long)(1ll << 62)));
-            } else {
-                rec[0] = -2.0; rec[1] = __longlong_as_double((long long)(1ll << 62));
-            }
-        }
-        if (global_x) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_s_waitcnt(0);      // this wave's record stores are acknowledged
-        }
-        __syncthreads();
-        // ---- device-wide exchange
-        if (global_x) {
-            if (t == 0) {
-                const unsigned target = a.base + unsigned(a.P) * unsigned(j + 1);
-                __hip_atomic_fetch_add(a.bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                long spins = 0;
-                while (__hip_atomic_load(a.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                    if (++spins > (1L << 24)) { dead = 1; __hip_atomic_store(a.flags, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    for (int c = 0; c < OXI_LEAF_W; ++c) rs[REC_N + 3 + c] = double(v[s][c]);
             }
             __syncthreads();
-            if (!dead && a.P <= LEAF_MERGE_P) {
-                // all records and the diagonal row into LDS in one pass
-                const int nrec = a.P * REC_W;
-                for (int e = t; e < nrec + OXI_LEAF_W; e += LT) {
-                    if (e < nrec) {
-                        const int q = e / REC_W, f = e - q * REC_W;
-                        rec_sh[e] = get_agent(&a.pub[(size_t(j) * a.P + q) * OXI_PUB_STRIDE + f]);
-                    } else {
-                        rec_sh[e] = get_agent(&drow[e - nrec]);
-                    }
-                }
-                __syncthreads();
-                if (wave == 0) {
-                    double k = -3.0; long long r = (1ll << 62); int wg = 0;
-                    for (int q = lane; q < a.P; q += 64) {
-                        const double kq = rec_sh[q * REC_W];
-                        const long long rr = __double_as_longlong(rec_sh[q * REC_W + 1]);
-                        if (better(kq, rr, k, r)) { k = kq; r = rr; wg = q; }
-                    }
-#pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) {
-                        double ok = __shfl_xor(k, off);
-                        long long orr = __shfl_xor(r, off);
-                        int ow = __shfl_xor(wg, off);
-                        if (better(ok, orr, k, r)) { k = ok; r = orr; wg = ow; }
-                    }
-                    if (lane == 0) win_row = r;
-                    if (lane < OXI_LEAF_W) {
-                        u_sh[lane] = T(rec_sh[wg * REC_W + 3 + lane]);
-                        dg_sh[lane] = T(rec_sh[nrec + lane]);
-                    }
-                }
-            } else if (!dead) {
-                // winner over all P records (wave 0)
-                if (wave == 0) {
-                    double k = -3.0; long long r = (1ll << 62); int wg = 0;
-                    for (int q = lane; q < a.P; q += 64) {
-                        const double* rq = a.pub + (size_t(j) * a.P + q) * OXI_PUB_STRIDE;
-                        double kq = get_agent(&rq[0]); long long rr = __double_as_longlong(get_agent(&rq[1]));
-                        if (better(kq, rr, k, r)) { k = kq; r = rr; wg = q; }
-                    }
-#pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) {
-                        double ok = __shfl_xor(k, off);
-                        long long orr = __shfl_xor(r, off);
-                        int ow = __shfl_xor(wg, off);
-                        if (better(ok, orr, k, r)) { k = ok; r = orr; wg = ow; }
-                    }
-                    if (lane == 0) { win_row = r; win_wg = wg; }
-                }
-                __syncthreads();
-                const double* rw = a.pub + (size_t(j) * a.P + win_wg) * OXI_PUB_STRIDE;
-                if (t < OXI_LEAF_W) { u_sh[t] = T(get_agent(&rw[3 + t])); dg_sh[t] = T(get_agent(&drow[t])); }
-            }
-            __syncthreads();
+            piv = lrow;
+            up = rs + 3;
+            dp = rs + REC_N + 3;
         } else {
-            if (t == 0) win_row = lrow;
-            if (t < OXI_LEAF_W) u_sh[t] = T(lds_rec[3 + t]);
-            __syncthreads();
-        }
-        if (!dead) {
-            const long long piv = win_row;
-            if (p == 0 && t == 0) a.ipiv[a.c0 + j] = a.c0 + piv + 1;
-            // ---- swap rows j and piv across the leaf
-            if (piv != j) {
+            const u64 tag = a.tag0 + u64(a.c0 + j);
+            double* rec = a.pub + (size_t(j) * a.P + p) * OXI_PUB_STRIDE;
+            double* drec = a.pub + (size_t(w) * a.P + j) * OXI_PUB_STRIDE;
 #pragma unroll
-                for (int s = 0; s < RPT; ++s) {
-                    if (lr[s] == j) {
+            for (int s = 0; s < RPT; ++s) {
+                if (lr[s] == lrow && lrow < a.mrows)
+                    publish(rec, tag, [&](int f) {
+                        return f == 0 ? lkey : f == 1 ? __longlong_as_double(lrow) : double(v[s][f - 3]);
+                    });
+                if (p == 0 && lr[s] == j)
+                    publish(drec, tag, [&](int f) { return f < 3 ? 0.0 : double(v[s][f - 3]); });
+            }
+            if (lrow >= a.mrows && t == 0)   // no candidate rows in this workgroup
+                publish(rec, tag, [&](int f) {
+                    return f == 0 ? -2.0 : f == 1 ? __longlong_as_double((long long)(1ll << 62)) : 0.0;
+                });
+            // gather: record q (q < P candidates, q == P row j) by lanes 4i..4i+3 of
+            // one wave; lane sub holds fields sub, sub+4, ..., sub+32
+            const int P1 = a.P + 1, npass = (P1 + PASS - 1) / PASS;
+            const int sub = lane & 3;
+            bool okp[NPASS_MAX];
 #pragma unroll
-                        for (int c = 0; c < OXI_LEAF_W; ++c) v[s][c] = u_sh[c];
-                    } else if (lr[s] == piv) {
+            for (int ps = 0; ps < NPASS_MAX; ++ps) okp[ps] = false;
+            unsigned spins = 0;
+            while (true) {
+                bool all_ok = true;
 #pragma unroll
-                        for (int c = 0; c < OXI_LEAF_W; ++c) v[s][c] = dg_sh[c];
+                for (int ps = 0; ps < NPASS_MAX; ++ps) {
+                    const int q = ps * PASS + wave * RPW + (lane >> 2);
+                    if (ps < npass && q < P1 && !okp[ps]) {
+                        const double* src = q < a.P ? a.pub + (size_t(j) * a.P + q) * OXI_PUB_STRIDE : drec;
+                        double fv[REC_N / 4];
+                        u64 hx = 0;
+#pragma unroll
+                        for (int i = 0; i < REC_N / 4; ++i) {
+                            const int f = sub + 4 * i;
+                            fv[i] = get_agent(&src[f]);
+                            if (f < REC_CHK) hx ^= rec_term(bits(fv[i]), tag, f);
+                        }
+                        hx ^= __shfl_xor(hx, 1);
+                        hx ^= __shfl_xor(hx, 2);
+                        // the checksum sits in lane sub 3 (field 35), the tag in sub 2 (field 2)
+                        const u64 chk = bits(__shfl(fv[REC_N / 4 - 1], (lane & ~3) | 3));
+                        const u64 tg = bits(__shfl(fv[0], (lane & ~3) | 2));
+                        if (hx == chk && tg == tag) {
+                            okp[ps] = true;
+#pragma unroll
+                            for (int i = 0; i < REC_N / 4; ++i) rs[q * REC_N + sub + 4 * i] = fv[i];
+                        }
                     }
+                    if (ps < npass && q < P1 && !okp[ps]) all_ok = false;
+                }
+                if (__syncthreads_and(all_ok)) break;
+                if (++spins > (1u << 22)) {
+                    if (t == 0) __hip_atomic_store(a.flags, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    dead = 1;   // every thread writes the same value: uniform exit
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (spins > (1u << 22)) return;
+            // the winner over the P candidates, in every wave
+            double k = -3.0; long long r = (1ll << 62); int wq = 0;
+            for (int q = lane; q < a.P; q += 64) {
+                const double kq = rs[q * REC_N];
+                const long long rq = __double_as_longlong(rs[q * REC_N + 1]);
+                if (better(kq, rq, k, r)) { k = kq; r = rq; wq = q; }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ok = __shfl_xor(k, off);
+                const long long orr = __shfl_xor(r, off);
+                const int ow = __shfl_xor(wq, off);
+                if (better(ok, orr, k, r)) { k = ok; r = orr; wq = ow; }
+            }
+            piv = r;
+            up = rs + wq * REC_N + 3;
+            dp = rs + a.P * REC_N + 3;
+        }
+
+        // ---- interchange, scale, rank-1 update
+        if (p == 0 && t == 0) a.ipiv[a.c0 + j] = a.c0 + piv + 1;
+        if (piv != j) {
+#pragma unroll
+            for (int s = 0; s < RPT; ++s) {
+                if (lr[s] == j) {
+#pragma unroll
+                    for (int c = 0; c < OXI_LEAF_W; ++c) v[s][c] = T(up[c]);
+                } else if (lr[s] == piv) {
+#pragma unroll
+                    for (int c = 0; c < OXI_LEAF_W; ++c) v[s][c] = T(dp[c]);
                 }
             }
-            const T d = u_sh[j];
-            if (d == T(0)) {
-                if (p == 0 && t == 0 && *a.info == 0) *a.info = a.c0 + j + 1;
-            } else {
-                // ---- scale and rank-1 update of the rows below the pivot
+        }
+        const T d = T(up[j]);
+        if (d == T(0)) {
+            if (p == 0 && t == 0 && *a.info == 0) *a.info = a.c0 + j + 1;
+        } else {
+            T ucol[OXI_LEAF_W];
 #pragma unroll
-                for (int s = 0; s < RPT; ++s) {
-                    if (lr[s] > j && lr[s] < a.mrows) {
-                        const T l = v[s][j] / d;
-                        v[s][j] = l;
+            for (int c = j + 1; c < OXI_LEAF_W; ++c) ucol[c] = T(up[c]);
 #pragma unroll
-                        for (int c = j + 1; c < OXI_LEAF_W; ++c)
-                            v[s][c] = fma_t(-l, u_sh[c], v[s][c]);
-                    }
+            for (int s = 0; s < RPT; ++s) {
+                if (lr[s] > j && lr[s] < a.mrows) {
+                    const T l = v[s][j] / d;
+                    v[s][j] = l;
+#pragma unroll
+                    for (int c = j + 1; c < OXI_LEAF_W; ++c) v[s][c] = fma_t(-l, ucol[c], v[s][c]);
                 }
             }
-        }
-        __syncthreads();
         }
     };
     [&]<int... J>(std::integer_sequence<int, J...>) __attribute__((always_inline)) {

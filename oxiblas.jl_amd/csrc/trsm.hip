@@ -12,64 +12,107 @@ __device__ __forceinline__ float fma_t(float a, float b, float c) { return __bui
 
 constexpr int TRL = 64;
 
-// Right side: X (rows x n) <- X * op(T)^-1, one thread per row of X.  Element
-// order matches the reference's right leaf (src/blas/level3.jl:319-333):
+// Right side: X (rows x n) <- X * op(T)^-1, n <= 64.  Element order matches
+// the reference's right leaf (src/blas/level3.jl:319-333):
 // op(T) upper: x_c = (b_c - sum_{p<c} x_p U[p][c]) / U[c][c], c ascending;
 // op(T) lower: x_c = (b_c - sum_{p>c} x_p L[p][c]) / L[c][c], c descending.
-// The loops run as a column sweep (once x_c is final it is applied to every
-// later element), which performs each element's operations in exactly that
-// order but keeps up to 63 independent FMAs in flight.  The triangle is the
-// same for every lane, so its entries are read with wave-uniform loads (the
-// scalar path) straight from memory: no LDS staging and no LDS traffic.
+// One wave per row, lane q = column q: lane q keeps column q of op(T) in
+// registers, and the solve is a column sweep (step c: lane c finishes x_c,
+// the wave reads it with readlane, every later lane applies one FMA), so each
+// element sees exactly the operations above, in that order.  A wave takes
+// `rpw` consecutive rows in turn.
+__device__ __forceinline__ double rlane(double v, int l) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane(unsigned(u), l);
+    const unsigned hi = __builtin_amdgcn_readlane(unsigned(u >> 32), l);
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+__device__ __forceinline__ float rlane(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(unsigned, v), l));
+}
+
 template <typename TY, bool LOWER, bool UNIT, bool TRANS, int NW>
-__global__ __launch_bounds__(64) void trsm_right_leaf(TY* __restrict__ X, i64 ldx, i64 rows, int n_rt,
-                                                      const TY* __restrict__ T, i64 ldt) {
-    // NW > 0: the width is the compile-time NW (no guards); NW == 0: runtime n <= TRL
-    const int n = NW > 0 ? NW : n_rt;
-    const i64 r = i64(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
-    // op(T)[p][c]
-    auto u = [&](int pp, int c) __attribute__((always_inline)) -> TY {
-        return TRANS ? T[c + i64(pp) * ldt] : T[pp + i64(c) * ldt];
-    };
-    constexpr int W = NW > 0 ? NW : TRL;
-    TY x[W];
+__global__ __launch_bounds__(256) void trsm_right_leaf(TY* __restrict__ X, i64 ldx, i64 rows, int rpw,
+                                                       const TY* __restrict__ T, i64 ldt) {
+    const int lane = threadIdx.x & 63;
+    const i64 wave = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const i64 r0 = wave * rpw;
+    if (r0 >= rows) return;
+    const int q = lane < NW ? lane : NW - 1;          // lanes beyond the width mirror the last column
+    TY u[NW];                                         // u[c] = op(T)[c][q]
 #pragma unroll
-    for (int c = 0; c < W; ++c) x[c] = c < n ? X[r + c * ldx] : TY(0);
-    if (!LOWER) {
+    for (int c = 0; c < NW; ++c) u[c] = TRANS ? T[q + i64(c) * ldt] : T[c + i64(q) * ldt];
+    const i64 r1 = min(rows, r0 + rpw);
+    for (i64 r = r0; r < r1; ++r) {
+        TY x = X[r + i64(q) * ldx];
+        if (!LOWER) {
 #pragma unroll
-        for (int c = 0; c < W; ++c) {
-            if (c < n) {
-                if (!UNIT) x[c] = x[c] / u(c, c);
-                const TY xc = x[c];
+            for (int c = 0; c < NW; ++c) {
+                if (!UNIT && lane == c) x = x / u[c];
+                const TY xc = rlane(x, c);
+                if (lane > c) x = fma_t(-xc, u[c], x);
+            }
+        } else {
 #pragma unroll
-                for (int q = c + 1; q < W; ++q)
-                    if (q < n) x[q] = fma_t(-xc, u(c, q), x[q]);
+            for (int c = NW - 1; c >= 0; --c) {
+                if (!UNIT && lane == c) x = x / u[c];
+                const TY xc = rlane(x, c);
+                if (lane < c) x = fma_t(-xc, u[c], x);
             }
         }
-    } else {
-#pragma unroll
-        for (int c = W - 1; c >= 0; --c) {
-            if (c < n) {
-                if (!UNIT) x[c] = x[c] / u(c, c);
-                const TY xc = x[c];
-#pragma unroll
-                for (int q = c - 1; q >= 0; --q) x[q] = fma_t(-xc, u(c, q), x[q]);
-            }
-        }
+        if (lane < NW) X[r + i64(q) * ldx] = x;
     }
+}
+
+// Any width n <= 64: the width-64 layout with the steps past n skipped.
+template <typename TY, bool LOWER, bool UNIT, bool TRANS>
+__global__ __launch_bounds__(256) void trsm_right_leaf_any(TY* __restrict__ X, i64 ldx, i64 rows, int rpw, int n,
+                                                           const TY* __restrict__ T, i64 ldt) {
+    const int lane = threadIdx.x & 63;
+    const i64 wave = i64(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const i64 r0 = wave * rpw;
+    if (r0 >= rows) return;
+    const int q = lane < n ? lane : n - 1;
+    TY u[TRL];
 #pragma unroll
-    for (int c = 0; c < W; ++c) if (c < n) X[r + c * ldx] = x[c];
+    for (int c = 0; c < TRL; ++c) u[c] = c < n ? (TRANS ? T[q + i64(c) * ldt] : T[c + i64(q) * ldt]) : TY(0);
+    const i64 r1 = min(rows, r0 + rpw);
+    for (i64 r = r0; r < r1; ++r) {
+        TY x = X[r + i64(q) * ldx];
+        if (!LOWER) {
+#pragma unroll
+            for (int c = 0; c < TRL; ++c) {
+                if (c < n) {
+                    if (!UNIT && lane == c) x = x / u[c];
+                    const TY xc = rlane(x, c);
+                    if (lane > c) x = fma_t(-xc, u[c], x);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int c = TRL - 1; c >= 0; --c) {
+                if (c < n) {
+                    if (!UNIT && lane == c) x = x / u[c];
+                    const TY xc = rlane(x, c);
+                    if (lane < c) x = fma_t(-xc, u[c], x);
+                }
+            }
+        }
+        if (lane < n) X[r + i64(q) * ldx] = x;
+    }
 }
 
 template <typename TY, bool LOWER, bool UNIT, bool TRANS>
 void launch_right_leaf_w(hipStream_t s, i64 rows, int n, const TY* T, i64 ldt, TY* X, i64 ldx) {
-    // one wave per block: tall X spreads over many CUs
-    const dim3 grid(unsigned(cdiv(rows, 64))), block(64);
-    if (n == 64) hipLaunchKernelGGL((trsm_right_leaf<TY, LOWER, UNIT, TRANS, 64>), grid, block, 0, s, X, ldx, rows, n, T, ldt);
-    else if (n == 32) hipLaunchKernelGGL((trsm_right_leaf<TY, LOWER, UNIT, TRANS, 32>), grid, block, 0, s, X, ldx, rows, n, T, ldt);
-    else if (n == 16) hipLaunchKernelGGL((trsm_right_leaf<TY, LOWER, UNIT, TRANS, 16>), grid, block, 0, s, X, ldx, rows, n, T, ldt);
-    else hipLaunchKernelGGL((trsm_right_leaf<TY, LOWER, UNIT, TRANS, 0>), grid, block, 0, s, X, ldx, rows, n, T, ldt);
+    // about 1024 waves in flight; consecutive rows per wave share cache lines
+    const int rpw = int(std::max<i64>(1, cdiv(rows, 1024)));
+    const i64 waves = cdiv(rows, rpw);
+    const dim3 grid(unsigned(cdiv(waves, 4))), block(256);
+    if (n == 64) hipLaunchKernelGGL((trsm_right_leaf<TY, LOWER, UNIT, TRANS, 64>), grid, block, 0, s, X, ldx, rows, rpw, T, ldt);
+    else if (n == 32) hipLaunchKernelGGL((trsm_right_leaf<TY, LOWER, UNIT, TRANS, 32>), grid, block, 0, s, X, ldx, rows, rpw, T, ldt);
+    else if (n == 16) hipLaunchKernelGGL((trsm_right_leaf<TY, LOWER, UNIT, TRANS, 16>), grid, block, 0, s, X, ldx, rows, rpw, T, ldt);
+    else if (n == 8) hipLaunchKernelGGL((trsm_right_leaf<TY, LOWER, UNIT, TRANS, 8>), grid, block, 0, s, X, ldx, rows, rpw, T, ldt);
+    else hipLaunchKernelGGL((trsm_right_leaf_any<TY, LOWER, UNIT, TRANS>), grid, block, 0, s, X, ldx, rows, rpw, n, T, ldt);
 }
 
 template <typename TY, bool LOWER>

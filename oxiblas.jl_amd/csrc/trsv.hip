@@ -27,6 +27,7 @@
 namespace {
 
 constexpr int RB = 128, NT = 512;
+constexpr int XB = 8;                          // published chunks whose x is fetched in one round
 constexpr unsigned SPIN_LIMIT = 1u << 20;
 
 struct TrsvCtrl { unsigned ticket, ready, err, pad; };
@@ -77,6 +78,7 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
     __shared__ TM D[RB * (RB + 1)];            // D[k][i] = T(i, k), logical in-block indices
     __shared__ TV red[NRED][RB];
     __shared__ TV rinv[RB];
+    __shared__ TV xs[XB * RB];                 // x of the chunks being applied
     __shared__ int s_b, s_ready;
 
     const int t = threadIdx.x;
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
 #pragma unroll
     for (int r = 0; r < (TRANS ? RPT : VEC); ++r) acc[r] = TV(0);
     int avail = 0;
-    for (int c = 0; c < b; ++c) {
+    for (int c = 0; c < b;) {
         if (c >= avail) {
             if (t == 0) {
                 unsigned v, spins = 0;
@@ -128,50 +130,63 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
             __syncthreads();
             if (avail <= c) return;            // timed out: uniform exit
         }
-        const i64 q0 = BWD ? n - i64(c + 1) * RB : i64(c) * RB;   // chunk = full block
-        if constexpr (!TRANS) {
-            // T = M: block rows (contiguous) x chunk columns
-            const int pr = (t % LPC) * VEC, cg = t / LPC;
-            if (pr < rows) {
+        // x of up to XB published chunks into LDS with one round of
+        // device-scope loads; xs[k * RB + e] = element e (physical order) of chunk c + k
+        const int nbat = min(min(avail, b) - c, XB);
+        for (int e = t; e < nbat * RB; e += NT) {
+            const int k = e / RB, ee = e - k * RB;
+            const i64 qk = BWD ? n - i64(c + k + 1) * RB : i64(c + k) * RB;
+            xs[e] = get_agent(&x[qk + ee]);
+        }
+        __syncthreads();
+        for (int k = 0; k < nbat; ++k) {
+            const i64 q0 = BWD ? n - i64(c + k + 1) * RB : i64(c + k) * RB;   // chunk = full block
+            const TV* xc = xs + k * RB;
+            if constexpr (!TRANS) {
+                // T = M: block rows (contiguous) x chunk columns
+                const int pr = (t % LPC) * VEC, cg = t / LPC;
+                if (pr < rows) {
 #pragma unroll 4
-                for (int q = cg; q < RB; q += CG) {
-                    const i64 col = q0 + q;
-                    const TV xv = get_agent(&x[col]);
-                    const TM* src = M + (p0 + pr) + col * ld;
-                    if constexpr (VECOK) {
-                        const typename Vec<TM>::type m = *reinterpret_cast<const typename Vec<TM>::type*>(src);
-#pragma unroll
-                        for (int v = 0; v < VEC; ++v) acc[v] = fma_t(TV(m[v]), xv, acc[v]);
-                    } else {
-                        acc[0] = fma_t(TV(src[0]), xv, acc[0]);
-                    }
-                }
-            }
-        } else {
-            // T = M^T: block rows are columns of M, chunk indices are rows of M (contiguous)
-            const int u = t % LPR, rg = t / LPR;
-#pragma unroll 2
-            for (int it = 0; it < ITS; ++it) {
-                const int jj = (it * LPR + u) * VEC;
-                TV xv[VEC];
-#pragma unroll
-                for (int v = 0; v < VEC; ++v) xv[v] = get_agent(&x[q0 + jj + v]);
-#pragma unroll
-                for (int r = 0; r < RPT; ++r) {
-                    const int pr = rg + r * (NT / LPR);
-                    if (pr < rows) {
-                        const TM* src = M + (q0 + jj) + (p0 + pr) * ld;
+                    for (int q = cg; q < RB; q += CG) {
+                        const TV xv = xc[q];
+                        const TM* src = M + (p0 + pr) + (q0 + q) * ld;
                         if constexpr (VECOK) {
                             const typename Vec<TM>::type m = *reinterpret_cast<const typename Vec<TM>::type*>(src);
 #pragma unroll
-                            for (int v = 0; v < VEC; ++v) acc[r] = fma_t(TV(m[v]), xv[v], acc[r]);
+                            for (int v = 0; v < VEC; ++v) acc[v] = fma_t(TV(m[v]), xv, acc[v]);
                         } else {
-                            acc[r] = fma_t(TV(src[0]), xv[0], acc[r]);
+                            acc[0] = fma_t(TV(src[0]), xv, acc[0]);
+                        }
+                    }
+                }
+            } else {
+                // T = M^T: block rows are columns of M, chunk indices are rows of M (contiguous)
+                const int u = t % LPR, rg = t / LPR;
+#pragma unroll 2
+                for (int it = 0; it < ITS; ++it) {
+                    const int jj = (it * LPR + u) * VEC;
+                    TV xv[VEC];
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) xv[v] = xc[jj + v];
+#pragma unroll
+                    for (int r = 0; r < RPT; ++r) {
+                        const int pr = rg + r * (NT / LPR);
+                        if (pr < rows) {
+                            const TM* src = M + (q0 + jj) + (p0 + pr) * ld;
+                            if constexpr (VECOK) {
+                                const typename Vec<TM>::type m = *reinterpret_cast<const typename Vec<TM>::type*>(src);
+#pragma unroll
+                                for (int v = 0; v < VEC; ++v) acc[r] = fma_t(TV(m[v]), xv[v], acc[r]);
+                            } else {
+                                acc[r] = fma_t(TV(src[0]), xv[0], acc[r]);
+                            }
                         }
                     }
                 }
             }
         }
+        __syncthreads();                        // xs is refilled next
+        c += nbat;
     }
 
     // 3. partial sums -> LDS (fixed order: deterministic)
