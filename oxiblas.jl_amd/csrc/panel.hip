@@ -17,6 +17,11 @@
 // the others; every workgroup picks the same winner.  The wait is bounded:
 // on timeout a flag is raised and every wave leaves, so the grid always
 // drains.  With P == 1 the exchange stays in LDS.
+//
+// Row interchanges act on the registers (columns j and right of it) at once;
+// the multipliers left of column j are already in memory and receive the
+// interchanges of their later steps from a second, small kernel, which gives
+// the same rows as swapping whole rows at each step.
 #include "panel_kernel.hpp"
 
 int oxi_launch_lu_leaf(oxi_handle_t h, hipStream_t s, const LeafArgs& a, int kind) {
@@ -24,5 +29,12 @@ int oxi_launch_lu_leaf(oxi_handle_t h, hipStream_t s, const LeafArgs& a, int kin
     else hipLaunchKernelGGL((lu_leaf_kernel<double, 256, 2>), dim3(unsigned(a.P)), dim3(256), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "lu_leaf launch: %s", hipGetErrorString(e));
+    // the leaf's interchanges on its stored multipliers
+    if (a.w > 1) {
+        hipLaunchKernelGGL(leaf_lswap_kernel<double>, dim3(unsigned(a.w - 1)), dim3(64), 0, s, (double*)a.Wm, a.ldw, a.c0, a.w,
+                           (const i64*)a.ipiv, (const int*)a.flags);
+        e = hipGetLastError();
+        if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "leaf_lswap launch: %s", hipGetErrorString(e));
+    }
     return OXI_OK;
 }

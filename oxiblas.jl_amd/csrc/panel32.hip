@@ -10,5 +10,12 @@ int oxi_launch_lu_leaf32(oxi_handle_t h, hipStream_t s, const LeafArgs& a, int k
     else hipLaunchKernelGGL((lu_leaf_kernel<float, 256, 4>), dim3(unsigned(a.P)), dim3(256), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "lu_leaf32 launch: %s", hipGetErrorString(e));
+    // the leaf's interchanges on its stored multipliers
+    if (a.w > 1) {
+        hipLaunchKernelGGL(leaf_lswap_kernel<float>, dim3(unsigned(a.w - 1)), dim3(64), 0, s, (float*)a.Wm, a.ldw, a.c0, a.w,
+                           (const i64*)a.ipiv, (const int*)a.flags);
+        e = hipGetLastError();
+        if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "leaf_lswap launch: %s", hipGetErrorString(e));
+    }
     return OXI_OK;
 }

@@ -42,6 +42,7 @@ __device__ __forceinline__ bool better(double k1, long long r1, double k2, long 
 constexpr int REC_N = 4 + OXI_LEAF_W;           // key, row, tag, 32 values, checksum
 constexpr int REC_CHK = REC_N - 1;
 constexpr int REC_SLOTS = OXI_LEAF_MAX_P + 1;   // records per column: P candidates + the diagonal row
+static_assert(REC_N == 36, "load9_agent reads 9 fields per lane");
 
 __device__ __forceinline__ void put_agent(double* p, double v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -51,35 +52,124 @@ __device__ __forceinline__ double get_agent(const double* p) {
 }
 __device__ __forceinline__ u64 bits(double v) { return __builtin_bit_cast(u64, v); }
 __device__ __forceinline__ double from_bits(u64 v) { return __builtin_bit_cast(double, v); }
+// Nine device-scope loads 32 bytes apart (one lane's fields of a record),
+// issued back to back and waited for once.  Relaxed atomic loads written in
+// C++ are issued one at a time, each waited for before the next.
+__device__ __forceinline__ void load9_agent(const double* p, double (&f)[9]) {
+    asm volatile(
+        "global_load_dwordx2 %0, %9, off sc1\n\t"
+        "global_load_dwordx2 %1, %9, off offset:32 sc1\n\t"
+        "global_load_dwordx2 %2, %9, off offset:64 sc1\n\t"
+        "global_load_dwordx2 %3, %9, off offset:96 sc1\n\t"
+        "global_load_dwordx2 %4, %9, off offset:128 sc1\n\t"
+        "global_load_dwordx2 %5, %9, off offset:160 sc1\n\t"
+        "global_load_dwordx2 %6, %9, off offset:192 sc1\n\t"
+        "global_load_dwordx2 %7, %9, off offset:224 sc1\n\t"
+        "global_load_dwordx2 %8, %9, off offset:256 sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(f[0]), "=&v"(f[1]), "=&v"(f[2]), "=&v"(f[3]), "=&v"(f[4]), "=&v"(f[5]), "=&v"(f[6]), "=&v"(f[7]),
+          "=&v"(f[8])
+        : "v"(p)
+        : "memory");
+}
 // field f's share of the checksum (fields 0 .. REC_N-2)
 __device__ __forceinline__ u64 rec_term(u64 b, u64 tag, int f) {
     return (b ^ (tag * 0x9E3779B97F4A7C15ull + u64(f + 1) * 0xD1B54A32D192ED03ull)) * 0x94D049BB133111EBull;
 }
 
-// Publish a record: fields 0..REC_N-2 from vals(f) (field 2 is the tag), then
-// the checksum.
+// ---------------------------------------------------------------------------
+// Cross-lane helpers (DPP: data moves inside the VALU, no LDS round trip).
+// ---------------------------------------------------------------------------
+template <int CTRL, int ROWM = 0xf, int BANKM = 0xf>
+__device__ __forceinline__ u64 dpp64(u64 x) {
+    // lanes whose source is outside the row (or masked off) keep their own value
+    const int lo = __builtin_amdgcn_update_dpp(int(unsigned(x)), int(unsigned(x)), CTRL, ROWM, BANKM, false);
+    const int hi = __builtin_amdgcn_update_dpp(int(unsigned(x >> 32)), int(unsigned(x >> 32)), CTRL, ROWM, BANKM, false);
+    return (u64(unsigned(hi)) << 32) | unsigned(lo);
+}
+template <int CTRL, int ROWM = 0xf, int BANKM = 0xf>
+__device__ __forceinline__ double dppd(double x) { return from_bits(dpp64<CTRL, ROWM, BANKM>(bits(x))); }
+__device__ __forceinline__ u64 readlane64(u64 x, int l) {
+    const unsigned lo = __builtin_amdgcn_readlane(unsigned(x), l), hi = __builtin_amdgcn_readlane(unsigned(x >> 32), l);
+    return (u64(hi) << 32) | lo;
+}
+// maximum over the 64 lanes of a wave (all lanes active, no NaN), in every lane:
+// inclusive prefix maxima inside each row of 16, then across rows to lane 63
+__device__ __forceinline__ double wave_max(double x) {
+    x = __builtin_fmax(x, dppd<0x111>(x));         // row_shr:1
+    x = __builtin_fmax(x, dppd<0x112>(x));         // row_shr:2
+    x = __builtin_fmax(x, dppd<0x114>(x));         // row_shr:4
+    x = __builtin_fmax(x, dppd<0x118>(x));         // row_shr:8
+    x = __builtin_fmax(x, dppd<0x142, 0xa>(x));    // row_bcast:15 into rows 1, 3
+    x = __builtin_fmax(x, dppd<0x143, 0xc>(x));    // row_bcast:31 into rows 2, 3
+    return from_bits(readlane64(bits(x), 63));
+}
+// XOR over the 16 lanes of each row, in every lane of the row
+__device__ __forceinline__ u64 row_xor16(u64 h) {
+    h ^= dpp64<0x128>(h);   // row_ror:8
+    h ^= dpp64<0x124>(h);   // row_ror:4
+    h ^= dpp64<0x122>(h);   // row_ror:2
+    h ^= dpp64<0x121>(h);   // row_ror:1
+    return h;
+}
+// XOR over each quad of lanes, and a quad member's value, in every lane of the quad
+__device__ __forceinline__ u64 quad_xor(u64 h) {
+    h ^= dpp64<0xB1>(h);    // quad_perm [1,0,3,2]
+    h ^= dpp64<0x4E>(h);    // quad_perm [2,3,0,1]
+    return h;
+}
+template <int SEL>
+__device__ __forceinline__ u64 quad_bcast(u64 x) { return dpp64<SEL * 0x55>(x); }   // quad_perm [SEL x 4]
+__device__ __forceinline__ int first_lane(u64 m) { return int(__builtin_ctzll(m)); }
+
+// Publish one record by the 16 lanes of a row of a wave: lane L stores fields
+// L, L + 16 and L + 32 (the checksum is field 35 = lane 3's third) with
+// device-scope stores; the checksum is reduced across the row.
 template <typename F>
-__device__ __forceinline__ void publish(double* rec, u64 tag, F vals) {
+__device__ __forceinline__ void publish16(double* rec, u64 tag, int L, F field) {
+    double fv[3];
     u64 h = 0;
 #pragma unroll
-    for (int f = 0; f < REC_CHK; ++f) {
-        const double v = f == 2 ? from_bits(tag) : vals(f);
-        h ^= rec_term(bits(v), tag, f);
-        put_agent(&rec[f], v);
+    for (int i = 0; i < 3; ++i) {
+        const int f = L + 16 * i;
+        fv[i] = f < REC_CHK ? (f == 2 ? from_bits(tag) : field(f)) : 0.0;
+        if (f < REC_CHK) h ^= rec_term(bits(fv[i]), tag, f);
     }
-    put_agent(&rec[REC_CHK], from_bits(h));
+    h = row_xor16(h);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int f = L + 16 * i;
+        if (f < REC_CHK) put_agent(&rec[f], fv[i]);
+        else if (f == REC_CHK) put_agent(&rec[f], from_bits(h));
+    }
 }
 
+// ---------------------------------------------------------------------------
+// The leaf kernel.  Thread t of workgroup p owns the row at position
+// p*R + s*LT + t (s < RPT) in registers.  The registers of a row rotate by
+// one per column: at column j register c holds column j + c, so the pivot
+// column is always register 0 and the rank-1 update is the same 31 FMAs at
+// every step (v[c] <- fma(-l, u[c+1], v[c+1])).  The column loop is rolled
+// and every register index is a compile-time constant.
+//
+// Values leave the registers as they become final: the multiplier of column
+// j goes to memory at step j (at the row's position at that time), and the
+// pivot row at position j goes to memory at step j (later interchanges only
+// involve positions below j).  Interchanges of later steps are applied to
+// the stored multipliers afterwards by leaf_lswap_kernel, as getrf2 applies
+// the right half's interchanges to the left half.
+// ---------------------------------------------------------------------------
 template <typename T, int LTT, int RPT>
 __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
-    constexpr int LT = LTT, NWV = LT / 64;
+    constexpr int LT = LTT, NWV = LT / 64, W = OXI_LEAF_W;
     constexpr int RPW = 16;                           // records one wave validates per pass (4 lanes each)
     constexpr int PASS = NWV * RPW;
     constexpr int NPASS_MAX = (REC_SLOTS + PASS - 1) / PASS;
+    constexpr unsigned SPIN_LIMIT = 1u << 22;
     __shared__ double wkey[2][NWV];
     __shared__ long long wrow[2][NWV];
-    __shared__ double rec_sh[2][REC_SLOTS * REC_N];   // by column parity
-    __shared__ int dead;
+    __shared__ double wvals[2][NWV + 1][W];           // each wave's best row; [NWV]: position j
+    __shared__ double rec_sh[2][REC_SLOTS * REC_N];   // gathered records, by column parity
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int p = blockIdx.x;
     const i64 R = i64(LT) * RPT;
@@ -89,89 +179,91 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
     // reported as failed, leave at once
     if (__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
 
-    T v[RPT][OXI_LEAF_W];
+    T v[RPT][W];
     T* Wm = (T*)a.Wm;
     i64 lr[RPT];
+    T* wrow_of[RPT];   // this position's row of W (leaf column 0)
 #pragma unroll
     for (int s = 0; s < RPT; ++s) {
         lr[s] = i64(p) * R + i64(s) * LT + t;
         const bool ok = lr[s] < a.mrows;
-        const T* src = Wm + a.c0 + (a.c0 + lr[s]) * a.ldw;
+        wrow_of[s] = Wm + a.c0 + (a.c0 + lr[s]) * a.ldw;
 #pragma unroll
-        for (int c = 0; c < OXI_LEAF_W; ++c) v[s][c] = (ok && c < w) ? src[c] : T(0);
+        for (int c = 0; c < W; ++c) v[s][c] = (ok && c < w) ? wrow_of[s][c] : T(0);
     }
-    if (t == 0) dead = 0;
-    __syncthreads();
+    // the rows have arrived before the loop: otherwise the first use inside a
+    // conditional block leaves a wait on all vector memory operations (the
+    // stores of the previous column included) at the top of every column
+    __builtin_amdgcn_s_waitcnt(0);
 
-    // One elimination step per leaf column.  The column index is a
-    // compile-time constant so that v[][] stays in registers and each step
-    // touches only the columns right of its pivot.
-    auto step = [&](auto jc) __attribute__((always_inline)) {
-        constexpr int j = decltype(jc)::value;
-        if (j >= w || dead) return;
+#pragma unroll 1
+    for (int j0 = 0; j0 < w; ++j0) {
+        const int j = __builtin_amdgcn_readfirstlane(j0);
         const int par = j & 1;
-        // ---- local candidate: waves, then every thread over the waves
-        double bk = -2.0; long long br = (1ll << 62);
+        // ---- local candidate: per thread, then the wave (DPP maximum, ties
+        // to the lowest row through ballots), then over the waves in LDS
+        double bk = -2.0; long long br = (1ll << 62); int bs = -1;
 #pragma unroll
         for (int s = 0; s < RPT; ++s) {
             if (lr[s] < a.mrows && lr[s] >= j) {
-                double k = pivot_key(double(v[s][j]), lr[s] == j);
-                if (better(k, lr[s], bk, br)) { bk = k; br = lr[s]; }
+                const double k = pivot_key(double(v[s][0]), lr[s] == j);
+                if (better(k, lr[s], bk, br)) { bk = k; br = lr[s]; bs = s; }
             }
         }
+        const double wk = wave_max(bk);
+        // rows of slot s increase with the lane, and every row of slot s is
+        // below every row of slot s + 1
+        int wl = 0, ws = -1;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            double ok = __shfl_xor(bk, off);
-            long long orr = __shfl_xor(br, off);
-            if (better(ok, orr, bk, br)) { bk = ok; br = orr; }
+        for (int s = RPT - 1; s >= 0; --s) {
+            const u64 m = __ballot(bk == wk && bs == s);
+            if (m) { wl = first_lane(m); ws = s; }
         }
-        if (lane == 0) { wkey[par][wave] = bk; wrow[par][wave] = br; }
+        const long long wr = (long long)readlane64(u64(br), wl);
+#pragma unroll
+        for (int s = 0; s < RPT; ++s) {
+            if (lane == wl && ws == s)
+#pragma unroll
+                for (int c = 0; c < W; ++c) wvals[par][wave][c] = double(v[s][c]);
+            if (lr[s] == j)
+#pragma unroll
+                for (int c = 0; c < W; ++c) wvals[par][NWV][c] = double(v[s][c]);
+        }
+        if (lane == 0) { wkey[par][wave] = wk; wrow[par][wave] = ws >= 0 ? wr : (1ll << 62); }
         __syncthreads();
         double lkey = wkey[par][0];
         long long lrow = wrow[par][0];
+        int lw = 0;
 #pragma unroll
         for (int q = 1; q < NWV; ++q)
-            if (better(wkey[par][q], wrow[par][q], lkey, lrow)) { lkey = wkey[par][q]; lrow = wrow[par][q]; }
+            if (better(wkey[par][q], wrow[par][q], lkey, lrow)) { lkey = wkey[par][q]; lrow = wrow[par][q]; lw = q; }
 
-        // ---- exchange: the winner's row (up) and row j (dp), as doubles in LDS
-        double* rs = rec_sh[par];
+        // ---- exchange: the winner's row (up) and position j's row (dp), as
+        // doubles in LDS, in the rotated register order
         const double* up;
         const double* dp;
         long long piv;
         if (!global_x) {
-            // one workgroup: the candidate is the winner
-#pragma unroll
-            for (int s = 0; s < RPT; ++s) {
-                if (lr[s] == lrow && lrow < a.mrows)
-#pragma unroll
-                    for (int c = 0; c < OXI_LEAF_W; ++c) rs[3 + c] = double(v[s][c]);
-                if (lr[s] == j)
-#pragma unroll
-                    for (int c = 0; c < OXI_LEAF_W; ++c) rs[REC_N + 3 + c] = double(v[s][c]);
-            }
-            __syncthreads();
             piv = lrow;
-            up = rs + 3;
-            dp = rs + REC_N + 3;
+            up = wvals[par][lw];
+            dp = wvals[par][NWV];
         } else {
             const u64 tag = a.tag0 + u64(a.c0 + j);
             double* rec = a.pub + (size_t(j) * a.P + p) * OXI_PUB_STRIDE;
             double* drec = a.pub + (size_t(w) * a.P + j) * OXI_PUB_STRIDE;
-#pragma unroll
-            for (int s = 0; s < RPT; ++s) {
-                if (lr[s] == lrow && lrow < a.mrows)
-                    publish(rec, tag, [&](int f) {
-                        return f == 0 ? lkey : f == 1 ? __longlong_as_double(lrow) : double(v[s][f - 3]);
-                    });
-                if (p == 0 && lr[s] == j)
-                    publish(drec, tag, [&](int f) { return f < 3 ? 0.0 : double(v[s][f - 3]); });
-            }
-            if (lrow >= a.mrows && t == 0)   // no candidate rows in this workgroup
-                publish(rec, tag, [&](int f) {
-                    return f == 0 ? -2.0 : f == 1 ? __longlong_as_double((long long)(1ll << 62)) : 0.0;
+            if (wave == 0 && lane < 16) {
+                const bool has = lrow < a.mrows;
+                publish16(rec, tag, lane, [&](int f) {
+                    return f == 0 ? (has ? lkey : -2.0)
+                         : f == 1 ? __longlong_as_double(has ? lrow : (long long)(1ll << 62))
+                         : (has ? wvals[par][lw][f - 3] : 0.0);
                 });
-            // gather: record q (q < P candidates, q == P row j) by lanes 4i..4i+3 of
-            // one wave; lane sub holds fields sub, sub+4, ..., sub+32
+            }
+            if (p == 0 && wave == NWV - 1 && lane < 16)
+                publish16(drec, tag, lane, [&](int f) { return f < 3 ? 0.0 : wvals[par][NWV][f - 3]; });
+            // gather: record q (q < P candidates, q == P position j) by lanes
+            // 4i..4i+3 of one wave; lane sub holds fields sub, sub+4, ..., sub+32
+            double* rs = rec_sh[par];
             const int P1 = a.P + 1, npass = (P1 + PASS - 1) / PASS;
             const int sub = lane & 3;
             bool okp[NPASS_MAX];
@@ -186,18 +278,17 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
                     if (ps < npass && q < P1 && !okp[ps]) {
                         const double* src = q < a.P ? a.pub + (size_t(j) * a.P + q) * OXI_PUB_STRIDE : drec;
                         double fv[REC_N / 4];
+                        load9_agent(src + sub, fv);
                         u64 hx = 0;
 #pragma unroll
                         for (int i = 0; i < REC_N / 4; ++i) {
                             const int f = sub + 4 * i;
-                            fv[i] = get_agent(&src[f]);
                             if (f < REC_CHK) hx ^= rec_term(bits(fv[i]), tag, f);
                         }
-                        hx ^= __shfl_xor(hx, 1);
-                        hx ^= __shfl_xor(hx, 2);
-                        // the checksum sits in lane sub 3 (field 35), the tag in sub 2 (field 2)
-                        const u64 chk = bits(__shfl(fv[REC_N / 4 - 1], (lane & ~3) | 3));
-                        const u64 tg = bits(__shfl(fv[0], (lane & ~3) | 2));
+                        hx = quad_xor(hx);
+                        // the checksum sits in quad lane 3 (field 35), the tag in quad lane 2 (field 2)
+                        const u64 chk = quad_bcast<3>(bits(fv[REC_N / 4 - 1]));
+                        const u64 tg = quad_bcast<2>(bits(fv[0]));
                         if (hx == chk && tg == tag) {
                             okp[ps] = true;
 #pragma unroll
@@ -207,77 +298,116 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
                     if (ps < npass && q < P1 && !okp[ps]) all_ok = false;
                 }
                 if (__syncthreads_and(all_ok)) break;
-                if (++spins > (1u << 22)) {
+                if (++spins > SPIN_LIMIT) {   // uniform: every thread leaves
                     if (t == 0) __hip_atomic_store(a.flags, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    dead = 1;   // every thread writes the same value: uniform exit
-                    break;
+                    return;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
-            if (spins > (1u << 22)) return;
-            // the winner over the P candidates, in every wave
-            double k = -3.0; long long r = (1ll << 62); int wq = 0;
+            // the winner over the P candidates, in every wave; record q holds
+            // rows of workgroup q only, so ties go to the lowest q
+            double k = -3.0; long long r = (1ll << 62); int wq = -1;
             for (int q = lane; q < a.P; q += 64) {
                 const double kq = rs[q * REC_N];
                 const long long rq = __double_as_longlong(rs[q * REC_N + 1]);
                 if (better(kq, rq, k, r)) { k = kq; r = rq; wq = q; }
             }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ok = __shfl_xor(k, off);
-                const long long orr = __shfl_xor(r, off);
-                const int ow = __shfl_xor(wq, off);
-                if (better(ok, orr, k, r)) { k = ok; r = orr; wq = ow; }
-            }
-            piv = r;
-            up = rs + wq * REC_N + 3;
+            const double gk = wave_max(k);
+            const u64 m0 = __ballot(k == gk && wq >= 0 && wq < 64);
+            const int gl = m0 ? first_lane(m0) : first_lane(__ballot(k == gk && wq >= 64));
+            piv = (long long)readlane64(u64(r), gl);
+            const int gq = __builtin_amdgcn_readlane(wq, gl);
+            up = rs + gq * REC_N + 3;
             dp = rs + a.P * REC_N + 3;
         }
 
-        // ---- interchange, scale, rank-1 update
+        // ---- interchange (the registers: columns >= j), store position j,
+        // scale, store the multipliers, rotate with the rank-1 update
         if (p == 0 && t == 0) a.ipiv[a.c0 + j] = a.c0 + piv + 1;
-        if (piv != j) {
+#pragma unroll
+        for (int s = 0; s < RPT; ++s) {
+            if (piv != j && (lr[s] == j || lr[s] == piv)) {
+                const double* src = lr[s] == j ? up : dp;
+#pragma unroll
+                for (int c = 0; c < W; ++c) v[s][c] = T(src[c]);
+            }
+            if (lr[s] == j)   // columns j .. w-1 of the pivot row are final
+#pragma unroll
+                for (int c = 0; c < W; ++c) if (c < w - j) wrow_of[s][j + c] = v[s][c];
+        }
+        const T d = T(up[0]);
+        const bool upd = d != T(0);
+        if (!upd && p == 0 && t == 0 && *a.info == 0) *a.info = a.c0 + j + 1;
+        T l[RPT];
+#pragma unroll
+        for (int s = 0; s < RPT; ++s) {
+            l[s] = upd ? v[s][0] / d : v[s][0];   // a zero pivot leaves the column as it is
+            if (lr[s] > j && lr[s] < a.mrows) wrow_of[s][j] = l[s];
+        }
+        if (upd) {
+            T u[W];
+#pragma unroll
+            for (int c = 1; c < W; ++c) u[c] = T(up[c]);
 #pragma unroll
             for (int s = 0; s < RPT; ++s) {
-                if (lr[s] == j) {
 #pragma unroll
-                    for (int c = 0; c < OXI_LEAF_W; ++c) v[s][c] = T(up[c]);
-                } else if (lr[s] == piv) {
-#pragma unroll
-                    for (int c = 0; c < OXI_LEAF_W; ++c) v[s][c] = T(dp[c]);
-                }
+                for (int c = 0; c + 1 < W; ++c) v[s][c] = fma_t(-l[s], u[c + 1], v[s][c + 1]);
+                v[s][W - 1] = T(0);
             }
-        }
-        const T d = T(up[j]);
-        if (d == T(0)) {
-            if (p == 0 && t == 0 && *a.info == 0) *a.info = a.c0 + j + 1;
         } else {
-            T ucol[OXI_LEAF_W];
-#pragma unroll
-            for (int c = j + 1; c < OXI_LEAF_W; ++c) ucol[c] = T(up[c]);
 #pragma unroll
             for (int s = 0; s < RPT; ++s) {
-                if (lr[s] > j && lr[s] < a.mrows) {
-                    const T l = v[s][j] / d;
-                    v[s][j] = l;
 #pragma unroll
-                    for (int c = j + 1; c < OXI_LEAF_W; ++c) v[s][c] = fma_t(-l, ucol[c], v[s][c]);
-                }
+                for (int c = 0; c + 1 < W; ++c) v[s][c] = v[s][c + 1];
+                v[s][W - 1] = T(0);
             }
-        }
-    };
-    [&]<int... J>(std::integer_sequence<int, J...>) __attribute__((always_inline)) {
-        (step(std::integral_constant<int, J>{}), ...);
-    }(std::make_integer_sequence<int, OXI_LEAF_W>{});
-    // ---- write back
-#pragma unroll
-    for (int s = 0; s < RPT; ++s) {
-        if (lr[s] < a.mrows) {
-            T* dst = Wm + a.c0 + (a.c0 + lr[s]) * a.ldw;
-#pragma unroll
-            for (int c = 0; c < OXI_LEAF_W; ++c) if (c < w) dst[c] = v[s][c];
         }
     }
+}
+
+// Interchanges of a leaf applied to its stored multipliers: column c (one
+// wave per column) takes the interchanges of steps c+1 .. w-1 in order.  The
+// positions involved are the steps' own rows (slots 0 .. n-1) and pivot rows
+// below the leaf (one slot per distinct row, n + first occurrence); the
+// values sit one per lane and each interchange swaps two lanes.
+template <typename T>
+__global__ __launch_bounds__(64) void leaf_lswap_kernel(T* Wm, i64 ldw, i64 c0, int w, const i64* ipiv,
+                                                        const int* flags) {
+    if (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    const int c = blockIdx.x, lane = threadIdx.x;
+    const int n = w - 1 - c;   // interchanges of steps c+1 .. w-1
+    if (n <= 0) return;
+    // lane i < n: step c+1+i and its pivot position (relative to c0)
+    const i64 pi = lane < n ? ipiv[c0 + c + 1 + lane] - 1 - c0 : -1;
+    // first occurrence of this pivot position among the steps (for pivot rows >= w)
+    int first = lane;
+    for (int i = 0; i < n; ++i) {
+        const i64 q = (i64)readlane64(u64(pi), i);
+        if (i < first && q == pi) first = i;
+    }
+    // slot values: lane k < n holds position c+1+k; lane n+i holds pivot row
+    // p_i >= w for the first occurrence i of that row
+    i64 pos = -1;
+    if (lane < n) pos = c + 1 + lane;
+    const int src_i = lane - n;   // differs per lane: a shuffle, not readlane
+    const int src_l = (src_i >= 0 && src_i < n) ? src_i : 0;
+    const i64 pos_out = (i64)__shfl((long long)pi, src_l);
+    const int first_out = __shfl(first, src_l);
+    if (src_i >= 0 && src_i < n && pos_out >= w && first_out == src_i) pos = pos_out;
+    T val = T(0);
+    if (pos >= 0) val = Wm[(c0 + pos) * ldw + c0 + c];
+    for (int i = 0; i < n; ++i) {
+        const i64 q = (i64)readlane64(u64(pi), i);
+        const int fi = __builtin_amdgcn_readlane(first, i);
+        const int sa = i;
+        const int sb = q < w ? int(q - c - 1) : n + fi;
+        if (sa == sb) continue;
+        const T va = T(__builtin_bit_cast(double, readlane64(__builtin_bit_cast(u64, double(val)), sa)));
+        const T vb = T(__builtin_bit_cast(double, readlane64(__builtin_bit_cast(u64, double(val)), sb)));
+        if (lane == sa) val = vb;
+        else if (lane == sb) val = va;
+    }
+    if (pos >= 0) Wm[(c0 + pos) * ldw + c0 + c] = val;
 }
 
 }  // namespace
