@@ -17,10 +17,10 @@
 
 namespace {
 
-constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
+constexpr int BM = 128, BN = 128, BK = 32, NT = 256;
 constexpr int PAD = 1;                 // one double of padding per LDS row
 constexpr int LDS_A = BM + PAD, LDS_B = BN + PAD;
-constexpr int LOADS = BM * BK / NT;    // doubles per thread per operand per slab (8)
+constexpr int LOADS = BM * BK / NT;    // doubles per thread per operand per slab (16)
 
 // Load one BK-deep slab of an operand tile into registers.
 //   OUTER_CONTIG: element (o, kk) at X[(o0+o) + (k0+kk)*ldx] (op(A) for A 'N',

@@ -25,10 +25,12 @@
 //
 // The leaf runs as one persistent kernel over P workgroups (P co-resident,
 // P <= number of CUs) that each own a contiguous block of rows in registers;
-// per column the workgroups publish their candidate (key, row, and the whole
-// row), meet at a device-scope counter barrier (release/acquire at agent
-// scope, bounded spin with a timeout flag so every wave exits), and each reads
-// all candidates to pick the same winner.  With P == 1 the exchange stays in LDS.
+// per column every workgroup publishes its candidate (key, row and the whole
+// row) as a self-validating record and reads all the others until they are
+// valid (bounded wait with a timeout flag so every wave exits), so all pick
+// the same winner; with P == 1 the exchange stays in LDS (panel.hip).  A
+// small kernel after each leaf applies its interchanges to its multipliers
+// and to the rest of the panel.
 #include "oxi_internal.hpp"
 
 #include <vector>
@@ -306,6 +308,7 @@ struct GetrfCtx {
     int* flags;
     int prec;          // FP32 only: -1 FP32 trailing update, 0 BF16, 1 FP16, 2 FP8 e4m3
     void* lowp_ws; size_t lowp_bytes;
+    i64 pc0 = 0, pc1 = 0;   // the top-level panel being factored: columns [pc0, pc1)
 };
 
 template <typename T>
@@ -356,22 +359,24 @@ int leaf(GetrfCtx<T>& g, i64 c0, int w) {
     if (P > std::min<i64>(g.h->num_cus, OXI_LEAF_MAX_P))
         return oxi_set_error(g.h, OXI_ERR_UNSUPPORTED, "panel of %lld rows exceeds the leaf kernel capacity",
                              (long long)mrows);
-    LeafArgs a{(void*)g.Wm, g.ldw, c0, w, mrows, g.ipiv, g.dinfo, g.tag0, g.pub, int(P), g.flags};
+    LeafArgs a{(void*)g.Wm, g.ldw, c0, w, mrows, g.ipiv, g.dinfo, g.tag0, g.pub, int(P), g.flags, g.pc0, g.pc1};
     if (sizeof(T) == 8) OXI_TRY(oxi_launch_lu_leaf(g.h, g.s, a, kind));
     else OXI_TRY(oxi_launch_lu_leaf32(g.h, g.s, a, kind));
     return OXI_OK;
 }
 
 // getrf2 recursion on the panel A[c0:m, c0:c0+np] in W coordinates
-// (src/lapack/factorizations.jl:114-152).
+// (src/lapack/factorizations.jl:114-152).  Each leaf applies its
+// interchanges to all other columns of the top-level panel [g.pc0, g.pc1)
+// (leaf_swaps_kernel), which covers the recursion's "left half's
+// interchanges on the right half" and "right half's interchanges on the left
+// half" at every level.
 template <typename T>
 int panel(GetrfCtx<T>& g, i64 c0, i64 np) {
     if (np <= OXI_LEAF_W) return leaf(g, c0, int(np));
     const i64 mp = g.m - c0;
     const i64 h = np / 2;
     OXI_TRY(panel(g, c0, h));
-    // left half's interchanges on the right half columns
-    OXI_TRY(swap_w(g, c0, c0 + h - 1, c0 + h, c0 + np, 0));
     // A12 <- L11^-1 A12 : W[c0+h : c0+np, c0 : c0+h] <- W[..] * (L11^T)^-1, L11^T = W[c0:c0+h, c0:c0+h] upper unit
     OXI_TRY(trsm_ru(g.h, g.s, np - h, h, g.Wm + c0 + c0 * g.ldw, g.ldw, g.Wm + (c0 + h) + c0 * g.ldw, g.ldw));
     if (mp > h) {
@@ -379,10 +384,16 @@ int panel(GetrfCtx<T>& g, i64 c0, i64 np) {
         OXI_TRY(gemm_nn<T>(g.h, g.s, np - h, mp - h, h, T(-1), g.Wm + (c0 + h) + c0 * g.ldw, g.ldw,
                            g.Wm + c0 + (c0 + h) * g.ldw, g.ldw, T(1), g.Wm + (c0 + h) + (c0 + h) * g.ldw, g.ldw));
         OXI_TRY(panel(g, c0 + h, np - h));
-        // right half's interchanges on the left half columns
-        OXI_TRY(swap_w(g, c0 + h, c0 + std::min(mp, np) - 1, c0, c0 + h, 0));
     }
     return OXI_OK;
+}
+
+// A top-level panel: columns [c0, c0 + np) of the factorization.
+template <typename T>
+int panel_top(GetrfCtx<T>& g, i64 c0, i64 np) {
+    g.pc0 = c0;
+    g.pc1 = c0 + np;
+    return panel(g, c0, np);
 }
 
 // Trailing update of A columns [r0, r1) (W rows) with panel [j0, j1):
@@ -475,7 +486,7 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     gs.s = h->side;
     gs.pairs = (int*)(base + off_pairs2);
     bool panel_on_side = false;
-    OXI_TRY(panel(g, 0, std::min<i64>(nb, q)));
+    OXI_TRY(panel_top(g, 0, std::min<i64>(nb, q)));
     for (i64 j0 = 0; j0 < q; j0 += nb) {
         const i64 jb = std::min<i64>(nb, q - j0);
         const i64 j1 = j0 + jb;
@@ -493,7 +504,7 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
             OXI_TRY(oxi_prof_end(h, s, pe, 2.0 * double(j2 - j1) * double(m - j1) * double(jb)));
             OXI_HIP(h, hipEventRecord(h->ev_a, s));
             OXI_HIP(h, hipStreamWaitEvent(h->side, h->ev_a, 0));
-            OXI_TRY(panel(gs, j1, j2 - j1));
+            OXI_TRY(panel_top(gs, j1, j2 - j1));
             OXI_HIP(h, hipEventRecord(h->ev_b, h->side));
             panel_on_side = true;
         }
@@ -508,7 +519,7 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
             }
         }
         if (next && !lookahead) {
-            OXI_TRY(panel(g, j1, std::min<i64>(nb, q - j1)));
+            OXI_TRY(panel_top(g, j1, std::min<i64>(nb, q - j1)));
             panel_on_side = false;
         }
     }

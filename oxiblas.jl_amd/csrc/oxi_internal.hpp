@@ -132,6 +132,7 @@ struct LeafArgs {
     double* pub;     // published records [w][P][PUB_STRIDE] + diag rows [w][PUB_STRIDE]
     int P;
     int* flags;      // flags[0] = 1 on barrier timeout
+    i64 xlo, xhi;    // columns [xlo, xhi) of the enclosing panel: receive the leaf's interchanges
 };
 
 // kind 0: 256 threads x 1 row per workgroup; kind 1: 256 threads x 2 rows.

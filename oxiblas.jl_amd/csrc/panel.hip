@@ -21,7 +21,9 @@
 // Row interchanges act on the registers (columns j and right of it) at once;
 // the multipliers left of column j are already in memory and receive the
 // interchanges of their later steps from a second, small kernel, which gives
-// the same rows as swapping whole rows at each step.
+// the same rows as swapping whole rows at each step.  The same kernel applies
+// the leaf's interchanges to the other columns of the enclosing panel (the
+// sibling swaps of the getrf2 recursion, done once per leaf).
 #include "panel_kernel.hpp"
 
 int oxi_launch_lu_leaf(oxi_handle_t h, hipStream_t s, const LeafArgs& a, int kind) {
@@ -29,12 +31,13 @@ int oxi_launch_lu_leaf(oxi_handle_t h, hipStream_t s, const LeafArgs& a, int kin
     else hipLaunchKernelGGL((lu_leaf_kernel<double, 256, 2>), dim3(unsigned(a.P)), dim3(256), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "lu_leaf launch: %s", hipGetErrorString(e));
-    // the leaf's interchanges on its stored multipliers
-    if (a.w > 1) {
-        hipLaunchKernelGGL(leaf_lswap_kernel<double>, dim3(unsigned(a.w - 1)), dim3(64), 0, s, (double*)a.Wm, a.ldw, a.c0, a.w,
-                           (const i64*)a.ipiv, (const int*)a.flags);
+    // the leaf's interchanges on its stored multipliers and on the rest of its panel
+    const unsigned nblk = unsigned(std::max(a.w - 1, 0) + cdiv(a.c0 - a.xlo, 64) + cdiv(a.xhi - a.c0 - a.w, 64));
+    if (nblk > 0) {
+        hipLaunchKernelGGL(leaf_swaps_kernel<double>, dim3(nblk), dim3(64), 0, s, (double*)a.Wm, a.ldw, a.c0, a.w,
+                           (const i64*)a.ipiv, (const int*)a.flags, a.xlo, a.xhi);
         e = hipGetLastError();
-        if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "leaf_lswap launch: %s", hipGetErrorString(e));
+        if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "leaf_swaps launch: %s", hipGetErrorString(e));
     }
     return OXI_OK;
 }

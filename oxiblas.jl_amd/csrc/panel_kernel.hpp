@@ -156,7 +156,7 @@ __device__ __forceinline__ void publish16(double* rec, u64 tag, int L, F field) 
 // j goes to memory at step j (at the row's position at that time), and the
 // pivot row at position j goes to memory at step j (later interchanges only
 // involve positions below j).  Interchanges of later steps are applied to
-// the stored multipliers afterwards by leaf_lswap_kernel, as getrf2 applies
+// the stored multipliers afterwards by leaf_swaps_kernel, as getrf2 applies
 // the right half's interchanges to the left half.
 // ---------------------------------------------------------------------------
 template <typename T, int LTT, int RPT>
@@ -365,49 +365,89 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
     }
 }
 
-// Interchanges of a leaf applied to its stored multipliers: column c (one
-// wave per column) takes the interchanges of steps c+1 .. w-1 in order.  The
-// positions involved are the steps' own rows (slots 0 .. n-1) and pivot rows
-// below the leaf (one slot per distinct row, n + first occurrence); the
-// values sit one per lane and each interchange swaps two lanes.
-template <typename T>
-__global__ __launch_bounds__(64) void leaf_lswap_kernel(T* Wm, i64 ldw, i64 c0, int w, const i64* ipiv,
-                                                        const int* flags) {
-    if (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
-    const int c = blockIdx.x, lane = threadIdx.x;
-    const int n = w - 1 - c;   // interchanges of steps c+1 .. w-1
-    if (n <= 0) return;
-    // lane i < n: step c+1+i and its pivot position (relative to c0)
-    const i64 pi = lane < n ? ipiv[c0 + c + 1 + lane] - 1 - c0 : -1;
-    // first occurrence of this pivot position among the steps (for pivot rows >= w)
+// The slots of the interchanges of steps k_lo .. w-1 of a leaf at c0 (one
+// wave): lane i < n (n = w - k_lo) is step k_lo + i's own position; lane
+// n + i is the pivot position p_i of step k_lo + i when it lies below the
+// leaf's w rows and i is its first occurrence.  pos = -1 for unused lanes.
+// The interchanges are then replayed on slot ids: afterwards lane s holds
+// the slot whose original content ends at slot s.
+__device__ __forceinline__ void leaf_slots(const i64* ipiv, i64 c0, int w, int k_lo, int lane, i64& pos, int& src) {
+    const int n = w - k_lo;
+    const i64 pi = lane < n ? ipiv[c0 + k_lo + lane] - 1 - c0 : -1;
     int first = lane;
     for (int i = 0; i < n; ++i) {
         const i64 q = (i64)readlane64(u64(pi), i);
         if (i < first && q == pi) first = i;
     }
-    // slot values: lane k < n holds position c+1+k; lane n+i holds pivot row
-    // p_i >= w for the first occurrence i of that row
-    i64 pos = -1;
-    if (lane < n) pos = c + 1 + lane;
+    pos = lane < n ? k_lo + lane : -1;
     const int src_i = lane - n;   // differs per lane: a shuffle, not readlane
     const int src_l = (src_i >= 0 && src_i < n) ? src_i : 0;
     const i64 pos_out = (i64)__shfl((long long)pi, src_l);
     const int first_out = __shfl(first, src_l);
     if (src_i >= 0 && src_i < n && pos_out >= w && first_out == src_i) pos = pos_out;
-    T val = T(0);
-    if (pos >= 0) val = Wm[(c0 + pos) * ldw + c0 + c];
+    src = lane;
     for (int i = 0; i < n; ++i) {
         const i64 q = (i64)readlane64(u64(pi), i);
         const int fi = __builtin_amdgcn_readlane(first, i);
-        const int sa = i;
-        const int sb = q < w ? int(q - c - 1) : n + fi;
+        const int sa = i, sb = q < w ? int(q - k_lo) : n + fi;
         if (sa == sb) continue;
-        const T va = T(__builtin_bit_cast(double, readlane64(__builtin_bit_cast(u64, double(val)), sa)));
-        const T vb = T(__builtin_bit_cast(double, readlane64(__builtin_bit_cast(u64, double(val)), sb)));
-        if (lane == sa) val = vb;
-        else if (lane == sb) val = va;
+        const int ia = __builtin_amdgcn_readlane(src, sa), ib = __builtin_amdgcn_readlane(src, sb);
+        if (lane == sa) src = ib;
+        else if (lane == sb) src = ia;
     }
-    if (pos >= 0) Wm[(c0 + pos) * ldw + c0 + c] = val;
+}
+
+// A leaf's interchanges applied where the leaf did not apply them, one wave
+// per block:
+//  * its own multiplier columns c < w - 1 (one block each): column c takes
+//    the interchanges of steps c+1 .. w-1 (the multipliers of column c were
+//    stored at the positions of step c);
+//  * the other columns of the enclosing panel, [x_lo, c0) and [c0+w, x_hi),
+//    64 per block: all w interchanges.
+// Applying a leaf's interchanges to the rest of its panel at once gives the
+// same columns as the getrf2 recursion, which applies them level by level:
+// a left subtree's multipliers are not read again once its right sibling
+// has started, and the right sibling's columns receive them before any
+// update reads them.
+template <typename T>
+__global__ __launch_bounds__(64) void leaf_swaps_kernel(T* Wm, i64 ldw, i64 c0, int w, const i64* ipiv,
+                                                        const int* flags, i64 x_lo, i64 x_hi) {
+    if (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    const int lane = threadIdx.x;
+    const int b = blockIdx.x;
+    if (b < w - 1) {
+        const int c = b;
+        i64 pos; int src;
+        leaf_slots(ipiv, c0, w, c + 1, lane, pos, src);
+        const i64 psrc = (i64)__shfl((long long)pos, src);
+        T v = T(0);
+        if (pos >= 0 && psrc != pos) v = Wm[(c0 + psrc) * ldw + c0 + c];
+        if (pos >= 0 && psrc != pos) Wm[(c0 + pos) * ldw + c0 + c] = v;
+        return;
+    }
+    const i64 fb = b - (w - 1);
+    const i64 tl = (c0 - x_lo + 63) / 64;
+    i64 x;
+    bool ok;
+    if (fb < tl) { x = x_lo + fb * 64 + lane; ok = x < c0; }
+    else { x = c0 + w + (fb - tl) * 64 + lane; ok = x < x_hi; }
+    i64 pos; int src;
+    leaf_slots(ipiv, c0, w, 0, lane, pos, src);
+    // slot s moves when its source differs; at most 2w <= 64 slots
+    T v[64];
+#pragma unroll
+    for (int s = 0; s < 64; ++s) {
+        const i64 ps = (i64)readlane64(u64(pos), s);
+        const int ss = __builtin_amdgcn_readlane(src, s);
+        v[s] = T(0);
+        if (ps >= 0 && ss != s && ok) v[s] = Wm[(c0 + (i64)readlane64(u64(pos), ss)) * ldw + x];
+    }
+#pragma unroll
+    for (int s = 0; s < 64; ++s) {
+        const i64 ps = (i64)readlane64(u64(pos), s);
+        const int ss = __builtin_amdgcn_readlane(src, s);
+        if (ps >= 0 && ss != s && ok) Wm[(c0 + ps) * ldw + x] = v[s];
+    }
 }
 
 }  // namespace
