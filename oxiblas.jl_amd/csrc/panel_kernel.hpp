@@ -152,6 +152,13 @@ __device__ __forceinline__ void publish16(double* rec, u64 tag, int L, F field) 
 // every step (v[c] <- fma(-l, u[c+1], v[c+1])).  The column loop is rolled
 // and every register index is a compile-time constant.
 //
+// The exchange of column j+1 is overlapped with the update of column j: once
+// the multipliers of column j are known, every row computes only its next
+// column (register 1 after the update), the candidates for column j+1 are
+// chosen from those values, the candidate rows (and position j+1's row) are
+// updated on their own and published, and only then do all rows take the
+// full update while the records travel.
+//
 // Values leave the registers as they become final: the multiplier of column
 // j goes to memory at step j (at the row's position at that time), and the
 // pivot row at position j goes to memory at step j (later interchanges only
@@ -168,8 +175,15 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
     constexpr unsigned SPIN_LIMIT = 1u << 22;
     __shared__ double wkey[2][NWV];
     __shared__ long long wrow[2][NWV];
-    __shared__ double wvals[2][NWV + 1][W];           // each wave's best row; [NWV]: position j
-    __shared__ double rec_sh[2][REC_SLOTS * REC_N];   // gathered records, by column parity
+    // LDS copies of records keep field f at offset f + 1 of a 40-double slot,
+    // so that the row values (fields 3..34) start 16-byte aligned
+    constexpr int RS_N = 40;
+    __shared__ __align__(16) double wvals[2][NWV + 1][W];        // each wave's best row; [NWV]: position j
+    __shared__ __align__(16) double rec_sh[2][REC_SLOTS * RS_N];  // gathered records, by column parity
+    __shared__ __align__(16) double raw[NWV + 1][W];              // candidate rows before the pending update
+    __shared__ double raw_l[NWV + 1];                            // and their multipliers
+    __shared__ long long lrow_sh[2];                  // P == 1: the winner's position
+    __shared__ int lw_sh[2];                          // P == 1: the winner's wave
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int p = blockIdx.x;
     const i64 R = i64(LT) * RPT;
@@ -196,17 +210,18 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
     // stores of the previous column included) at the top of every column
     __builtin_amdgcn_s_waitcnt(0);
 
-#pragma unroll 1
-    for (int j0 = 0; j0 < w; ++j0) {
-        const int j = __builtin_amdgcn_readfirstlane(j0);
-        const int par = j & 1;
-        // ---- local candidate: per thread, then the wave (DPP maximum, ties
-        // to the lowest row through ballots), then over the waves in LDS
+    // ---- candidates of column jn and their publication.  x[s]: row s's
+    // value in column jn.  pend: the registers still await the update of
+    // column jn - 1 (multipliers lm, pivot row upp, upd false for a zero
+    // pivot); the candidate rows are then updated here, one column per lane.
+    auto select_publish = [&](int jn, const T (&x)[RPT], bool pend, const T (&lm)[RPT], const double* upp,
+                              bool upd) __attribute__((always_inline)) {
+        const int parn = jn & 1;
         double bk = -2.0; long long br = (1ll << 62); int bs = -1;
 #pragma unroll
         for (int s = 0; s < RPT; ++s) {
-            if (lr[s] < a.mrows && lr[s] >= j) {
-                const double k = pivot_key(double(v[s][0]), lr[s] == j);
+            if (lr[s] < a.mrows && lr[s] >= jn) {
+                const double k = pivot_key(double(x[s]), lr[s] == jn);
                 if (better(k, lr[s], bk, br)) { bk = k; br = lr[s]; bs = s; }
             }
         }
@@ -220,96 +235,147 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
             if (m) { wl = first_lane(m); ws = s; }
         }
         const long long wr = (long long)readlane64(u64(br), wl);
+        // the wave's candidate and position jn's row: as they are (pend false)
+        // or through raw[] and one updated column per lane
+        double (*dst)[W] = pend ? raw : wvals[parn];
+        bool diag_here = false;
 #pragma unroll
         for (int s = 0; s < RPT; ++s) {
-            if (lane == wl && ws == s)
+            const bool cand = lane == wl && ws == s, diag = lr[s] == jn;
+            diag_here |= diag;
+            if (cand)
 #pragma unroll
-                for (int c = 0; c < W; ++c) wvals[par][wave][c] = double(v[s][c]);
-            if (lr[s] == j)
+                for (int c = 0; c < W; ++c) dst[wave][c] = double(v[s][c]);
+            if (diag)
 #pragma unroll
-                for (int c = 0; c < W; ++c) wvals[par][NWV][c] = double(v[s][c]);
+                for (int c = 0; c < W; ++c) dst[NWV][c] = double(v[s][c]);
+            if (pend && cand) raw_l[wave] = double(lm[s]);
+            if (pend && diag) raw_l[NWV] = double(lm[s]);
         }
-        if (lane == 0) { wkey[par][wave] = wk; wrow[par][wave] = ws >= 0 ? wr : (1ll << 62); }
+        if (pend) {
+            diag_here = __ballot(diag_here) != 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");   // the lanes' LDS writes before the reads below
+            __builtin_amdgcn_wave_barrier();
+            if (lane < W) {
+                auto updated = [&](int slot) __attribute__((always_inline)) -> double {
+                    if (lane + 1 >= W) return 0.0;
+                    const T r = T(raw[slot][lane + 1]);
+                    return double(upd ? fma_t(-T(raw_l[slot]), T(upp[lane + 1]), r) : r);
+                };
+                if (ws >= 0) wvals[parn][wave][lane] = updated(wave);
+                if (diag_here) wvals[parn][NWV][lane] = updated(NWV);
+            }
+        }
+        if (lane == 0) { wkey[parn][wave] = wk; wrow[parn][wave] = ws >= 0 ? wr : (1ll << 62); }
         __syncthreads();
-        double lkey = wkey[par][0];
-        long long lrow = wrow[par][0];
+        double lkey = wkey[parn][0];
+        long long lrow = wrow[parn][0];
         int lw = 0;
 #pragma unroll
         for (int q = 1; q < NWV; ++q)
-            if (better(wkey[par][q], wrow[par][q], lkey, lrow)) { lkey = wkey[par][q]; lrow = wrow[par][q]; lw = q; }
+            if (better(wkey[parn][q], wrow[parn][q], lkey, lrow)) { lkey = wkey[parn][q]; lrow = wrow[parn][q]; lw = q; }
+        if (!global_x) {
+            if (t == 0) { lrow_sh[parn] = lrow; lw_sh[parn] = lw; }
+            return;
+        }
+        const u64 tag = a.tag0 + u64(a.c0 + jn);
+        double* rec = a.pub + (size_t(jn) * a.P + p) * OXI_PUB_STRIDE;
+        double* drec = a.pub + (size_t(w) * a.P + jn) * OXI_PUB_STRIDE;
+        if (wave == 0 && lane < 16) {
+            const bool has = lrow < a.mrows;
+            publish16(rec, tag, lane, [&](int f) {
+                return f == 0 ? (has ? lkey : -2.0)
+                     : f == 1 ? __longlong_as_double(has ? lrow : (long long)(1ll << 62))
+                     : (has ? wvals[parn][lw][f - 3] : 0.0);
+            });
+        }
+        if (p == 0 && wave == NWV - 1 && lane < 16)
+            publish16(drec, tag, lane, [&](int f) { return f < 3 ? 0.0 : wvals[parn][NWV][f - 3]; });
+    };
 
-        // ---- exchange: the winner's row (up) and position j's row (dp), as
-        // doubles in LDS, in the rotated register order
+    // ---- the records of column jn into rec_sh (P > 1); a barrier either
+    // way, so that no wave overwrites the buffers of column jn - 1 while
+    // another still reads them.  Returns false on timeout.
+    auto gather = [&](int jn) __attribute__((always_inline)) -> bool {
+        if (!global_x) { __syncthreads(); return true; }
+        const int parn = jn & 1;
+        const u64 tag = a.tag0 + u64(a.c0 + jn);
+        const double* drec = a.pub + (size_t(w) * a.P + jn) * OXI_PUB_STRIDE;
+        // record q (q < P candidates, q == P position jn) by lanes 4i..4i+3 of
+        // one wave; lane sub holds fields sub, sub+4, ..., sub+32
+        double* rs = rec_sh[parn];
+        const int P1 = a.P + 1, npass = (P1 + PASS - 1) / PASS;
+        const int sub = lane & 3;
+        bool okp[NPASS_MAX];
+#pragma unroll
+        for (int ps = 0; ps < NPASS_MAX; ++ps) okp[ps] = false;
+        unsigned spins = 0;
+        while (true) {
+            bool all_ok = true;
+#pragma unroll
+            for (int ps = 0; ps < NPASS_MAX; ++ps) {
+                const int q = ps * PASS + wave * RPW + (lane >> 2);
+                if (ps < npass && q < P1 && !okp[ps]) {
+                    const double* src = q < a.P ? a.pub + (size_t(jn) * a.P + q) * OXI_PUB_STRIDE : drec;
+                    double fv[REC_N / 4];
+                    load9_agent(src + sub, fv);
+                    u64 hx = 0;
+#pragma unroll
+                    for (int i = 0; i < REC_N / 4; ++i) {
+                        const int f = sub + 4 * i;
+                        if (f < REC_CHK) hx ^= rec_term(bits(fv[i]), tag, f);
+                    }
+                    hx = quad_xor(hx);
+                    // the checksum sits in quad lane 3 (field 35), the tag in quad lane 2 (field 2)
+                    const u64 chk = quad_bcast<3>(bits(fv[REC_N / 4 - 1]));
+                    const u64 tg = quad_bcast<2>(bits(fv[0]));
+                    if (hx == chk && tg == tag) {
+                        okp[ps] = true;
+#pragma unroll
+                        for (int i = 0; i < REC_N / 4; ++i) rs[q * RS_N + 1 + sub + 4 * i] = fv[i];
+                    }
+                }
+                if (ps < npass && q < P1 && !okp[ps]) all_ok = false;
+            }
+            if (__syncthreads_and(all_ok)) return true;
+            if (++spins > SPIN_LIMIT) {   // uniform: every thread leaves
+                if (t == 0) __hip_atomic_store(a.flags, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    };
+
+    // column 0: the registers as loaded
+    {
+        T x0[RPT];
+#pragma unroll
+        for (int s = 0; s < RPT; ++s) x0[s] = v[s][0];
+        select_publish(0, x0, false, x0, nullptr, false);
+        if (!gather(0)) return;
+    }
+
+#pragma unroll 1
+    for (int j0 = 0; j0 < w; ++j0) {
+        const int j = __builtin_amdgcn_readfirstlane(j0);
+        const int par = j & 1;
+        // ---- the winner of column j: the pivot row (up) and position j's
+        // row (dp), as doubles in LDS, in the rotated register order
         const double* up;
         const double* dp;
         long long piv;
         if (!global_x) {
-            piv = lrow;
-            up = wvals[par][lw];
+            piv = lrow_sh[par];
+            up = wvals[par][lw_sh[par]];
             dp = wvals[par][NWV];
         } else {
-            const u64 tag = a.tag0 + u64(a.c0 + j);
-            double* rec = a.pub + (size_t(j) * a.P + p) * OXI_PUB_STRIDE;
-            double* drec = a.pub + (size_t(w) * a.P + j) * OXI_PUB_STRIDE;
-            if (wave == 0 && lane < 16) {
-                const bool has = lrow < a.mrows;
-                publish16(rec, tag, lane, [&](int f) {
-                    return f == 0 ? (has ? lkey : -2.0)
-                         : f == 1 ? __longlong_as_double(has ? lrow : (long long)(1ll << 62))
-                         : (has ? wvals[par][lw][f - 3] : 0.0);
-                });
-            }
-            if (p == 0 && wave == NWV - 1 && lane < 16)
-                publish16(drec, tag, lane, [&](int f) { return f < 3 ? 0.0 : wvals[par][NWV][f - 3]; });
-            // gather: record q (q < P candidates, q == P position j) by lanes
-            // 4i..4i+3 of one wave; lane sub holds fields sub, sub+4, ..., sub+32
-            double* rs = rec_sh[par];
-            const int P1 = a.P + 1, npass = (P1 + PASS - 1) / PASS;
-            const int sub = lane & 3;
-            bool okp[NPASS_MAX];
-#pragma unroll
-            for (int ps = 0; ps < NPASS_MAX; ++ps) okp[ps] = false;
-            unsigned spins = 0;
-            while (true) {
-                bool all_ok = true;
-#pragma unroll
-                for (int ps = 0; ps < NPASS_MAX; ++ps) {
-                    const int q = ps * PASS + wave * RPW + (lane >> 2);
-                    if (ps < npass && q < P1 && !okp[ps]) {
-                        const double* src = q < a.P ? a.pub + (size_t(j) * a.P + q) * OXI_PUB_STRIDE : drec;
-                        double fv[REC_N / 4];
-                        load9_agent(src + sub, fv);
-                        u64 hx = 0;
-#pragma unroll
-                        for (int i = 0; i < REC_N / 4; ++i) {
-                            const int f = sub + 4 * i;
-                            if (f < REC_CHK) hx ^= rec_term(bits(fv[i]), tag, f);
-                        }
-                        hx = quad_xor(hx);
-                        // the checksum sits in quad lane 3 (field 35), the tag in quad lane 2 (field 2)
-                        const u64 chk = quad_bcast<3>(bits(fv[REC_N / 4 - 1]));
-                        const u64 tg = quad_bcast<2>(bits(fv[0]));
-                        if (hx == chk && tg == tag) {
-                            okp[ps] = true;
-#pragma unroll
-                            for (int i = 0; i < REC_N / 4; ++i) rs[q * REC_N + sub + 4 * i] = fv[i];
-                        }
-                    }
-                    if (ps < npass && q < P1 && !okp[ps]) all_ok = false;
-                }
-                if (__syncthreads_and(all_ok)) break;
-                if (++spins > SPIN_LIMIT) {   // uniform: every thread leaves
-                    if (t == 0) __hip_atomic_store(a.flags, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    return;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
             // the winner over the P candidates, in every wave; record q holds
             // rows of workgroup q only, so ties go to the lowest q
+            const double* rs = rec_sh[par];
             double k = -3.0; long long r = (1ll << 62); int wq = -1;
             for (int q = lane; q < a.P; q += 64) {
-                const double kq = rs[q * REC_N];
-                const long long rq = __double_as_longlong(rs[q * REC_N + 1]);
+                const double kq = rs[q * RS_N + 1];
+                const long long rq = __double_as_longlong(rs[q * RS_N + 2]);
                 if (better(kq, rq, k, r)) { k = kq; r = rq; wq = q; }
             }
             const double gk = wave_max(k);
@@ -317,12 +383,12 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
             const int gl = m0 ? first_lane(m0) : first_lane(__ballot(k == gk && wq >= 64));
             piv = (long long)readlane64(u64(r), gl);
             const int gq = __builtin_amdgcn_readlane(wq, gl);
-            up = rs + gq * REC_N + 3;
-            dp = rs + a.P * REC_N + 3;
+            up = (const double*)__builtin_assume_aligned(rs + gq * RS_N + 4, 16);
+            dp = (const double*)__builtin_assume_aligned(rs + a.P * RS_N + 4, 16);
         }
 
         // ---- interchange (the registers: columns >= j), store position j,
-        // scale, store the multipliers, rotate with the rank-1 update
+        // scale, store the multipliers
         if (p == 0 && t == 0) a.ipiv[a.c0 + j] = a.c0 + piv + 1;
 #pragma unroll
         for (int s = 0; s < RPT; ++s) {
@@ -331,10 +397,9 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
 #pragma unroll
                 for (int c = 0; c < W; ++c) v[s][c] = T(src[c]);
             }
-            if (lr[s] == j)   // columns j .. w-1 of the pivot row are final
-#pragma unroll
-                for (int c = 0; c < W; ++c) if (c < w - j) wrow_of[s][j + c] = v[s][c];
         }
+        // position j now holds the pivot row, final in columns j .. w-1
+        if (p == 0 && wave == 0 && lane < w - j) Wm[a.c0 + j + (a.c0 + j) * a.ldw + lane] = T(up[lane]);
         const T d = T(up[0]);
         const bool upd = d != T(0);
         if (!upd && p == 0 && t == 0 && *a.info == 0) *a.info = a.c0 + j + 1;
@@ -344,6 +409,18 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
             l[s] = upd ? v[s][0] / d : v[s][0];   // a zero pivot leaves the column as it is
             if (lr[s] > j && lr[s] < a.mrows) wrow_of[s][j] = l[s];
         }
+
+        // ---- column j+1: its values, candidates and records first
+        const bool more = j + 1 < w;
+        if (more) {
+            T x[RPT];
+            const T u1 = T(up[1]);
+#pragma unroll
+            for (int s = 0; s < RPT; ++s) x[s] = upd ? fma_t(-l[s], u1, v[s][1]) : v[s][1];
+            select_publish(j + 1, x, true, l, up, upd);
+        }
+
+        // ---- the rank-1 update with the rotation, while the records travel
         if (upd) {
             T u[W];
 #pragma unroll
@@ -362,6 +439,7 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
                 v[s][W - 1] = T(0);
             }
         }
+        if (more && !gather(j + 1)) return;
     }
 }
 
