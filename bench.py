@@ -78,22 +78,24 @@ def mixed_extra(ox, torch, n, kappa, t_fp64, seed=0x0B1A5):
     A = ox.DMatrix(At.T.contiguous().reshape(-1), n, n)   # column-major A
     bd = ox.DMatrix(b.clone(), n, 1)
     out = {}
-    for low, solver in (("float32", "ir"), ("float32", "gmres"), ("float16", "gmres")):
-        r = ox.mixed_solve(A, bd, lowtype=low, solver=solver, maxiter=20, gmres_restart=30)   # warm
+    runs = (("float32", "ir", "partial"), ("float32", "gmres", "partial"), ("float32", "gmres", "rbt"))
+    for low, solver, pivot in runs:
+        kw = dict(lowtype=low, solver=solver, maxiter=20, gmres_restart=30, pivot=pivot)
+        r = ox.mixed_solve(A, bd, **kw)   # warm
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        r = ox.mixed_solve(A, bd, lowtype=low, solver=solver, maxiter=20, gmres_restart=30)
+        r = ox.mixed_solve(A, bd, **kw)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         x = r.x.buf[:n]
         fwd = (torch.linalg.norm(x - xt) / torch.linalg.norm(xt)).item()
-        key = f"mixed_{low}_{solver}"
+        key = f"mixed_{low}_{solver}" + ("_rbt" if pivot == "rbt" else "")
         out[key] = {"ms": round(t * 1e3, 2), "speedup_vs_fp64_lu_solve": round(t_fp64 / t, 3),
                     "converged": r.converged, "backward_error": r.backward_error,
                     "backward_error_over_n": r.backward_error / n, "forward_error": fwd,
                     "iterations": r.iterations, "gmres_iterations": r.inner_iterations, "fallback": r.fallback,
                     "factor_ms": round(r.factor_ms, 2), "refine_ms": round(r.refine_ms, 2)}
-        log(f"mixed {low}/{solver} n={n} kappa={kappa:g}: {t * 1e3:.1f} ms ({t_fp64 / t:.2f}x), "
+        log(f"mixed {low}/{solver}/{pivot} n={n} kappa={kappa:g}: {t * 1e3:.1f} ms ({t_fp64 / t:.2f}x), "
             f"berr {r.backward_error:.2e}, fwd {fwd:.2e}, it {r.iterations}/{r.inner_iterations}, "
             f"factor {r.factor_ms:.1f} ms refine {r.refine_ms:.1f} ms")
     del A, bd, At, b
@@ -262,6 +264,14 @@ def main():
         "cpu_openblas": cpu_ob,
         "parity": {"backward_error_over_n_eps": berr, "bound": 30, "ok": bool(berr <= 30)},
     }
+    # HBM bytes per trailing-update launch, from the FETCH_SIZE / WRITE_SIZE counter runs of this
+    # workload (scaled by a plain-copy calibration); measured only at n=16384 with the tuned blocksize
+    tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "lu_n16384_trailing_dgemm_traffic.json")
+    if n == 16384 and not args.nb and os.path.exists(tf):
+        t = json.load(open(tf))
+        out["roofline"]["traffic"] = t["hbm_bytes_per_launch"]
+        out["roofline"]["traffic_algorithmic"] = t["algorithmic_bytes_per_launch"]
+        out["roofline"]["traffic_launches"] = f"{t['launches_measured']} launches of more than 128 workgroups"
     if mixed:
         best = max(mixed.values(), key=lambda v: v["speedup_vs_fp64_lu_solve"] if v["converged"] else 0)
         out["mixed"] = {"kappa": args.kappa, "n": n, "best_speedup_vs_fp64_lu_solve":

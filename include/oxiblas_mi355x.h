@@ -172,10 +172,18 @@ typedef struct {
  * the correction accumulation and X stay FP64.  A (n x n) is not modified.
  * A factorization that fails in low precision falls back to FP64
  * (extended_lapack.jl:195-211).  Returns OXI_ERR_NOT_FINITE for non-finite
- * input (:182). */
+ * input (:182).
+ * pivot 0: the low-precision LU pivots partially, as the reference.  pivot 1
+ * (no reference counterpart): A_eq is first transformed by two-level random
+ * butterflies, U^T A_eq V, and factored without pivoting, which needs no
+ * device-wide pivot search per column; each correction applies U^T before and
+ * V after the triangular solves.  It applies to prec = OXI_PREC_FP32 (factors
+ * rounded further lose too much to multipliers that pivoting no longer bounds);
+ * n must be a multiple of 4 and the factor free of zero pivots.  Otherwise the
+ * partially pivoted factor is used. */
 int oxi_dgesv_mixed(oxi_handle_t handle, int prec, int solver, int64_t n, int64_t nrhs, const double* A,
                     int64_t lda, const double* B, int64_t ldb, double* X, int64_t ldx, int32_t maxiter, double rtol,
-                    int32_t gmres_restart, oxi_ir_info* info);
+                    int32_t gmres_restart, int32_t pivot, oxi_ir_info* info);
 
 /* equilibrate(A; radix)  (src/lapack/dense_utilities.jl:191-212): row scales
  * rows[i] = max_j |A[i,j]| (1 for a zero row), then column scales of A ./ rows,

@@ -138,13 +138,15 @@ int getrs_api(oxi_handle_t h, char trans, i64 n, i64 nrhs, const T* LU, i64 lda,
     const size_t xbytes = size_t(n) * size_t(nrhs) * sizeof(T);
     void* ws;
     const size_t xpad = (xbytes + 255) & ~size_t(255), ppad = (size_t(n) * 8 + 255) & ~size_t(255);
-    OXI_TRY(oxi_workspace(h, xpad + ppad + 256, &ws));
+    const bool chain = nrhs <= OXI_TRSV_MAX_RHS;
+    OXI_TRY(oxi_workspace(h, xpad + ppad + (chain ? oxi_trsv_ws_bytes(n, sizeof(T)) : 256), &ws));
     T* Y = (T*)ws;
     i64* dperm = (i64*)((char*)ws + xpad);
     void* ctrl = (char*)ws + xpad + ppad;
     OXI_HIP(h, hipMemcpyAsync(dperm, perm.data(), size_t(n) * 8, hipMemcpyHostToDevice, s));
+    if (chain) OXI_TRY(oxi_trsv_prepare(h, s, tk, n, LU, lda, (T*)nullptr, ctrl));
     OXI_TRY(getrs_perm<T>(h, s, tk, n, nrhs, LU, lda, dperm, B, ldb, Y, ctrl));
-    if (nrhs <= OXI_TRSV_MAX_RHS) return oxi_trsv_check(h, s, ctrl);   // the chain kernel ran: its status
+    if (chain) return oxi_trsv_check(h, s, ctrl);   // the chain kernel ran: its status
     OXI_HIP(h, hipStreamSynchronize(s));
     return OXI_OK;
 }

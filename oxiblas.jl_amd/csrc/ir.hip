@@ -400,7 +400,12 @@ struct Mixed {
     u64* stats;        // [0] berr [1] max|r| [2] max|b| [3] max|x| [4] ||A||_inf [5] magnitude max
     unsigned* flags;   // [0] non-finite input [1] matrix conversion [2],[3] RHS conversion [4] non-finite X
     double* magout;
-    void* ctrl;
+    void* ctrl;        // trsv workspace (control words and diagonal-block inverses)
+    int prepared = -1; // what the inverses in ctrl were made for (see prepare())
+    // random butterfly transform (pivot == 1): A' = U^T A_eq V factored without pivoting
+    bool rbt = false;
+    float* R32 = nullptr;
+    double* R64 = nullptr;
     // GMRES
     int m = 0;
     double *V = nullptr, *tmp = nullptr, *part = nullptr, *hdev = nullptr;
@@ -477,15 +482,47 @@ int residual(Mixed& c, double* berr, double* nberr, bool* xfinite) {
 }
 
 // X (init) or X + correction, from the residual Rin (n x nrhs, ld ldr)
+// The diagonal-block inverses of the single-RHS triangular solves, made once
+// per kind: 0 FP32 factors and FP32 vectors (IR correction), 1 FP32 factors
+// and FP64 vectors (GMRES preconditioner), 2 FP64 factors (after a fallback).
+int prepare(Mixed& c, int kind) {
+    if (c.prepared == kind) return OXI_OK;
+    if (kind == 0) OXI_TRY(oxi_trsv_prepare(c.h, c.s, 0, c.n, (const float*)c.A32, c.n, (float*)nullptr, c.ctrl));
+    else if (kind == 1) OXI_TRY(oxi_trsv_prepare(c.h, c.s, 0, c.n, (const float*)c.A32, c.n, (double*)nullptr, c.ctrl));
+    else OXI_TRY(oxi_trsv_prepare(c.h, c.s, 0, c.n, (const double*)c.E64, c.n, (double*)nullptr, c.ctrl));
+    c.prepared = kind;
+    return OXI_OK;
+}
+
 int correction(Mixed& c, const double* Rin, i64 ldr, bool init) {
+    // after the butterflies (no pivoting) the few-RHS correction solves run in
+    // FP64 arithmetic with the FP32 factors: the triangles of a factor without
+    // pivoting are less well conditioned, and FP32 block-inverse products
+    // would slow the contraction down
+    const bool wide = c.rbt && c.nrhs <= OXI_TRSV_MAX_RHS;
+    if (c.nrhs <= OXI_TRSV_MAX_RHS) OXI_TRY(prepare(c, c.fallback ? 2 : wide ? 1 : 0));
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, c.s, &c.stats[5], 1);
     hipLaunchKernelGGL(corr_scale_kernel, c.vgrid(c.nrhs), dim3(TPB), 0, c.s, Rin, ldr, c.n, c.nrhs, c.rows, c.S,
                        &c.stats[5], c.flags);
-    if (!c.fallback) {
+    if (wide) {
+        hipLaunchKernelGGL(corr_low_kernel<double>, c.vgrid(c.nrhs), dim3(TPB), 0, c.s, c.S, c.n, c.nrhs,
+                           &c.stats[5], c.Yd, c.magout, c.flags);
+        OXI_TRY(launch_ok(c.h, "correction"));
+        OXI_TRY(oxi_rbt_vec(c.h, c.s, 1, c.n, c.nrhs, c.Yd, c.n, c.R64));
+        for (i64 j = 0; j < c.nrhs; ++j) {
+            OXI_TRY(oxi_trsv(c.h, c.s, 0, c.n, (const float*)c.A32, c.n, c.Yd + j * c.n, c.ctrl));
+            OXI_TRY(oxi_trsv(c.h, c.s, 1, c.n, (const float*)c.A32, c.n, c.Yd + j * c.n, c.ctrl));
+        }
+        OXI_TRY(oxi_rbt_vec(c.h, c.s, 0, c.n, c.nrhs, c.Yd, c.n, c.R64));
+        hipLaunchKernelGGL(corr_back_kernel<double>, c.vgrid(c.nrhs), dim3(TPB), 0, c.s, c.Yd, c.n, c.nrhs, c.magout,
+                           c.cols, c.X, c.ldx, int(init));
+    } else if (!c.fallback) {
         hipLaunchKernelGGL(corr_low_kernel<float>, c.vgrid(c.nrhs), dim3(TPB), 0, c.s, c.S, c.n, c.nrhs, &c.stats[5],
                            c.Yf, c.magout, c.flags);
         OXI_TRY(launch_ok(c.h, "correction"));
+        if (c.rbt) OXI_TRY(oxi_rbt_vec(c.h, c.s, 1, c.n, c.nrhs, c.Yf, c.n, c.R32));
         OXI_TRY(oxi_getrs_perm_f32(c.h, c.s, 0, c.n, c.nrhs, c.A32, c.n, c.dperm, c.Yf, c.n, c.Ywf, c.ctrl));
+        if (c.rbt) OXI_TRY(oxi_rbt_vec(c.h, c.s, 0, c.n, c.nrhs, c.Yf, c.n, c.R32));
         hipLaunchKernelGGL(corr_back_kernel<float>, c.vgrid(c.nrhs), dim3(TPB), 0, c.s, c.Yf, c.n, c.nrhs, c.magout,
                            c.cols, c.X, c.ldx, int(init));
     } else {
@@ -501,12 +538,15 @@ int correction(Mixed& c, const double* Rin, i64 ldr, bool init) {
 
 // out = M^-1 v with M^-1 = Dc^-1 U^-1 L^-1 P Dr^-1 (low factors, FP64 arithmetic)
 int precond(Mixed& c, const double* v, double* out) {
+    OXI_TRY(prepare(c, c.fallback ? 2 : 1));
     const unsigned g = unsigned(cdiv(c.n, TPB));
     hipLaunchKernelGGL(prec_in_kernel, dim3(g), dim3(TPB), 0, c.s, v, c.n, c.dperm, c.rinv, out);
     OXI_TRY(launch_ok(c.h, "precond"));
     if (!c.fallback) {
+        if (c.rbt) OXI_TRY(oxi_rbt_vec(c.h, c.s, 1, c.n, 1, out, c.n, c.R64));
         OXI_TRY(oxi_trsv(c.h, c.s, 0, c.n, (const float*)c.A32, c.n, out, c.ctrl));
         OXI_TRY(oxi_trsv(c.h, c.s, 1, c.n, (const float*)c.A32, c.n, out, c.ctrl));
+        if (c.rbt) OXI_TRY(oxi_rbt_vec(c.h, c.s, 0, c.n, 1, out, c.n, c.R64));
     } else {
         OXI_TRY(oxi_trsv(c.h, c.s, 0, c.n, (const double*)c.E64, c.n, out, c.ctrl));
         OXI_TRY(oxi_trsv(c.h, c.s, 1, c.n, (const double*)c.E64, c.n, out, c.ctrl));
@@ -628,7 +668,7 @@ struct EventPair {
 
 extern "C" int oxi_dgesv_mixed(oxi_handle_t h, int prec, int solver, int64_t n, int64_t nrhs, const double* A,
                                int64_t lda, const double* B, int64_t ldb, double* X, int64_t ldx, int32_t maxiter,
-                               double rtol, int32_t gmres_restart, oxi_ir_info* info) {
+                               double rtol, int32_t gmres_restart, int32_t pivot, oxi_ir_info* info) {
     if (!h) return -1;
     if (prec < OXI_PREC_BF16 || prec > OXI_PREC_FP32) return -2;
     if (solver < 0 || solver > 2) return -3;
@@ -640,7 +680,8 @@ extern "C" int oxi_dgesv_mixed(oxi_handle_t h, int prec, int solver, int64_t n, 
     if (maxiter < 0) return -12;
     if (!std::isfinite(rtol) || rtol < 0) return -13;
     if (gmres_restart < 0 || gmres_restart > 1000) return -14;
-    if (!info) return -15;
+    if (pivot < 0 || pivot > 1) return -15;
+    if (!info) return -16;
     std::lock_guard<std::recursive_mutex> lk(h->guard);
     *info = oxi_ir_info{};
     info->converged = 1;
@@ -671,8 +712,22 @@ extern "C" int oxi_dgesv_mixed(oxi_handle_t h, int prec, int solver, int64_t n, 
                  oYwf = take(nr * 4);
     const size_t partial = std::max(size_t(c.nchunk) * nr, size_t(nrowchunk) * size_t(n));
     const size_t oP = take(partial * 8), oQ = take(partial * 8);
-    const size_t ostats = take(8 * 8), oflags = take(8 * 4), omag = take(8), octrl = take(16);
+    const size_t ostats = take(8 * 8), oflags = take(8 * 4), omag = take(8),
+                 octrl = take(oxi_trsv_ws_bytes(n, sizeof(double)));
     const size_t opart = take(size_t(c.nseg) * size_t(c.m + 1) * 8), ohdev = take(size_t(4 * (c.m + 1) + 1) * 8);
+    // the random butterfly transform applies to the FP32 factor and needs
+    // n % 4 == 0; otherwise the pivoted factor runs (factors rounded below
+    // FP32 without pivoting lose too much to unbounded multipliers)
+    const bool want_rbt = pivot == 1 && n % 4 == 0 && prec == OXI_PREC_FP32;
+    const int lprec = prec == OXI_PREC_FP32 ? -1 : prec;
+    const size_t nopiv_bytes = want_rbt ? oxi_getrf_nopiv_ws(lprec, n) : 0;
+    size_t oR32 = 0, oR64 = 0, onopiv = 0, odinfo = 0;
+    if (want_rbt) {
+        oR32 = take(size_t(4) * n * 4);
+        oR64 = take(size_t(4) * n * 8);
+        odinfo = take(8);
+        onopiv = take(nopiv_bytes);
+    }
     size_t oV = 0, otmp = 0;
     if (want_gmres) {
         oV = take(size_t(n) * size_t(c.m + 1) * 8);
@@ -693,6 +748,7 @@ extern "C" int oxi_dgesv_mixed(oxi_handle_t h, int prec, int solver, int64_t n, 
     c.ctrl = base + octrl;
     c.part = (double*)(base + opart); c.hdev = (double*)(base + ohdev);
     if (want_gmres) { c.V = (double*)(base + oV); c.tmp = (double*)(base + otmp); }
+    if (want_rbt) { c.R32 = (float*)(base + oR32); c.R64 = (double*)(base + oR64); }
 
     EventPair ev;
     for (auto& e : ev.e) OXI_HIP(h, hipEventCreate(&e));
@@ -711,9 +767,34 @@ extern "C" int oxi_dgesv_mixed(oxi_handle_t h, int prec, int solver, int64_t n, 
         OXI_TRY(read_stats(c, st, fl));
         if (fl[0]) return oxi_set_error(h, OXI_ERR_NOT_FINITE, "mixed_solve: finite mixed input required");
         c.anorm = from_bits(st[4]);
-        // 2. low-precision factor, FP64 fallback
+        // 2. low-precision factor (random butterflies and no pivoting, or
+        // partial pivoting), FP64 fallback
         i64 finfo = 0;
-        if (!fl[1]) OXI_TRY(oxi_getrf32(h, n, n, c.A32, n, c.ipiv, 0, &finfo, prec == OXI_PREC_FP32 ? -1 : prec));
+        if (!fl[1] && want_rbt) {
+            i64* dinfo = (i64*)(base + odinfo);
+            OXI_TRY(oxi_rbt_make(h, c.s, n, c.R32, c.R64));
+            OXI_TRY(oxi_rbt_matrix(h, c.s, n, c.A32, n, c.R32));
+            hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, c.s, (u64*)dinfo, 1);
+            OXI_TRY(launch_ok(h, "rbt"));
+            OXI_TRY(oxi_getrf_nopiv32(h, c.s, lprec, n, c.A32, n, dinfo, base + onopiv, nopiv_bytes));
+            i64 zinfo = 0;
+            OXI_HIP(h, hipMemcpyAsync(&zinfo, dinfo, 8, hipMemcpyDeviceToHost, c.s));
+            OXI_HIP(h, hipStreamSynchronize(c.s));
+            if (zinfo == 0) {
+                c.rbt = true;
+                std::vector<i64> ident(n);
+                for (i64 i = 0; i < n; ++i) ident[i] = i + 1;
+                OXI_HIP(h, hipMemcpyAsync(c.ipiv, ident.data(), size_t(n) * 8, hipMemcpyHostToDevice, c.s));
+                OXI_HIP(h, hipStreamSynchronize(c.s));
+            } else {
+                // a zero pivot after the transform: the pivoted factor of the original matrix
+                hipLaunchKernelGGL(equil_convert_kernel<float>,
+                                   dim3(unsigned(cdiv(n, TPB)), unsigned(std::min<i64>(n, 65535))), dim3(TPB), 0, c.s,
+                                   A, lda, n, n, c.rows, c.cols, c.A32, n, c.flags);
+                OXI_TRY(launch_ok(h, "equilibrate"));
+            }
+        }
+        if (!fl[1] && !c.rbt) OXI_TRY(oxi_getrf32(h, n, n, c.A32, n, c.ipiv, 0, &finfo, lprec));
         if (fl[1] || finfo != 0) {
             c.fallback = true;
             if (hipMalloc(&c.E64, nn * 8) != hipSuccess) {

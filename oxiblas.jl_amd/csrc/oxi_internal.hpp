@@ -95,6 +95,16 @@ int oxi_getrs_perm_f64(oxi_handle_t h, hipStream_t s, int trans, i64 n, i64 nrhs
                        const i64* dperm, double* B, i64 ldb, double* Y, void* ctrl);
 int oxi_getrs_perm_f32(oxi_handle_t h, hipStream_t s, int trans, i64 n, i64 nrhs, const float* LU, i64 lda,
                        const i64* dperm, float* B, i64 ldb, float* Y, void* ctrl);
+// Random butterfly transform and LU without pivoting (nopiv.hip).  R32 / R64:
+// 4n butterfly entries (level 0 U, level 0 V, level 1 U, level 1 V).
+int oxi_rbt_make(oxi_handle_t h, hipStream_t s, i64 n, float* R32, double* R64);
+int oxi_rbt_matrix(oxi_handle_t h, hipStream_t s, i64 n, float* A, i64 lda, const float* R32);
+// X <- U^T X (ut = 1) or X <- V X (ut = 0)
+int oxi_rbt_vec(oxi_handle_t h, hipStream_t s, int ut, i64 n, i64 ncols, float* X, i64 ldx, const float* R32);
+int oxi_rbt_vec(oxi_handle_t h, hipStream_t s, int ut, i64 n, i64 ncols, double* X, i64 ldx, const double* R64);
+size_t oxi_getrf_nopiv_ws(int prec, i64 n);
+int oxi_getrf_nopiv32(oxi_handle_t h, hipStream_t s, int prec, i64 n, float* A, i64 lda, i64* dinfo, void* ws,
+                      size_t ws_bytes);
 // FP32 getrf with a trailing update in prec (-1 FP32, 0 BF16, 1 FP16, 2 FP8)
 int oxi_getrf32(oxi_handle_t h, i64 m, i64 n, float* A, i64 lda, i64* ipiv, i64 nb, i64* info, int prec);
 
@@ -102,12 +112,23 @@ int oxi_getrf32(oxi_handle_t h, i64 m, i64 n, float* A, i64 lda, i64* ipiv, i64 
 // synchronises the stream.
 // Single right-hand-side triangular solve against LU factors (trsv.hip).
 // which: 0 = L unit forward, 1 = U backward, 2 = U^T forward, 3 = L^T unit
-// backward.  ctrl is 16 bytes of device scratch; the kernel's bounded waits
-// record a timeout there, read by oxi_trsv_check (which synchronises).
+// backward.  ctrl is the trsv workspace below, prepared for these factors;
+// the kernel's bounded waits record a timeout there, read by oxi_trsv_check
+// (which synchronises).
 int oxi_trsv(oxi_handle_t h, hipStream_t s, int which, i64 n, const double* M, i64 ld, double* x, void* ctrl);
 int oxi_trsv(oxi_handle_t h, hipStream_t s, int which, i64 n, const float* M, i64 ld, float* x, void* ctrl);
 int oxi_trsv(oxi_handle_t h, hipStream_t s, int which, i64 n, const float* M, i64 ld, double* x, void* ctrl);
 int oxi_trsv_check(oxi_handle_t h, hipStream_t s, const void* ctrl);
+// The trsv workspace (ctrl above): control words and the inverses of the
+// 128 x 128 diagonal blocks of the two triangles of a solve, in the vector
+// type (tv bytes).  oxi_trsv_prepare fills them for the solve kind tk
+// (0: L then U, 1: U^T then L^T) of the factors M; the vector pointer only
+// selects the vector type.  Every later oxi_trsv on the same factors and kind
+// uses them.
+size_t oxi_trsv_ws_bytes(i64 n, size_t tv);
+int oxi_trsv_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const double* M, i64 ld, double*, void* ws);
+int oxi_trsv_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const float* M, i64 ld, float*, void* ws);
+int oxi_trsv_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const float* M, i64 ld, double*, void* ws);
 // LU solves with at most this many right-hand sides go column by column
 // through oxi_trsv (when the caller supplies ctrl), more through trsm.
 constexpr i64 OXI_TRSV_MAX_RHS = 4;

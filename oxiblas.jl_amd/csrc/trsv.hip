@@ -2,20 +2,29 @@
 // trsv!(x, A; uplo, trans, unit)  (src/blas/level2.jl trsv!, the vector branch
 // of the LU solve, src/api/linalg_api.jl:168-184).
 //
-// One launch per triangle.  The n rows are cut into blocks of RB=128 rows in
-// solve order; block b needs x of every earlier block.  Workgroups take block
-// numbers from an atomic ticket, so a workgroup only ever waits on blocks
-// owned by workgroups that are already running (no dependence on dispatch
-// order, no deadlock).  A workgroup
-//   1. stages its RB x RB diagonal block in LDS and inverts its diagonal,
+// The n rows are cut into blocks of RB=128 rows in solve order; block b needs
+// x of every earlier block.
+//
+// Preparation (oxi_trsv_prepare, once per factor and triangle): the inverse
+// of every RB x RB diagonal block, computed by parallel column sweeps, in the
+// arithmetic type.  A solve then does no sequential sweep: each block's x is
+// a matrix-vector product with its inverse.
+//
+// The solve (one launch per triangle): workgroups take block numbers from an
+// atomic ticket, so a workgroup only ever waits on blocks owned by
+// workgroups that are already running (no dependence on dispatch order, no
+// deadlock).  A workgroup
+//   1. stages the inverse of its diagonal block in LDS,
 //   2. streams the off-diagonal part of its block row, chunk by chunk, as the
 //      earlier blocks publish their x (one agent-scope `ready` counter that
 //      only grows: block b stores b+1 after its x is written),
-//   3. reduces the partial sums, and one wave solves the diagonal block by
-//      column sweep (lane i owns rows i and i+64; x_k broadcast by readlane),
+//   3. reduces the partial sums into r = b_blk - sum and forms x = inv * r,
 //   4. writes its x and then bumps `ready`.
 // The matrix is read once (HBM-bound: n^2/2 elements per triangle); the
-// dependence chain is the n column-sweep steps plus one publish per block.
+// dependence chain is one publish, one x fetch and one small product per
+// block.  Using the block inverse instead of substitution inside the block
+// changes the rounding (not the result beyond it); the solve stays
+// deterministic.
 //
 // The matrix type TM and the arithmetic/vector type TV may differ: FP32
 // factors solved in FP64 arithmetic is the preconditioner of GMRES-IR.
@@ -65,9 +74,87 @@ __global__ void trsv_ctrl_reset(TrsvCtrl* c) {
     if (threadIdx.x == 0) { c->ticket = 0; c->ready = 0; c->err = 0; c->pad = 0; }
 }
 
+// The diagonal block of block b, D[k * (RB + 1) + i] = T(i, k) in logical
+// (solve-order) indices, into LDS.
+template <typename TM, bool TRANS, bool BWD>
+__device__ __forceinline__ void load_diag(const TM* __restrict__ M, i64 ld, i64 p0, int rows, TM* D, int t) {
+    for (int e = t; e < RB * RB; e += NT) {
+        const int pr = e % RB, pc = e / RB;
+        if (pr < rows && pc < rows) {
+            const TM v = M[(p0 + pr) + (p0 + pc) * ld];
+            const int lr = BWD ? rows - 1 - pr : pr, lc = BWD ? rows - 1 - pc : pc;
+            const int i = TRANS ? lc : lr, k = TRANS ? lr : lc;
+            D[k * (RB + 1) + i] = v;
+        }
+    }
+}
+
+// Inverse of every diagonal block (logical indices, lower triangular):
+// inv[b][k * RB + i] = Dinv(i, k), zero above the diagonal and outside the
+// block.  Wave w solves T X = I for columns 16w .. 16w+15, four at a time, by
+// column sweep (lane i owns rows i and i+64; x_k broadcast by readlane).
+template <typename TM, typename TV, bool TRANS, bool UNIT, bool BWD>
+__global__ __launch_bounds__(NT) void trsv_dinv_kernel(const TM* __restrict__ M, i64 ld, i64 n, TV* __restrict__ inv) {
+    __shared__ TM D[RB * (RB + 1)];
+    __shared__ TV rinv[RB];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int b = blockIdx.x;
+    const i64 lo = i64(b) * RB;
+    const int rows = int(min(i64(RB), n - lo));
+    const i64 p0 = BWD ? n - lo - rows : lo;
+    load_diag<TM, TRANS, BWD>(M, ld, p0, rows, D, t);
+    __syncthreads();
+    if (!UNIT)
+        for (int k = t; k < rows; k += NT) rinv[k] = TV(1) / TV(D[k * (RB + 1) + k]);
+    __syncthreads();
+    TV* out = inv + size_t(b) * RB * RB;
+    constexpr int CPW = RB / (NT / 64);   // columns per wave
+    for (int c0 = CPW * w; c0 < CPW * (w + 1); c0 += 4) {
+        TV s[4][2];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            s[r][0] = lane == c0 + r ? TV(1) : TV(0);
+            s[r][1] = lane + 64 == c0 + r ? TV(1) : TV(0);
+        }
+        for (int k = c0; k < rows; ++k) {
+            const TV d0 = TV(D[k * (RB + 1) + lane]), d1 = TV(D[k * (RB + 1) + 64 + lane]);
+            const TV rk = UNIT ? TV(1) : rinv[k];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const TV sk = k < 64 ? s[r][0] : s[r][1];
+                const TV xk = bcast(UNIT ? sk : sk * rk, k & 63);
+                if (lane > k) s[r][0] = fma_t(-d0, xk, s[r][0]);
+                if (lane + 64 > k) s[r][1] = fma_t(-d1, xk, s[r][1]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int c = c0 + r;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int i = lane + 64 * h;
+                TV v = TV(0);
+                if (c < rows && i < rows) v = UNIT ? s[r][h] : s[r][h] * rinv[i];
+                out[c * RB + i] = v;
+            }
+        }
+    }
+}
+
+// Two device-scope loads issued back to back and waited for once (relaxed
+// atomic loads written in C++ are issued one at a time, each waited for).
+__device__ __forceinline__ void load2_agent(const double* p0, const double* p1, double& a, double& b) {
+    asm volatile("global_load_dwordx2 %0, %2, off sc1\n\tglobal_load_dwordx2 %1, %3, off sc1\n\ts_waitcnt vmcnt(0)"
+                 : "=&v"(a), "=&v"(b) : "v"(p0), "v"(p1) : "memory");
+}
+__device__ __forceinline__ void load2_agent(const float* p0, const float* p1, float& a, float& b) {
+    asm volatile("global_load_dword %0, %2, off sc1\n\tglobal_load_dword %1, %3, off sc1\n\ts_waitcnt vmcnt(0)"
+                 : "=&v"(a), "=&v"(b) : "v"(p0), "v"(p1) : "memory");
+}
+
 template <typename TM, typename TV, bool TRANS, bool UNIT, bool BWD, bool VECOK>
 __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M, i64 ld, i64 n, TV* x,
-                                                        TrsvCtrl* ctrl) {
+                                                        TrsvCtrl* ctrl, const TV* __restrict__ inv) {
     constexpr int VEC = VECOK ? Vec<TM>::N : 1;
     constexpr int LPC = RB / VEC;              // no-trans: lanes per column
     constexpr int CG = NT / LPC;               // no-trans: column groups
@@ -75,9 +162,11 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
     constexpr int LPR = 8;                     // trans: lanes per row
     constexpr int RPT = RB / (NT / LPR);       // trans: rows per thread
     constexpr int ITS = RB / (LPR * VEC);      // trans: iterations per chunk
-    __shared__ TM D[RB * (RB + 1)];            // D[k][i] = T(i, k), logical in-block indices
-    __shared__ TV red[NRED][RB];
-    __shared__ TV rinv[RB];
+    constexpr int KP = NT / RB;                // product: threads per row
+    static_assert(XB * RB == 2 * NT, "two x loads per thread per round");
+    __shared__ TV DI[RB * RB];                 // DI[k * RB + i] = inverse(i, k), logical indices
+    __shared__ TV red[NRED > KP ? NRED : KP][RB];
+    __shared__ TV rv[RB];
     __shared__ TV xs[XB * RB];                 // x of the chunks being applied
     __shared__ int s_b, s_ready;
 
@@ -89,19 +178,11 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
     const int rows = int(min(i64(RB), n - lo));
     const i64 p0 = BWD ? n - lo - rows : lo;   // first physical row of the block
 
-    // 1. diagonal block -> LDS (physical offsets pr, pc; stored by logical index)
-    for (int e = t; e < RB * RB; e += NT) {
-        const int pr = e % RB, pc = e / RB;
-        if (pr < rows && pc < rows) {
-            const TM v = M[(p0 + pr) + (p0 + pc) * ld];
-            const int lr = BWD ? rows - 1 - pr : pr, lc = BWD ? rows - 1 - pc : pc;
-            const int i = TRANS ? lc : lr, k = TRANS ? lr : lc;
-            D[k * (RB + 1) + i] = v;
-        }
+    // 1. the inverse of the diagonal block -> LDS
+    {
+        const TV* src = inv + size_t(b) * RB * RB;
+        for (int e = t; e < RB * RB; e += NT) DI[e] = src[e];
     }
-    __syncthreads();
-    if (!UNIT)
-        for (int k = t; k < rows; k += NT) rinv[k] = TV(1) / TV(D[k * (RB + 1) + k]);
 
     // 2. off-diagonal block row, chunk by chunk as earlier blocks publish
     TV acc[TRANS ? RPT : VEC];
@@ -120,7 +201,7 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
                         v = 0;
                         break;
                     }
-                    __builtin_amdgcn_s_sleep(2);
+                    __builtin_amdgcn_s_sleep(1);
                 }
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 s_ready = int(v);
@@ -131,12 +212,21 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
             if (avail <= c) return;            // timed out: uniform exit
         }
         // x of up to XB published chunks into LDS with one round of
-        // device-scope loads; xs[k * RB + e] = element e (physical order) of chunk c + k
+        // device-scope loads (two per thread, issued together);
+        // xs[k * RB + e] = element e (physical order) of chunk c + k
         const int nbat = min(min(avail, b) - c, XB);
-        for (int e = t; e < nbat * RB; e += NT) {
-            const int k = e / RB, ee = e - k * RB;
-            const i64 qk = BWD ? n - i64(c + k + 1) * RB : i64(c + k) * RB;
-            xs[e] = get_agent(&x[qk + ee]);
+        {
+            const int e0 = t, e1 = t + NT;
+            const int k0 = e0 / RB, k1 = e1 / RB;
+            const i64 q0 = BWD ? n - i64(c + k0 + 1) * RB : i64(c + k0) * RB;
+            const i64 q1 = BWD ? n - i64(c + k1 + 1) * RB : i64(c + k1) * RB;
+            // entries beyond the batch read the first entry again (in bounds) and are not used
+            const TV* a0 = k0 < nbat ? &x[q0 + (e0 - k0 * RB)] : &x[BWD ? n - i64(c + 1) * RB : i64(c) * RB];
+            const TV* a1 = k1 < nbat ? &x[q1 + (e1 - k1 * RB)] : a0;
+            TV v0, v1;
+            load2_agent(a0, a1, v0, v1);
+            if (k0 < nbat) xs[e0] = v0;
+            if (k1 < nbat) xs[e1] = v1;
         }
         __syncthreads();
         for (int k = 0; k < nbat; ++k) {
@@ -189,7 +279,7 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
         c += nbat;
     }
 
-    // 3. partial sums -> LDS (fixed order: deterministic)
+    // 3. partial sums -> LDS (fixed order: deterministic), r = b_blk - sum
     if constexpr (!TRANS) {
         const int pr = (t % LPC) * VEC, cg = t / LPC;
 #pragma unroll
@@ -205,33 +295,38 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
         }
     }
     __syncthreads();
-    if (t >= 64) return;
-
-    // 4. diagonal block by column sweep in wave 0
-    TV s[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int i = t + 64 * h;
-        s[h] = TV(0);
+    if (t < RB) {
+        const int i = t;   // logical
+        TV r = TV(0);
         if (i < rows) {
             const int pr = BWD ? rows - 1 - i : i;
             TV sum = red[0][pr];
 #pragma unroll
             for (int g = 1; g < NRED; ++g) sum += red[g][pr];
-            s[h] = x[p0 + pr] - sum;
+            r = x[p0 + pr] - sum;
         }
+        rv[i] = r;
     }
-    for (int k = 0; k < rows; ++k) {
-        const TV sk = k < 64 ? s[0] : s[1];
-        const TV xk = bcast(UNIT ? sk : sk * rinv[k], k & 63);
-        const TV d0 = TV(D[k * (RB + 1) + t]), d1 = TV(D[k * (RB + 1) + 64 + t]);
-        if (t > k) s[0] = fma_t(-d0, xk, s[0]);
-        if (t + 64 > k) s[1] = fma_t(-d1, xk, s[1]);
+    __syncthreads();
+
+    // 4. x = inverse * r: KP threads per row over interleaved columns, then wave 0 combines
+    {
+        const int i = t % RB, part = t / RB;
+        TV a = TV(0);
+        for (int k = part; k <= i; k += KP) a = fma_t(DI[k * RB + i], rv[k], a);
+        red[part][i] = a;
     }
+    __syncthreads();
+    if (t >= 64) return;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int i = t + 64 * h;
-        if (i < rows) put_agent(&x[p0 + (BWD ? rows - 1 - i : i)], UNIT ? s[h] : s[h] * rinv[i]);
+        if (i < rows) {
+            TV xi = red[0][i];
+#pragma unroll
+            for (int g = 1; g < KP; ++g) xi += red[g][i];
+            put_agent(&x[p0 + (BWD ? rows - 1 - i : i)], xi);
+        }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_s_waitcnt(0);   // x of this block acknowledged before `ready` moves
@@ -239,29 +334,56 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
 }
 
 template <typename TM, typename TV, bool TRANS, bool UNIT, bool BWD>
-void launch_dir(hipStream_t s, i64 n, const TM* M, i64 ld, TV* x, TrsvCtrl* c, bool vecok) {
+void launch_dir(hipStream_t s, i64 n, const TM* M, i64 ld, TV* x, TrsvCtrl* c, const TV* inv, bool vecok) {
     const unsigned grid = unsigned(cdiv(n, RB));
     if (vecok)
         hipLaunchKernelGGL((trsv_chain_kernel<TM, TV, TRANS, UNIT, BWD, true>), dim3(grid), dim3(NT), 0, s, M, ld, n,
-                           x, c);
+                           x, c, inv);
     else
         hipLaunchKernelGGL((trsv_chain_kernel<TM, TV, TRANS, UNIT, BWD, false>), dim3(grid), dim3(NT), 0, s, M, ld,
-                           n, x, c);
+                           n, x, c, inv);
+}
+
+// Workspace: the control words, then the block inverses of the two triangles
+// of a solve (slot 0: the first triangle, slot 1: the second).
+size_t slot_bytes(i64 n, size_t tv) { return (size_t(cdiv(n, RB)) * RB * RB * tv + 255) & ~size_t(255); }
+template <typename TV>
+TV* slot_of(void* ws, i64 n, int slot) {
+    return (TV*)((char*)ws + 256 + size_t(slot) * slot_bytes(n, sizeof(TV)));
 }
 
 template <typename TM, typename TV>
-int trsv_t(oxi_handle_t h, hipStream_t s, int which, i64 n, const TM* M, i64 ld, TV* x, void* ctrl) {
+int prepare_t(oxi_handle_t h, hipStream_t s, int tk, i64 n, const TM* M, i64 ld, void* ws) {
+    if (n <= 0) return OXI_OK;
+    const unsigned grid = unsigned(cdiv(n, RB));
+    TV* i0 = slot_of<TV>(ws, n, 0);
+    TV* i1 = slot_of<TV>(ws, n, 1);
+    if (!tk) {
+        hipLaunchKernelGGL((trsv_dinv_kernel<TM, TV, false, true, false>), dim3(grid), dim3(NT), 0, s, M, ld, n, i0);
+        hipLaunchKernelGGL((trsv_dinv_kernel<TM, TV, false, false, true>), dim3(grid), dim3(NT), 0, s, M, ld, n, i1);
+    } else {
+        hipLaunchKernelGGL((trsv_dinv_kernel<TM, TV, true, false, false>), dim3(grid), dim3(NT), 0, s, M, ld, n, i0);
+        hipLaunchKernelGGL((trsv_dinv_kernel<TM, TV, true, true, true>), dim3(grid), dim3(NT), 0, s, M, ld, n, i1);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "trsv prepare launch: %s", hipGetErrorString(e));
+    return OXI_OK;
+}
+
+template <typename TM, typename TV>
+int trsv_t(oxi_handle_t h, hipStream_t s, int which, i64 n, const TM* M, i64 ld, TV* x, void* ws) {
     if (n <= 0) return OXI_OK;
     if (n > i64(RB) * 0x7fffffff) return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "trsv: n too large");
-    TrsvCtrl* c = static_cast<TrsvCtrl*>(ctrl);
+    TrsvCtrl* c = static_cast<TrsvCtrl*>(ws);
     constexpr int VEC = Vec<TM>::N;
     const bool vecok = (reinterpret_cast<uintptr_t>(M) % 16 == 0) && (ld % VEC == 0) && (n % VEC == 0);
+    const TV* inv = slot_of<TV>(ws, n, (which == 0 || which == 2) ? 0 : 1);
     hipLaunchKernelGGL(trsv_ctrl_reset, dim3(1), dim3(64), 0, s, c);
     switch (which) {
-        case 0: launch_dir<TM, TV, false, true, false>(s, n, M, ld, x, c, vecok); break;
-        case 1: launch_dir<TM, TV, false, false, true>(s, n, M, ld, x, c, vecok); break;
-        case 2: launch_dir<TM, TV, true, false, false>(s, n, M, ld, x, c, vecok); break;
-        case 3: launch_dir<TM, TV, true, true, true>(s, n, M, ld, x, c, vecok); break;
+        case 0: launch_dir<TM, TV, false, true, false>(s, n, M, ld, x, c, inv, vecok); break;
+        case 1: launch_dir<TM, TV, false, false, true>(s, n, M, ld, x, c, inv, vecok); break;
+        case 2: launch_dir<TM, TV, true, false, false>(s, n, M, ld, x, c, inv, vecok); break;
+        case 3: launch_dir<TM, TV, true, true, true>(s, n, M, ld, x, c, inv, vecok); break;
         default: return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "trsv: bad triangle %d", which);
     }
     hipError_t e = hipGetLastError();
@@ -270,6 +392,18 @@ int trsv_t(oxi_handle_t h, hipStream_t s, int which, i64 n, const TM* M, i64 ld,
 }
 
 }  // namespace
+
+size_t oxi_trsv_ws_bytes(i64 n, size_t tv) { return 256 + 2 * slot_bytes(std::max<i64>(n, 1), tv); }
+
+int oxi_trsv_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const double* M, i64 ld, double*, void* ws) {
+    return prepare_t<double, double>(h, s, tk, n, M, ld, ws);
+}
+int oxi_trsv_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const float* M, i64 ld, float*, void* ws) {
+    return prepare_t<float, float>(h, s, tk, n, M, ld, ws);
+}
+int oxi_trsv_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const float* M, i64 ld, double*, void* ws) {
+    return prepare_t<float, double>(h, s, tk, n, M, ld, ws);
+}
 
 int oxi_trsv(oxi_handle_t h, hipStream_t s, int which, i64 n, const double* M, i64 ld, double* x, void* ctrl) {
     return trsv_t<double, double>(h, s, which, n, M, ld, x, ctrl);

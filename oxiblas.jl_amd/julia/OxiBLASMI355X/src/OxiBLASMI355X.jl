@@ -364,7 +364,7 @@ classical refinement stalls.  Returns the reference's NamedTuple.
 """
 function mixed_solve(A::StridedMatrix{Float64}, b::StridedVecOrMat{Float64}; maxiter::Integer=20,
                      rtol::Real=16eps(Float64), lowtype=Float32, solver::Symbol=:ir, gmres_restart::Integer=0,
-                     factorization::Symbol=:lu)
+                     factorization::Symbol=:lu, pivot::Symbol=:partial)
     factorization === :lu || throw(ArgumentError("the GPU mixed solve implements factorization=:lu"))
     n = size(A, 1); size(A, 2) == n || throw(DimensionMismatch("mixed factorization dimensions differ"))
     B = reshape(b, size(b, 1), :); k = size(B, 2)
@@ -373,9 +373,10 @@ function mixed_solve(A::StridedMatrix{Float64}, b::StridedVecOrMat{Float64}; max
     h = handle()
     rc = ccall((:oxi_dgesv_mixed, LIB), Cint,
                (Ptr{Cvoid}, Cint, Cint, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64,
-                Int32, Float64, Int32, Ref{IRInfo}),
+                Int32, Float64, Int32, Int32, Ref{IRInfo}),
                h, _prec(lowtype), Cint(solver === :ir ? 0 : solver === :gmres ? 1 : 2), n, k, Ad.ptr, Ad.ld, Bd.ptr,
-               Bd.ld, Xd.ptr, Xd.ld, Int32(maxiter), Float64(rtol), Int32(gmres_restart), info)
+               Bd.ld, Xd.ptr, Xd.ld, Int32(maxiter), Float64(rtol), Int32(gmres_restart),
+               Int32(pivot === :rbt ? 1 : 0), info)
     rc == 6 && throw(ArgumentError("finite mixed input required"))
     _check(h, rc, "mixed_solve")
     r = info[]

@@ -69,7 +69,7 @@ SIGNATURES = {
     "oxi_sgetrf_lowp": (C.c_int, [_vp, C.c_int, _i64, _i64, _vp, _i64, _vp, _i64, C.POINTER(_i64)]),
     "oxi_sgetrs": (C.c_int, [_vp, _ch, _i64, _i64, _vp, _i64, _vp, _vp, _i64]),
     "oxi_dgesv_mixed": (C.c_int, [_vp, C.c_int, C.c_int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, C.c_int32,
-                                  _dbl, C.c_int32, C.POINTER(IRInfo)]),
+                                  _dbl, C.c_int32, C.c_int32, C.POINTER(IRInfo)]),
     "oxi_dequilibrate": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, C.c_int]),
     "oxi_profile_enable": (C.c_int, [_vp, C.c_int]),
     "oxi_profile_read": (C.c_int, [_vp, C.POINTER(_dbl), C.POINTER(_dbl), C.POINTER(_i64)]),

@@ -40,6 +40,7 @@ _PREC_NAMES = {
 _PREC_LABEL = {_lib.OXI_PREC_FP32: "Float32", _lib.OXI_PREC_BF16: "BFloat16",
                _lib.OXI_PREC_FP16: "Float16", _lib.OXI_PREC_FP8: "Float8_E4M3FN"}
 _SOLVERS = {"ir": 0, "gmres": 1, "gmres_ir": 1, "auto": 2}
+_PIVOTS = {"partial": 0, "rbt": 1}
 _REASONS = ("converged", "iteration_limit", "nonfinite_iteration", "working_factor_fallback")
 
 MixedSolveResult = namedtuple(
@@ -73,7 +74,7 @@ def _singular_from(h):
 
 
 def mixed_solve(A, b, maxiter=20, rtol=16 * EPS, lowtype=None, fallback=True, factorization="lu", solver="ir",
-                gmres_restart=0, threaded=True):
+                gmres_restart=0, threaded=True, pivot="partial"):
     """mixed_solve(A, b; maxiter=20, rtol=16eps, lowtype, fallback=true,
     factorization=:lu) (src/lapack/extended_lapack.jl:175-226).
 
@@ -82,7 +83,11 @@ def mixed_solve(A, b, maxiter=20, rtol=16 * EPS, lowtype=None, fallback=True, fa
     classical iteration; "gmres" runs GMRES-IR (the low factors as an FP64
     preconditioner) for condition numbers where the classical iteration
     stalls; "auto" switches to GMRES-IR once the error stops halving.
-    Returns the reference's NamedTuple fields (x, converged, iterations,
+    pivot "partial" factors with partial pivoting as the reference; "rbt"
+    applies a random butterfly transform and factors without pivoting (no
+    device-wide pivot search per column; refinement restores the accuracy).
+    "rbt" applies to the Float32 factor; other low types, n not a multiple of
+    4 or a zero pivot use partial pivoting.  Returns the reference's NamedTuple fields (x, converged, iterations,
     residual, backward_error, metric, low_type, fallback, factorization,
     finite, reason) plus the GPU's normwise error, GMRES iteration count and
     device timings."""
@@ -94,6 +99,8 @@ def mixed_solve(A, b, maxiter=20, rtol=16 * EPS, lowtype=None, fallback=True, fa
         raise ValueError("invalid mixed solve options")
     if solver not in _SOLVERS:
         raise ValueError("solver must be ir, gmres or auto")
+    if pivot not in _PIVOTS:
+        raise ValueError("pivot must be partial or rbt")
     prec = precision_code(lowtype)
     vec = not isinstance(b, DMatrix) and np.ndim(b) == 1
     Ad, Bd = as_dmatrix(A), as_dmatrix(b)
@@ -106,7 +113,7 @@ def mixed_solve(A, b, maxiter=20, rtol=16 * EPS, lowtype=None, fallback=True, fa
     info = _lib.IRInfo()
     h = _h()
     rc = h._L.oxi_dgesv_mixed(h.ptr, prec, _SOLVERS[solver], n, nrhs, Ad.ptr, Ad.ld, Bd.ptr, Bd.ld, Xd.ptr, Xd.ld,
-                              int(maxiter), float(rtol), int(gmres_restart), C.byref(info))
+                              int(maxiter), float(rtol), int(gmres_restart), _PIVOTS[pivot], C.byref(info))
     if rc == _lib.OXI_ERR_NOT_FINITE:
         raise ValueError("finite mixed input required")
     if rc == _lib.OXI_ERR_SINGULAR:

@@ -233,6 +233,61 @@ def test_mixed_solve_multiple_rhs(gpu, rng):
         assert r.converged
 
 
+# ---------------------------------------------------------------------------
+# mixed_solve with two-level random butterflies and a factor without pivoting
+# (pivot="rbt"): a wrong transform or a mismatch between the transform of the
+# matrix and of the corrections would leave the refinement without a
+# contraction, so convergence to 16 eps pins the whole chain
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [4, 8, 64, 300, 301, 1024])
+def test_mixed_solve_rbt_contract(gpu, rng, n):
+    A = rng.standard_normal((n, n)) + n * np.eye(n); b = rng.standard_normal(n)
+    r = gpu.mixed_solve(A, b, pivot="rbt")
+    assert r.converged and not r.fallback and r.backward_error <= 16 * EPS and r.iterations <= 10, r
+    o = O.mixed_solve(A, b, maxiter=20, rtol=16 * EPS)
+    np.testing.assert_allclose(r.x, o["x"], rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("prec,solver,kappa", [("float32", "gmres", 1e6), ("float32", "ir", 1e3),
+                                               ("float32", "auto", 1e5),
+                                               # below Float32 the option keeps partial pivoting
+                                               ("bfloat16", "gmres", 1e2), ("float16", "gmres", 1e3)])
+def test_mixed_solve_rbt_conditioned(gpu, rng, prec, solver, kappa):
+    n = 1024
+    A = conditioned(rng, n, kappa); xt = rng.standard_normal(n); b = A @ xt
+    r = gpu.mixed_solve(A, b, lowtype=prec, solver=solver, maxiter=30, gmres_restart=50, pivot="rbt")
+    assert r.converged and r.backward_error <= 16 * EPS and not r.fallback, r
+    assert np.linalg.norm(r.x - xt) / np.linalg.norm(xt) <= 100 * kappa * EPS
+    _, e = O.refinement_residual(A, r.x, b)
+    assert e <= 16 * EPS
+
+
+def test_mixed_solve_rbt_zero_diagonal(gpu, rng):
+    # every diagonal entry is zero: LU without pivoting of A itself stops at
+    # the first step; after the butterflies the factor needs no interchange
+    n = 512
+    A = 3.0 * np.roll(np.eye(n), 1, axis=1) + 0.1 * rng.standard_normal((n, n))
+    np.fill_diagonal(A, 0.0)
+    b = rng.standard_normal(n)
+    r = gpu.mixed_solve(A, b, pivot="rbt", solver="gmres")
+    assert r.converged and r.backward_error <= 16 * EPS, r
+
+
+def test_mixed_solve_rbt_multiple_rhs_and_determinism(gpu, rng):
+    n = 512
+    A = conditioned(rng, n, 1e3)
+    for nrhs in (3, 9):
+        B = rng.standard_normal((n, nrhs))
+        r = gpu.mixed_solve(A, B, pivot="rbt")
+        assert r.converged and r.x.shape == (n, nrhs)
+        _, e = O.refinement_residual(A, r.x, B)
+        assert e <= 16 * EPS
+        r2 = gpu.mixed_solve(A, B, pivot="rbt")
+        assert np.array_equal(r.x, r2.x)
+    with pytest.raises(ValueError):
+        gpu.mixed_solve(A, B, pivot="complete")
+
+
 def test_mixed_solve_fallback_and_errors(gpu, rng):
     # singular after rounding to Float32: the FP64 factor is used
     n = 64
