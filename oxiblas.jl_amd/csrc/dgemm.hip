@@ -7,35 +7,34 @@
 // coalesced global loads, double-buffered so that the next slab is in flight
 // while the matrix cores consume the current one.
 //
-// Work decomposition: a 256-thread workgroup owns a 128x128 tile of C; its
-// four waves each own 64x64, computed with v_mfma_f64_4x4x4_4b (the FP64
-// matrix instruction that reaches the full rate on gfx950; 64 FP64
-// accumulators per lane).  The epilogue applies alpha and beta as the
+// Work decomposition: a 256-thread workgroup owns a BM x BN tile of C (64 x 64
+// for most shapes, 32 x 32 where the grid would not fill the CUs); its four
+// waves split it 2 x 2 and compute with v_mfma_f64_4x4x4_4b (the FP64 matrix
+// instruction that reaches the full rate on gfx950).  The epilogue applies alpha and beta as the
 // reference's full tiles do: v = alpha*acc; C = beta == 0 ? v : fma(beta, C, v)
 // (src/blas/simd.jl:190-196); beta == 0 never reads C.
 #include "oxi_internal.hpp"
 
 namespace {
 
-constexpr int BM = 128, BN = 128, BK = 32, NT = 256;
+constexpr int BK = 32, NT = 256;
 constexpr int PAD = 1;                 // one double of padding per LDS row
-constexpr int LDS_A = BM + PAD, LDS_B = BN + PAD;
-constexpr int LOADS = BM * BK / NT;    // doubles per thread per operand per slab (16)
 
-// Load one BK-deep slab of an operand tile into registers.
+// Load one BK-deep slab of an operand tile (O rows of op(X)) into registers.
 //   OUTER_CONTIG: element (o, kk) at X[(o0+o) + (k0+kk)*ldx] (op(A) for A 'N',
 //                 op(B) for B 'T'); otherwise at X[(k0+kk) + (o0+o)*ldx].
 // Out-of-range elements are zero so partial tiles need no special casing in
 // the MFMA loop.
-template <bool OUTER_CONTIG>
-__device__ __forceinline__ void load_slab(double (&r)[LOADS], const double* __restrict__ X, i64 ldx, i64 o0, i64 k0,
-                                          i64 outer, i64 K, int t) {
-    if (o0 + BM <= outer && k0 + BK <= K) {
+template <bool OUTER_CONTIG, int O>
+__device__ __forceinline__ void load_slab(double (&r)[O * BK / NT], const double* __restrict__ X, i64 ldx, i64 o0,
+                                          i64 k0, i64 outer, i64 K, int t) {
+    constexpr int LOADS = O * BK / NT;
+    if (o0 + O <= outer && k0 + BK <= K) {
         // interior slab: no bounds checks (the common case)
 #pragma unroll
         for (int i = 0; i < LOADS; ++i) {
             int o, kk;
-            if (OUTER_CONTIG) { o = t % BM; kk = t / BM + i * (NT / BM); }
+            if (OUTER_CONTIG) { o = t % O; kk = t / O + i * (NT / O); }
             else { kk = t % BK; o = t / BK + i * (NT / BK); }
             const i64 go = o0 + o, gk = k0 + kk;
             r[i] = OUTER_CONTIG ? X[go + gk * ldx] : X[gk + go * ldx];
@@ -45,19 +44,20 @@ __device__ __forceinline__ void load_slab(double (&r)[LOADS], const double* __re
 #pragma unroll
     for (int i = 0; i < LOADS; ++i) {
         int o, kk;
-        if (OUTER_CONTIG) { o = t % BM; kk = t / BM + i * (NT / BM); }
+        if (OUTER_CONTIG) { o = t % O; kk = t / O + i * (NT / O); }
         else { kk = t % BK; o = t / BK + i * (NT / BK); }
         i64 go = o0 + o, gk = k0 + kk;
         r[i] = (go < outer && gk < K) ? (OUTER_CONTIG ? X[go + gk * ldx] : X[gk + go * ldx]) : 0.0;
     }
 }
 
-template <bool OUTER_CONTIG, int LDS>
-__device__ __forceinline__ void store_slab(double* __restrict__ S, const double (&r)[LOADS], int t) {
+template <bool OUTER_CONTIG, int O>
+__device__ __forceinline__ void store_slab(double* __restrict__ S, const double (&r)[O * BK / NT], int t) {
+    constexpr int LOADS = O * BK / NT, LDS = O + PAD;
 #pragma unroll
     for (int i = 0; i < LOADS; ++i) {
         int o, kk;
-        if (OUTER_CONTIG) { o = t % BM; kk = t / BM + i * (NT / BM); }
+        if (OUTER_CONTIG) { o = t % O; kk = t / O + i * (NT / O); }
         else { kk = t % BK; o = t / BK + i * (NT / BK); }
         S[kk * LDS + o] = r[i];
     }
@@ -86,11 +86,13 @@ __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int& 
     tn = in / gm;
 }
 
-template <bool TA, bool TB, bool BETA0>
+template <int BM, int BN, bool TA, bool TB, bool BETA0>
 __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, double alpha,
                                                         const double* __restrict__ A, i64 lda,
                                                         const double* __restrict__ B, i64 ldb, double beta,
                                                         double* __restrict__ C, i64 ldc, int tiles_m, int tiles_n) {
+    constexpr int LDS_A = BM + PAD, LDS_B = BN + PAD;
+    constexpr int QM = BM / 16, QN = BN / 16;       // 4x4 blocks per lane group, down and across
     __shared__ double As[2][BK * LDS_A];
     __shared__ double Bs[2][BK * LDS_B];
 
@@ -107,71 +109,80 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
     // v_mfma_f64_4x4x4_4b runs four independent 4x4x4 products, one per lane
     // group g = (lane % 16) / 4 (measured on gfx950): A_g[i][k] at lane
     // 16k + 4g + i, B_g[k][j] at lane 16k + 4g + j, D_g[i][j] at lane
-    // 16i + 4g + j.  Each lane group owns a 32x32 sub-tile of the wave's
-    // 64x64 tile: rows 8q + 4(g&1) + i and columns 8p + 4(g>>1) + j
-    // (q, p = 0..7), interleaved by 4 so that the groups' LDS reads of one k
-    // row fall on consecutive addresses.
-    double acc[8][8];
+    // 16i + 4g + j.  The four waves split the tile 2 x 2; each lane group
+    // owns a (BM/4) x (BN/4) share of its wave's (BM/2) x (BN/2) tile: rows
+    // 8q + 4(g&1) + i and columns 8p + 4(g>>1) + j, interleaved by 4 so that
+    // the groups' LDS reads of one k row fall on consecutive addresses.
+    double acc[QM][QN];
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < QM; ++i)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) acc[i][j] = 0.0;
+        for (int j = 0; j < QN; ++j) acc[i][j] = 0.0;
 
-    double ra[LOADS], rb[LOADS];
+    double ra[BM * BK / NT], rb[BN * BK / NT];
     const int nslab = int((K + BK - 1) / BK);
-    load_slab<A_OC>(ra, A, lda, m0, 0, M, K, t);
-    load_slab<B_OC>(rb, B, ldb, n0, 0, N, K, t);
-    store_slab<A_OC, LDS_A>(As[0], ra, t);
-    store_slab<B_OC, LDS_B>(Bs[0], rb, t);
+    load_slab<A_OC, BM>(ra, A, lda, m0, 0, M, K, t);
+    load_slab<B_OC, BN>(rb, B, ldb, n0, 0, N, K, t);
+    store_slab<A_OC, BM>(As[0], ra, t);
+    store_slab<B_OC, BN>(Bs[0], rb, t);
     __syncthreads();
 
     const int kq = lane >> 4, grp = (lane >> 2) & 3, li = lane & 3;
-    const int roff = wm * 64 + 4 * (grp & 1) + li;   // + 8q
-    const int coff = wn * 64 + 4 * (grp >> 1) + li;  // + 8p
+    const int roff = wm * (BM / 2) + 4 * (grp & 1) + li;   // + 8q
+    const int coff = wn * (BN / 2) + 4 * (grp >> 1) + li;  // + 8p
     for (int s = 0; s < nslab; ++s) {
         const int cur = s & 1;
         if (s + 1 < nslab) {
-            load_slab<A_OC>(ra, A, lda, m0, i64(s + 1) * BK, M, K, t);
-            load_slab<B_OC>(rb, B, ldb, n0, i64(s + 1) * BK, N, K, t);
+            load_slab<A_OC, BM>(ra, A, lda, m0, i64(s + 1) * BK, M, K, t);
+            load_slab<B_OC, BN>(rb, B, ldb, n0, i64(s + 1) * BK, N, K, t);
         }
         const double* as = As[cur];
         const double* bs = Bs[cur];
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
             const int kr = ks * 4 + kq;
-            double a[8], b[8];
+            double a[QM], b[QN];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) a[q] = as[kr * LDS_A + roff + 8 * q];
+            for (int q = 0; q < QM; ++q) a[q] = as[kr * LDS_A + roff + 8 * q];
 #pragma unroll
-            for (int p = 0; p < 8; ++p) b[p] = bs[kr * LDS_B + coff + 8 * p];
+            for (int p = 0; p < QN; ++p) b[p] = bs[kr * LDS_B + coff + 8 * p];
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
+            for (int q = 0; q < QM; ++q)
 #pragma unroll
-                for (int p = 0; p < 8; ++p)
+                for (int p = 0; p < QN; ++p)
                     acc[q][p] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[q], b[p], acc[q][p], 0, 0, 0);
         }
         if (s + 1 < nslab) {
-            store_slab<A_OC, LDS_A>(As[cur ^ 1], ra, t);
-            store_slab<B_OC, LDS_B>(Bs[cur ^ 1], rb, t);
+            store_slab<A_OC, BM>(As[cur ^ 1], ra, t);
+            store_slab<B_OC, BN>(Bs[cur ^ 1], rb, t);
         }
         __syncthreads();
     }
 
-    // D_g[i][j] at lane 16i + 4g + j: row 8q + 4(g&1) + i, column 8p + 4(g>>1) + j
-    const int erow = wm * 64 + 4 * (grp & 1) + kq;
-    const int ecol = wn * 64 + 4 * (grp >> 1) + li;
+    // D_g[i][j] at lane 16i + 4g + j: row 8q + 4(g&1) + i, column 8p + 4(g>>1) + j.
+    // All loads of C are issued before the first store, without branches (an
+    // element outside C reads the nearest element inside it and is not
+    // stored): a load behind a bounds branch is waited for before the next
+    // one is issued, one memory round trip per element.
+    const int erow = wm * (BM / 2) + 4 * (grp & 1) + kq;
+    const int ecol = wn * (BN / 2) + 4 * (grp >> 1) + li;
+    double cv[QM][QN];
+    if (!BETA0) {
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
+        for (int p = 0; p < QN; ++p) {
+            const i64 col = min(n0 + ecol + 8 * p, N - 1);
+#pragma unroll
+            for (int q = 0; q < QM; ++q) cv[q][p] = C[col * ldc + min(m0 + erow + 8 * q, M - 1)];
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < QN; ++p) {
         const i64 col = n0 + ecol + 8 * p;
-        if (col >= N) continue;
-        double* cc = C + col * ldc;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < QM; ++q) {
             const i64 row = m0 + erow + 8 * q;
-            if (row < M) {
-                const double v = alpha * acc[q][p];
-                cc[row] = BETA0 ? v : __builtin_fma(beta, cc[row], v);
-            }
+            const double v = alpha * acc[q][p];
+            if (row < M && col < N) C[col * ldc + row] = BETA0 ? v : __builtin_fma(beta, cv[q][p], v);
         }
     }
 }
@@ -185,17 +196,52 @@ __global__ void dscale_kernel(i64 m, i64 n, double beta, double* __restrict__ C,
     }
 }
 
-template <bool TA, bool TB>
-void launch_variant(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda, const double* B,
-                    i64 ldb, double beta, double* C, i64 ldc) {
+template <int BM, int BN, bool TA, bool TB>
+void launch_tiles(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda, const double* B,
+                  i64 ldb, double beta, double* C, i64 ldc) {
     const int tiles_m = int(cdiv(m, BM)), tiles_n = int(cdiv(n, BN));
     dim3 grid(tiles_m * tiles_n), block(NT);
     if (beta == 0.0)
-        hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, true>), grid, block, 0, s, m, n, k, alpha, A, lda, B, ldb, beta,
-                           C, ldc, tiles_m, tiles_n);
+        hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, TA, TB, true>), grid, block, 0, s, m, n, k, alpha, A, lda, B,
+                           ldb, beta, C, ldc, tiles_m, tiles_n);
     else
-        hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, false>), grid, block, 0, s, m, n, k, alpha, A, lda, B, ldb,
-                           beta, C, ldc, tiles_m, tiles_n);
+        hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, TA, TB, false>), grid, block, 0, s, m, n, k, alpha, A, lda, B,
+                           ldb, beta, C, ldc, tiles_m, tiles_n);
+}
+
+// Tile shape of C per workgroup (measured on MI355X over the LU's products
+// and square ones, K = 32 .. 8192).  64 x 64 tiles (66 KB of LDS, two
+// workgroups per CU) beat 128 x 128 (one per CU) at every size, 8192^3
+// included; 32 x 32 wins where the 64 x 64 grid has fewer than about 1.5
+// workgroups per CU (the LU's look-ahead block, its panel updates and the
+// TRSM updates: products of 32..256 rows).  A side no longer than 32 takes a
+// 32-wide tile.  OXIBLAS_DGEMM_TILE=BMxBN (128x128, 64x64, 32x32) fixes the
+// shape (tuning).
+constexpr int NCU = 256;
+void choose_tile(i64 m, i64 n, int& bm, int& bn) {
+    static int forced = -1;
+    if (forced < 0) {
+        forced = 0;
+        if (const char* e = getenv("OXIBLAS_DGEMM_TILE")) {
+            int a = 0, b = 0;
+            if (sscanf(e, "%dx%d", &a, &b) == 2 && a == b && (a == 32 || a == 64 || a == 128)) forced = a;
+        }
+    }
+    if (forced) { bm = bn = forced; return; }
+    bm = m <= 32 ? 32 : 64;
+    bn = n <= 32 ? 32 : 64;
+    if (cdiv(m, bm) * cdiv(n, bn) < i64(NCU) * 3 / 2) bm = bn = 32;
+}
+
+template <bool TA, bool TB>
+void launch_variant(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda, const double* B,
+                    i64 ldb, double beta, double* C, i64 ldc) {
+    int bm, bn;
+    choose_tile(m, n, bm, bn);
+#define OXI_TILE(X, Y)                                                                                              \
+    if (bm == X && bn == Y) return launch_tiles<X, Y, TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    OXI_TILE(64, 64) OXI_TILE(32, 32) OXI_TILE(64, 32) OXI_TILE(32, 64) OXI_TILE(128, 128)
+#undef OXI_TILE
 }
 
 }  // namespace
@@ -212,7 +258,7 @@ int oxi_launch_dgemm(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n
                      const double* A, i64 lda, const double* B, i64 ldb, double beta, double* C, i64 ldc) {
     if (m <= 0 || n <= 0) return OXI_OK;
     if (alpha == 0.0 || k <= 0) return oxi_launch_dscale(s, m, n, beta, C, ldc);
-    if (cdiv(m, BM) * cdiv(n, BN) > (i64(1) << 31) - 1)
+    if (cdiv(m, 128) * cdiv(n, 128) > (i64(1) << 31) - 1)
         return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "dgemm: %lld x %lld has too many tiles", (long long)m,
                              (long long)n);
     if (!ta && !tb) launch_variant<false, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);

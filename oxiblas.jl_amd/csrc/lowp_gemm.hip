@@ -158,22 +158,34 @@ __global__ __launch_bounds__(NT) void lowp_update_kernel(i64 M, i64 N, i64 Kp, f
         }
         __syncthreads();
     }
-    // epilogue: C += alpha * sa(row block) * sb(col block) * acc
+    // epilogue: C += alpha * sa(row block) * sb(col block) * acc.  All loads
+    // (C and the scales) are issued before the first store, without branches
+    // (an element outside C reads the nearest element inside it and is not
+    // stored): a load behind a bounds branch is waited for before the next
+    // one is issued, one memory round trip per element.
+    float cv[4][4][4], rsv[4][4], csv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const i64 col = min(n0 + wn * 64 + j * 16 + li, N - 1);
+        csv[j] = alpha * (sb ? sb[col >> 8] : 1.0f);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) cv[i][j][r] = C[col * ldc + min(m0 + wm * 64 + i * 16 + lg * 4 + r, M - 1)];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) rsv[i][r] = sa ? sa[min(m0 + wm * 64 + i * 16 + lg * 4 + r, M - 1) >> 8] : 1.0f;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const i64 col = n0 + wn * 64 + j * 16 + li;
-        if (col >= N) continue;
-        const float cs = alpha * (sb ? sb[col >> 8] : 1.0f);
-        float* cc = C + col * ldc;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const i64 row = m0 + wm * 64 + i * 16 + lg * 4 + r;
-                if (row < M) {
-                    const float rs = sa ? sa[row >> 8] : 1.0f;
-                    cc[row] = __builtin_fmaf(cs * rs, acc[i][j][r], cc[row]);
-                }
+                if (row < M && col < N) C[col * ldc + row] = __builtin_fmaf(csv[j] * rsv[i][r], acc[i][j][r], cv[i][j][r]);
             }
     }
 }

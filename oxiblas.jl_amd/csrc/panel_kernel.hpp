@@ -511,21 +511,26 @@ __global__ __launch_bounds__(64) void leaf_swaps_kernel(T* Wm, i64 ldw, i64 c0, 
     else { x = c0 + w + (fb - tl) * 64 + lane; ok = x < x_hi; }
     i64 pos; int src;
     leaf_slots(ipiv, c0, w, 0, lane, pos, src);
-    // slot s moves when its source differs; at most 2w <= 64 slots
+    // slot s moves when its source differs (at most 2w <= 64 slots).  All
+    // loads are issued before any store and without branches: a slot that
+    // does not move reads its own position, a lane outside the columns
+    // reads column c0 (both in bounds), and only moving slots are stored.
+    const i64 psrc = (i64)__shfl((long long)pos, src);
+    const bool moves = pos >= 0 && src != lane;
+    const u64 mv = __ballot(moves);
+    const i64 xr = ok ? x : c0;
     T v[64];
 #pragma unroll
     for (int s = 0; s < 64; ++s) {
-        const i64 ps = (i64)readlane64(u64(pos), s);
-        const int ss = __builtin_amdgcn_readlane(src, s);
-        v[s] = T(0);
-        if (ps >= 0 && ss != s && ok) v[s] = Wm[(c0 + (i64)readlane64(u64(pos), ss)) * ldw + x];
+        const i64 from = (i64)readlane64(u64((mv >> s) & 1 ? psrc : 0), s);
+        v[s] = Wm[(c0 + from) * ldw + xr];
     }
 #pragma unroll
-    for (int s = 0; s < 64; ++s) {
-        const i64 ps = (i64)readlane64(u64(pos), s);
-        const int ss = __builtin_amdgcn_readlane(src, s);
-        if (ps >= 0 && ss != s && ok) Wm[(c0 + ps) * ldw + x] = v[s];
-    }
+    for (int s = 0; s < 64; ++s)
+        if ((mv >> s) & 1) {
+            const i64 to = (i64)readlane64(u64(pos), s);
+            if (ok) Wm[(c0 + to) * ldw + x] = v[s];
+        }
 }
 
 }  // namespace

@@ -110,20 +110,32 @@ __global__ __launch_bounds__(NT) void sgemm_mfma_kernel(i64 M, i64 N, i64 K, flo
         }
         __syncthreads();
     }
+    // All loads of C are issued before the first store, without branches (an
+    // element outside C reads the nearest element inside it and is not
+    // stored): a load behind a bounds branch is waited for before the next
+    // one is issued, one memory round trip per element.
+    float cv[4][4][4];
+    if (!BETA0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const i64 col = min(n0 + wn * 64 + j * 16 + lcol, N - 1);
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    cv[i][j][r] = C[col * ldc + min(m0 + wm * 64 + i * 16 + lrow * 4 + r, M - 1)];
+            }
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const i64 col = n0 + wn * 64 + j * 16 + lcol;
-            if (col >= N) continue;
-            float* cc = C + col * ldc;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const i64 row = m0 + wm * 64 + i * 16 + lrow * 4 + r;
-                if (row < M) {
-                    const float v = alpha * acc[i][j][r];
-                    cc[row] = BETA0 ? v : __builtin_fmaf(beta, cc[row], v);
-                }
+                const float v = alpha * acc[i][j][r];
+                if (row < M && col < N) C[col * ldc + row] = BETA0 ? v : __builtin_fmaf(beta, cv[i][j][r], v);
             }
         }
 }
