@@ -7,17 +7,20 @@
 // coalesced global loads, double-buffered so that the next slab is in flight
 // while the matrix cores consume the current one.
 //
-// Work decomposition: a 256-thread workgroup owns a BM x BN tile of C (64 x 64
-// for most shapes, 32 x 32 where the grid would not fill the CUs); its four
-// waves split it 2 x 2 and compute with v_mfma_f64_4x4x4_4b (the FP64 matrix
-// instruction that reaches the full rate on gfx950).  The epilogue applies alpha and beta as the
+// Work decomposition: a 256-thread workgroup owns a tile of C, its four
+// waves split it 2 x 2.  Large products take 128 x 128 tiles on
+// v_mfma_f64_16x16x4 (dgemm_m16_kernel); products whose 128 x 128 grid would
+// leave CUs idle take 64 x 64 or 32 x 32 tiles on v_mfma_f64_4x4x4_4b
+// (dgemm_mfma_kernel).  Both instructions reach the full FP64 rate on gfx950.  The epilogue applies alpha and beta as the
 // reference's full tiles do: v = alpha*acc; C = beta == 0 ? v : fma(beta, C, v)
 // (src/blas/simd.jl:190-196); beta == 0 never reads C.
 #include "oxi_internal.hpp"
 
+#include <cstring>
+
 namespace {
 
-constexpr int BK = 32, NT = 256;
+constexpr int BK = 32, NT = 256;            // slab depth of the 4x4x4 kernel; threads per workgroup
 constexpr int PAD = 1;                 // one double of padding per LDS row
 
 // Load one BK-deep slab of an operand tile (O rows of op(X)) into registers.
@@ -25,7 +28,7 @@ constexpr int PAD = 1;                 // one double of padding per LDS row
 //                 op(B) for B 'T'); otherwise at X[(k0+kk) + (o0+o)*ldx].
 // Out-of-range elements are zero so partial tiles need no special casing in
 // the MFMA loop.
-template <bool OUTER_CONTIG, int O>
+template <bool OUTER_CONTIG, int O, int BK = 32>
 __device__ __forceinline__ void load_slab(double (&r)[O * BK / NT], const double* __restrict__ X, i64 ldx, i64 o0,
                                           i64 k0, i64 outer, i64 K, int t) {
     constexpr int LOADS = O * BK / NT;
@@ -51,7 +54,7 @@ __device__ __forceinline__ void load_slab(double (&r)[O * BK / NT], const double
     }
 }
 
-template <bool OUTER_CONTIG, int O>
+template <bool OUTER_CONTIG, int O, int BK = 32>
 __device__ __forceinline__ void store_slab(double* __restrict__ S, const double (&r)[O * BK / NT], int t) {
     constexpr int LOADS = O * BK / NT, LDS = O + PAD;
 #pragma unroll
@@ -123,11 +126,31 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
     const int nslab = int((K + BK - 1) / BK);
     load_slab<A_OC, BM>(ra, A, lda, m0, 0, M, K, t);
     load_slab<B_OC, BN>(rb, B, ldb, n0, 0, N, K, t);
+
+    // D_g[i][j] at lane 16i + 4g + j: row 8q + 4(g&1) + i, column 8p + 4(g>>1) + j.
+    // C is loaded before the K loop (tiles up to 64 x 64, whose share of C
+    // is 16 registers per lane), so that the loads travel while the matrix
+    // cores run; all loads are issued without branches (an element outside C
+    // reads the nearest element inside it and is not stored): a load behind
+    // a bounds branch is waited for before the next one is issued.
+    const int kq = lane >> 4, grp = (lane >> 2) & 3, li = lane & 3;
+    const int erow = wm * (BM / 2) + 4 * (grp & 1) + kq;
+    const int ecol = wn * (BN / 2) + 4 * (grp >> 1) + li;
+    constexpr bool C_EARLY = !BETA0 && QM * QN <= 16;
+    double cv[QM][QN];
+    auto load_c = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int p = 0; p < QN; ++p) {
+            const i64 col = min(n0 + ecol + 8 * p, N - 1);
+#pragma unroll
+            for (int q = 0; q < QM; ++q) cv[q][p] = C[col * ldc + min(m0 + erow + 8 * q, M - 1)];
+        }
+    };
+    if (C_EARLY) load_c();
     store_slab<A_OC, BM>(As[0], ra, t);
     store_slab<B_OC, BN>(Bs[0], rb, t);
     __syncthreads();
 
-    const int kq = lane >> 4, grp = (lane >> 2) & 3, li = lane & 3;
     const int roff = wm * (BM / 2) + 4 * (grp & 1) + li;   // + 8q
     const int coff = wn * (BN / 2) + 4 * (grp >> 1) + li;  // + 8p
     for (int s = 0; s < nslab; ++s) {
@@ -159,22 +182,7 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
         __syncthreads();
     }
 
-    // D_g[i][j] at lane 16i + 4g + j: row 8q + 4(g&1) + i, column 8p + 4(g>>1) + j.
-    // All loads of C are issued before the first store, without branches (an
-    // element outside C reads the nearest element inside it and is not
-    // stored): a load behind a bounds branch is waited for before the next
-    // one is issued, one memory round trip per element.
-    const int erow = wm * (BM / 2) + 4 * (grp & 1) + kq;
-    const int ecol = wn * (BN / 2) + 4 * (grp >> 1) + li;
-    double cv[QM][QN];
-    if (!BETA0) {
-#pragma unroll
-        for (int p = 0; p < QN; ++p) {
-            const i64 col = min(n0 + ecol + 8 * p, N - 1);
-#pragma unroll
-            for (int q = 0; q < QM; ++q) cv[q][p] = C[col * ldc + min(m0 + erow + 8 * q, M - 1)];
-        }
-    }
+    if (!BETA0 && !C_EARLY) load_c();
 #pragma unroll
     for (int p = 0; p < QN; ++p) {
         const i64 col = n0 + ecol + 8 * p;
@@ -184,6 +192,100 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
             const double v = alpha * acc[q][p];
             if (row < M && col < N) C[col * ldc + row] = BETA0 ? v : __builtin_fma(beta, cv[q][p], v);
         }
+    }
+}
+
+// 128 x 128 tiles on v_mfma_f64_16x16x4: each wave owns 64 x 64 of C as 4 x 4
+// blocks of 16 x 16 (four accumulators per lane each).  One instruction does
+// four times the work of a 4x4x4_4b one for the same two LDS reads, and the
+// 16-deep slabs (66 KB of LDS, 2 x 2 slabs in flight per CU with two
+// workgroups) give each barrier twice the matrix work of a 64 x 64 tile.
+// Lane layout (measured on gfx950): A[i][k] at lane 16k + i, B[k][j] at lane
+// 16k + j, D[i][j] in accumulator i / 4 of lane 16(i % 4) + j.
+typedef double d4v __attribute__((ext_vector_type(4)));
+constexpr int BK16 = 16;
+
+template <bool TA, bool TB, bool BETA0>
+__global__ __launch_bounds__(NT, 2) void dgemm_m16_kernel(i64 M, i64 N, i64 K, double alpha,
+                                                          const double* __restrict__ A, i64 lda,
+                                                          const double* __restrict__ B, i64 ldb, double beta,
+                                                          double* __restrict__ C, i64 ldc, int tiles_m, int tiles_n) {
+    constexpr int BM = 128, BN = 128, LDS_A = BM + PAD, LDS_B = BN + PAD;
+    __shared__ double As[2][BK16 * LDS_A];
+    __shared__ double Bs[2][BK16 * LDS_B];
+
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    int tm, tn;
+    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const i64 m0 = i64(tm) * BM, n0 = i64(tn) * BN;
+    constexpr bool A_OC = !TA, B_OC = TB;
+
+    d4v acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = d4v{0.0, 0.0, 0.0, 0.0};
+
+    double ra[BM * BK16 / NT], rb[BN * BK16 / NT];
+    const int nslab = int((K + BK16 - 1) / BK16);
+    load_slab<A_OC, BM, BK16>(ra, A, lda, m0, 0, M, K, t);
+    load_slab<B_OC, BN, BK16>(rb, B, ldb, n0, 0, N, K, t);
+    store_slab<A_OC, BM, BK16>(As[0], ra, t);
+    store_slab<B_OC, BN, BK16>(Bs[0], rb, t);
+    __syncthreads();
+
+    const int lk = lane >> 4, li = lane & 15;
+    for (int s = 0; s < nslab; ++s) {
+        const int cur = s & 1;
+        if (s + 1 < nslab) {
+            load_slab<A_OC, BM, BK16>(ra, A, lda, m0, i64(s + 1) * BK16, M, K, t);
+            load_slab<B_OC, BN, BK16>(rb, B, ldb, n0, i64(s + 1) * BK16, N, K, t);
+        }
+        const double* as = As[cur];
+        const double* bs = Bs[cur];
+#pragma unroll
+        for (int ks = 0; ks < BK16 / 4; ++ks) {
+            const int kr = ks * 4 + lk;
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = as[kr * LDS_A + wm * 64 + i * 16 + li];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = bs[kr * LDS_B + wn * 64 + j * 16 + li];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (s + 1 < nslab) {
+            store_slab<A_OC, BM, BK16>(As[cur ^ 1], ra, t);
+            store_slab<B_OC, BN, BK16>(Bs[cur ^ 1], rb, t);
+        }
+        __syncthreads();
+    }
+
+    // one block column at a time: its C loads (issued without branches, an
+    // element outside C reading the nearest one inside) before its stores
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const i64 col = n0 + wn * 64 + j * 16 + li;
+        const i64 colc = min(col, N - 1);
+        double cv[4][4];
+        if (!BETA0) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) cv[i][r] = C[colc * ldc + min(m0 + wm * 64 + i * 16 + lk + 4 * r, M - 1)];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const i64 row = m0 + wm * 64 + i * 16 + lk + 4 * r;
+                const double v = alpha * acc[i][j][r];
+                if (row < M && col < N) C[col * ldc + row] = BETA0 ? v : __builtin_fma(beta, cv[i][r], v);
+            }
     }
 }
 
@@ -209,14 +311,28 @@ void launch_tiles(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double
                            ldb, beta, C, ldc, tiles_m, tiles_n);
 }
 
+template <bool TA, bool TB>
+void launch_m16(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
+                double beta, double* C, i64 ldc) {
+    const int tiles_m = int(cdiv(m, 128)), tiles_n = int(cdiv(n, 128));
+    dim3 grid(tiles_m * tiles_n), block(NT);
+    if (beta == 0.0)
+        hipLaunchKernelGGL((dgemm_m16_kernel<TA, TB, true>), grid, block, 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                           ldc, tiles_m, tiles_n);
+    else
+        hipLaunchKernelGGL((dgemm_m16_kernel<TA, TB, false>), grid, block, 0, s, m, n, k, alpha, A, lda, B, ldb, beta,
+                           C, ldc, tiles_m, tiles_n);
+}
+
 // Tile shape of C per workgroup (measured on MI355X over the LU's products
-// and square ones, K = 32 .. 8192).  64 x 64 tiles (66 KB of LDS, two
-// workgroups per CU) beat 128 x 128 (one per CU) at every size, 8192^3
-// included; 32 x 32 wins where the 64 x 64 grid has fewer than about 1.5
-// workgroups per CU (the LU's look-ahead block, its panel updates and the
-// TRSM updates: products of 32..256 rows).  A side no longer than 32 takes a
-// 32-wide tile.  OXIBLAS_DGEMM_TILE=BMxBN (128x128, 64x64, 32x32) fixes the
-// shape (tuning).
+// and square ones, K = 32 .. 8192): 128 x 128 on the 16x16x4 instruction
+// where its grid has at least two workgroups per CU (8192^3: 62 TF/s against
+// 54 for 64 x 64); else 64 x 64 tiles on the 4x4x4_4b instruction (66 KB of
+// LDS, two workgroups per CU); 32 x 32 where the 64 x 64 grid has fewer than
+// about 1.5 workgroups per CU (the LU's look-ahead block, its panel updates
+// and the TRSM updates: products of 32..256 rows).  A side no longer than 32
+// takes a 32-wide tile.  OXIBLAS_DGEMM_TILE (m16, 64x64, 32x32, 128x128)
+// fixes the choice (tuning); bm = bn = 16 stands for the 16x16x4 kernel.
 constexpr int NCU = 256;
 void choose_tile(i64 m, i64 n, int& bm, int& bn) {
     static int forced = -1;
@@ -225,9 +341,11 @@ void choose_tile(i64 m, i64 n, int& bm, int& bn) {
         if (const char* e = getenv("OXIBLAS_DGEMM_TILE")) {
             int a = 0, b = 0;
             if (sscanf(e, "%dx%d", &a, &b) == 2 && a == b && (a == 32 || a == 64 || a == 128)) forced = a;
+            if (!strcmp(e, "m16")) forced = 16;
         }
     }
     if (forced) { bm = bn = forced; return; }
+    if (cdiv(m, 128) * cdiv(n, 128) >= 2 * NCU) { bm = bn = 16; return; }
     bm = m <= 32 ? 32 : 64;
     bn = n <= 32 ? 32 : 64;
     if (cdiv(m, bm) * cdiv(n, bn) < i64(NCU) * 3 / 2) bm = bn = 32;
@@ -238,6 +356,7 @@ void launch_variant(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const doub
                     i64 ldb, double beta, double* C, i64 ldc) {
     int bm, bn;
     choose_tile(m, n, bm, bn);
+    if (bm == 16) return launch_m16<TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 #define OXI_TILE(X, Y)                                                                                              \
     if (bm == X && bn == Y) return launch_tiles<X, Y, TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     OXI_TILE(64, 64) OXI_TILE(32, 32) OXI_TILE(64, 32) OXI_TILE(32, 64) OXI_TILE(128, 128)

@@ -474,6 +474,17 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
                            lda, g.Wm, ldw, m, n);
     OXI_TRY(launch_check(h, "transpose"));
 
+    // The factorization's main sequence runs on the bulk stream (kept off a
+    // few CUs that the panel on the side stream then always finds free) and
+    // returns to the caller's stream before the transpose back.
+    const hipStream_t s_caller = s;
+    if (h->bulk) {
+        OXI_HIP(h, hipEventRecord(h->ev_c, s));
+        OXI_HIP(h, hipStreamWaitEvent(h->bulk, h->ev_c, 0));
+        s = h->bulk;
+        g.s = s;
+    }
+
     // Right-looking blocked LU with one panel of look-ahead: after panel j's
     // interchanges reach the columns to its right, the next panel's columns
     // are updated first and panel j+1 is factored on the side stream while
@@ -526,6 +537,12 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     if (panel_on_side) OXI_HIP(h, hipStreamWaitEvent(s, h->ev_b, 0));
     // deferred interchanges on the columns left of each panel
     for (i64 j0 = nb; j0 < q; j0 += nb) OXI_TRY(swap_w(g, j0, std::min<i64>(j0 + nb, q) - 1, 0, j0, 0));
+
+    if (s != s_caller) {
+        OXI_HIP(h, hipEventRecord(h->ev_c, s));
+        OXI_HIP(h, hipStreamWaitEvent(s_caller, h->ev_c, 0));
+        s = s_caller;
+    }
 
     // W -> A
     if (square)

@@ -1,6 +1,9 @@
 // Handle, stream, workspace and memory entry points of liboxiblas_mi355x.so.
 #include "oxi_internal.hpp"
 
+#include <hip/hip_ext.h>
+#include <cstdlib>
+
 #include <cstdarg>
 #include <new>
 
@@ -21,6 +24,7 @@ int oxi_workspace(oxi_handle_t h, size_t bytes, void** out) {
     if (h->ws) {
         OXI_HIP(h, hipStreamSynchronize(h->stream));
         OXI_HIP(h, hipStreamSynchronize(h->side));
+        if (h->bulk) OXI_HIP(h, hipStreamSynchronize(h->bulk));
         OXI_HIP(h, hipFree(h->ws));
         h->ws = nullptr; h->ws_bytes = 0;
     }
@@ -39,6 +43,7 @@ int oxi_workspace2(oxi_handle_t h, size_t bytes, void** out) {
     if (h->ws2) {
         OXI_HIP(h, hipStreamSynchronize(h->stream));
         OXI_HIP(h, hipStreamSynchronize(h->side));
+        if (h->bulk) OXI_HIP(h, hipStreamSynchronize(h->bulk));
         OXI_HIP(h, hipFree(h->ws2));
         h->ws2 = nullptr; h->ws2_bytes = 0;
     }
@@ -76,6 +81,22 @@ int oxi_create(int device, oxi_handle_t* handle) {
         oxi_destroy(h);
         return OXI_ERR_HIP;
     }
+    // The bulk stream's CU mask: bit i of the mask selects a CU of XCD i % 8
+    // (measured on MI355X), so clearing the top `panel_cus` bits frees
+    // panel_cus / 8 CUs on every XCD.  OXIBLAS_PANEL_CUS overrides the count
+    // (a multiple of 8; 0 runs the trailing updates on the caller's stream).
+    h->panel_cus = 32;
+    if (const char* e = getenv("OXIBLAS_PANEL_CUS")) h->panel_cus = atoi(e);
+    if (h->panel_cus < 0 || h->panel_cus % 8 || h->panel_cus > h->num_cus / 2) h->panel_cus = 0;
+    if (h->panel_cus > 0) {
+        std::vector<uint32_t> mask(size_t(h->num_cus + 31) / 32, 0u);
+        for (int i = 0; i < h->num_cus - h->panel_cus; ++i) mask[i / 32] |= 1u << (i % 32);
+        if (hipExtStreamCreateWithCUMask(&h->bulk, uint32_t(mask.size()), mask.data()) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_c, hipEventDisableTiming) != hipSuccess) {
+            oxi_destroy(h);
+            return OXI_ERR_HIP;
+        }
+    }
     (void)hipMemset(h->dev_flags, 0, 64 * sizeof(int));
     (void)hipMemset(h->dev_info, 0, 8 * sizeof(i64));
     (void)hipDeviceSynchronize();
@@ -88,6 +109,7 @@ int oxi_destroy(oxi_handle_t h) {
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->side) (void)hipStreamSynchronize(h->side);
+    if (h->bulk) (void)hipStreamSynchronize(h->bulk);
     (void)hipDeviceSynchronize();
     if (h->ws) (void)hipFree(h->ws);
     if (h->ws2) (void)hipFree(h->ws2);
@@ -95,7 +117,9 @@ int oxi_destroy(oxi_handle_t h) {
     if (h->dev_info) (void)hipFree(h->dev_info);
     if (h->ev_a) (void)hipEventDestroy(h->ev_a);
     if (h->ev_b) (void)hipEventDestroy(h->ev_b);
+    if (h->ev_c) (void)hipEventDestroy(h->ev_c);
     if (h->side) (void)hipStreamDestroy(h->side);
+    if (h->bulk) (void)hipStreamDestroy(h->bulk);
     delete h;
     return OXI_OK;
 }
@@ -113,6 +137,7 @@ int oxi_sync(oxi_handle_t h) {
     (void)hipSetDevice(h->device);
     OXI_HIP(h, hipStreamSynchronize(h->stream));
     OXI_HIP(h, hipStreamSynchronize(h->side));
+    if (h->bulk) OXI_HIP(h, hipStreamSynchronize(h->bulk));
     return OXI_OK;
 }
 

@@ -21,7 +21,12 @@ struct oxi_handle_s {
     int device = 0;
     hipStream_t stream = nullptr;       // caller's stream (null = default stream)
     hipStream_t side = nullptr;         // internal stream for look-ahead work
-    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    // internal stream for getrf's trailing updates, kept off `panel_cus` CUs
+    // (spread evenly over the XCDs) so that the panel running beside them
+    // finds free CUs; null when no CU is kept free
+    hipStream_t bulk = nullptr;
+    int panel_cus = 0;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr;
     void* ws = nullptr;                 // device workspace (kernels' scratch)
     size_t ws_bytes = 0;
     void* ws2 = nullptr;                // second workspace for drivers that call the others
