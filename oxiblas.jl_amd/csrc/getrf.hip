@@ -422,8 +422,16 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     (void)hipSetDevice(h->device);
     hipStream_t s = h->stream;
     const i64 q = std::min(m, n);
+    const bool tuned = nb == 0;
     if (nb == 0) nb = choose_nb(m, n);
     if (nb > PERM_MAX_STEPS) nb = PERM_MAX_STEPS;
+    // Tuned block size on large matrices: 512-wide panels while more than 3/8
+    // of the columns remain (the trailing update bounds those steps and runs
+    // faster with K = 512), 256 after (the panel bounds them).  Measured on
+    // MI355X at n = 16384: 256 throughout 136 ms, 512 throughout 134 ms.
+    const i64 nb_wide = (tuned && q >= 12288) ? std::min<i64>(2 * nb, PERM_MAX_STEPS) : nb;
+    const i64 wide_until = q - (3 * q) / 8;
+    auto nb_at = [&](i64 j) { return j < wide_until ? nb_wide : nb; };
 
     // workspace: [W if rectangular][pub][pairs main][pairs side][low-precision operands]
     const bool square = (m == n);
@@ -435,7 +443,7 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     const size_t off_pairs = off_pub + ((pub_doubles * 8 + 255) & ~size_t(255));
     const size_t off_pairs2 = off_pairs + pairs_bytes;
     const size_t off_lowp = off_pairs2 + pairs_bytes;
-    const size_t lowp_bytes = (sizeof(T) == 4 && prec >= 0) ? oxi_lowp_update_ws(prec, n, m, nb) : 0;
+    const size_t lowp_bytes = (sizeof(T) == 4 && prec >= 0) ? oxi_lowp_update_ws(prec, n, m, nb_wide) : 0;
     const size_t total = off_lowp + lowp_bytes;
     void* ws;
     OXI_TRY(oxi_workspace(h, total, &ws));
@@ -497,15 +505,19 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     gs.s = h->side;
     gs.pairs = (int*)(base + off_pairs2);
     bool panel_on_side = false;
-    OXI_TRY(panel_top(g, 0, std::min<i64>(nb, q)));
-    for (i64 j0 = 0; j0 < q; j0 += nb) {
-        const i64 jb = std::min<i64>(nb, q - j0);
+    OXI_TRY(panel_top(g, 0, std::min<i64>(nb_at(0), q)));
+    for (i64 j0 = 0, jn = 0; j0 < q; j0 = jn) {
+        const i64 jb = std::min<i64>(nb_at(j0), q - j0);
         const i64 j1 = j0 + jb;
+        jn = j1;
         if (panel_on_side) OXI_HIP(h, hipStreamWaitEvent(s, h->ev_b, 0));
         OXI_TRY(swap_w(g, j0, j1 - 1, j1, n, 0));
-        if (j1 >= n) continue;
+        if (j1 >= n) {
+            if (j0 > 0) OXI_TRY(swap_w(g, j0, j1 - 1, 0, j0, 0));
+            continue;
+        }
         const bool next = j1 < q;                         // then j1 < m and j1 < n
-        const i64 j2 = next && lookahead ? std::min<i64>(j1 + nb, q) : j1;
+        const i64 j2 = next && lookahead ? std::min<i64>(j1 + nb_at(j1), q) : j1;
         // look-ahead block: the next panel's columns [j1, j2)
         if (j2 > j1) {
             OXI_TRY(trsm_ru(h, s, j2 - j1, jb, g.Wm + j0 + j0 * ldw, ldw, g.Wm + j1 + j0 * ldw, ldw));
@@ -529,14 +541,17 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
                 OXI_TRY(oxi_prof_end(h, s, pe, 2.0 * double(n - j2) * double(m - j1) * double(jb)));
             }
         }
+        // this panel's interchanges on the columns to its left: the last
+        // reader of those columns was the previous step's trailing update,
+        // which ran before this point on this stream (per column the
+        // interchanges still arrive in panel order)
+        if (j0 > 0) OXI_TRY(swap_w(g, j0, j1 - 1, 0, j0, 0));
         if (next && !lookahead) {
-            OXI_TRY(panel_top(g, j1, std::min<i64>(nb, q - j1)));
+            OXI_TRY(panel_top(g, j1, std::min<i64>(nb_at(j1), q - j1)));
             panel_on_side = false;
         }
     }
     if (panel_on_side) OXI_HIP(h, hipStreamWaitEvent(s, h->ev_b, 0));
-    // deferred interchanges on the columns left of each panel
-    for (i64 j0 = nb; j0 < q; j0 += nb) OXI_TRY(swap_w(g, j0, std::min<i64>(j0 + nb, q) - 1, 0, j0, 0));
 
     if (s != s_caller) {
         OXI_HIP(h, hipEventRecord(h->ev_c, s));
