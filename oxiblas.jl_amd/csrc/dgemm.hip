@@ -54,6 +54,24 @@ __device__ __forceinline__ void load_slab(double (&r)[O * BK / NT], const double
     }
 }
 
+// The same without bounds checks, for tiles that lie inside the operands
+// with K a multiple of the slab depth: no branch in the slab loop, whose
+// merge with the guarded path would otherwise leave a wait for all loads
+// between the two operands' loads.
+template <bool OUTER_CONTIG, int O, int BK>
+__device__ __forceinline__ void load_slab_inner(double (&r)[O * BK / NT], const double* __restrict__ X, i64 ldx,
+                                                i64 o0, i64 k0, int t) {
+    constexpr int LOADS = O * BK / NT;
+#pragma unroll
+    for (int i = 0; i < LOADS; ++i) {
+        int o, kk;
+        if (OUTER_CONTIG) { o = t % O; kk = t / O + i * (NT / O); }
+        else { kk = t % BK; o = t / BK + i * (NT / BK); }
+        const i64 go = o0 + o, gk = k0 + kk;
+        r[i] = OUTER_CONTIG ? X[go + gk * ldx] : X[gk + go * ldx];
+    }
+}
+
 template <bool OUTER_CONTIG, int O, int BK = 32>
 __device__ __forceinline__ void store_slab(double* __restrict__ S, const double (&r)[O * BK / NT], int t) {
     constexpr int LOADS = O * BK / NT, LDS = O + PAD;
@@ -205,22 +223,19 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
 typedef double d4v __attribute__((ext_vector_type(4)));
 constexpr int BK16 = 16;
 
-template <bool TA, bool TB, bool BETA0>
-__global__ __launch_bounds__(NT, 2) void dgemm_m16_kernel(i64 M, i64 N, i64 K, double alpha,
-                                                          const double* __restrict__ A, i64 lda,
-                                                          const double* __restrict__ B, i64 ldb, double beta,
-                                                          double* __restrict__ C, i64 ldc, int tiles_m, int tiles_n) {
+// One tile; EDGE: the tile crosses an edge of C or K is not a multiple of
+// the slab depth (bounds-checked loads and stores).
+template <bool EDGE, bool TA, bool TB, bool BETA0>
+__device__ __forceinline__ void m16_tile(double (*As)[BK16 * 129], double (*Bs)[BK16 * 129], i64 M, i64 N, i64 K,
+                                         double alpha, const double* __restrict__ A, i64 lda,
+                                         const double* __restrict__ B, i64 ldb, double beta, double* __restrict__ C,
+                                         i64 ldc, i64 m0, i64 n0) {
     constexpr int BM = 128, BN = 128, LDS_A = BM + PAD, LDS_B = BN + PAD;
-    __shared__ double As[2][BK16 * LDS_A];
-    __shared__ double Bs[2][BK16 * LDS_B];
-
+    static_assert(LDS_A == 129 && LDS_B == 129, "LDS row length");
+    constexpr bool A_OC = !TA, B_OC = TB;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    int tm, tn;
-    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
-    const i64 m0 = i64(tm) * BM, n0 = i64(tn) * BN;
-    constexpr bool A_OC = !TA, B_OC = TB;
 
     d4v acc[4][4];
 #pragma unroll
@@ -229,9 +244,17 @@ __global__ __launch_bounds__(NT, 2) void dgemm_m16_kernel(i64 M, i64 N, i64 K, d
         for (int j = 0; j < 4; ++j) acc[i][j] = d4v{0.0, 0.0, 0.0, 0.0};
 
     double ra[BM * BK16 / NT], rb[BN * BK16 / NT];
+    auto load = [&](i64 k0) __attribute__((always_inline)) {
+        if (EDGE) {
+            load_slab<A_OC, BM, BK16>(ra, A, lda, m0, k0, M, K, t);
+            load_slab<B_OC, BN, BK16>(rb, B, ldb, n0, k0, N, K, t);
+        } else {
+            load_slab_inner<A_OC, BM, BK16>(ra, A, lda, m0, k0, t);
+            load_slab_inner<B_OC, BN, BK16>(rb, B, ldb, n0, k0, t);
+        }
+    };
     const int nslab = int((K + BK16 - 1) / BK16);
-    load_slab<A_OC, BM, BK16>(ra, A, lda, m0, 0, M, K, t);
-    load_slab<B_OC, BN, BK16>(rb, B, ldb, n0, 0, N, K, t);
+    load(0);
     store_slab<A_OC, BM, BK16>(As[0], ra, t);
     store_slab<B_OC, BN, BK16>(Bs[0], rb, t);
     __syncthreads();
@@ -239,10 +262,7 @@ __global__ __launch_bounds__(NT, 2) void dgemm_m16_kernel(i64 M, i64 N, i64 K, d
     const int lk = lane >> 4, li = lane & 15;
     for (int s = 0; s < nslab; ++s) {
         const int cur = s & 1;
-        if (s + 1 < nslab) {
-            load_slab<A_OC, BM, BK16>(ra, A, lda, m0, i64(s + 1) * BK16, M, K, t);
-            load_slab<B_OC, BN, BK16>(rb, B, ldb, n0, i64(s + 1) * BK16, N, K, t);
-        }
+        if (s + 1 < nslab) load(i64(s + 1) * BK16);
         const double* as = As[cur];
         const double* bs = Bs[cur];
 #pragma unroll
@@ -265,18 +285,22 @@ __global__ __launch_bounds__(NT, 2) void dgemm_m16_kernel(i64 M, i64 N, i64 K, d
         __syncthreads();
     }
 
-    // one block column at a time: its C loads (issued without branches, an
-    // element outside C reading the nearest one inside) before its stores
+    // one block column at a time: its C loads (issued without branches; at
+    // an edge an element outside C reads the nearest one inside and is not
+    // stored) before its stores
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const i64 col = n0 + wn * 64 + j * 16 + li;
-        const i64 colc = min(col, N - 1);
+        const i64 colc = EDGE ? min(col, N - 1) : col;
         double cv[4][4];
         if (!BETA0) {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) cv[i][r] = C[colc * ldc + min(m0 + wm * 64 + i * 16 + lk + 4 * r, M - 1)];
+                for (int r = 0; r < 4; ++r) {
+                    const i64 row = m0 + wm * 64 + i * 16 + lk + 4 * r;
+                    cv[i][r] = C[colc * ldc + (EDGE ? min(row, M - 1) : row)];
+                }
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -284,9 +308,25 @@ __global__ __launch_bounds__(NT, 2) void dgemm_m16_kernel(i64 M, i64 N, i64 K, d
             for (int r = 0; r < 4; ++r) {
                 const i64 row = m0 + wm * 64 + i * 16 + lk + 4 * r;
                 const double v = alpha * acc[i][j][r];
-                if (row < M && col < N) C[col * ldc + row] = BETA0 ? v : __builtin_fma(beta, cv[i][r], v);
+                if (!EDGE || (row < M && col < N)) C[col * ldc + row] = BETA0 ? v : __builtin_fma(beta, cv[i][r], v);
             }
     }
+}
+
+template <bool TA, bool TB, bool BETA0>
+__global__ __launch_bounds__(NT, 2) void dgemm_m16_kernel(i64 M, i64 N, i64 K, double alpha,
+                                                          const double* __restrict__ A, i64 lda,
+                                                          const double* __restrict__ B, i64 ldb, double beta,
+                                                          double* __restrict__ C, i64 ldc, int tiles_m, int tiles_n) {
+    __shared__ double As[2][BK16 * 129];
+    __shared__ double Bs[2][BK16 * 129];
+    int tm, tn;
+    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const i64 m0 = i64(tm) * 128, n0 = i64(tn) * 128;
+    if (m0 + 128 <= M && n0 + 128 <= N && K % BK16 == 0)
+        m16_tile<false, TA, TB, BETA0>(As, Bs, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
+    else
+        m16_tile<true, TA, TB, BETA0>(As, Bs, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
 }
 
 __global__ void dscale_kernel(i64 m, i64 n, double beta, double* __restrict__ C, i64 ldc) {
