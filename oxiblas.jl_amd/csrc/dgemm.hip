@@ -146,25 +146,25 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
     load_slab<B_OC, BN>(rb, B, ldb, n0, 0, N, K, t);
 
     // D_g[i][j] at lane 16i + 4g + j: row 8q + 4(g&1) + i, column 8p + 4(g>>1) + j.
-    // C is loaded before the K loop (tiles up to 64 x 64, whose share of C
-    // is 16 registers per lane), so that the loads travel while the matrix
-    // cores run; all loads are issued without branches (an element outside C
-    // reads the nearest element inside it and is not stored): a load behind
-    // a bounds branch is waited for before the next one is issued.
     const int kq = lane >> 4, grp = (lane >> 2) & 3, li = lane & 3;
     const int erow = wm * (BM / 2) + 4 * (grp & 1) + kq;
     const int ecol = wn * (BN / 2) + 4 * (grp >> 1) + li;
-    constexpr bool C_EARLY = !BETA0 && QM * QN <= 16;
-    double cv[QM][QN];
-    auto load_c = [&]() __attribute__((always_inline)) {
+    // Tiles inside C (tiles up to 64 x 64) combine with C in a staged layout:
+    // thread t owns row sr = t % BM of columns sc0 + SCS*k, a wave covering
+    // BM consecutive rows of a column.  Those C values are loaded before the
+    // K loop, so that they travel while the matrix cores run, and without
+    // branches (an element outside C reads the nearest one inside; edge tiles
+    // do not use them): a load behind a bounds branch is waited for at once.
+    constexpr bool STAGED = BM * BN <= 64 * 64;
+    constexpr int CPT = STAGED ? BM * BN / NT : 1, SCS = NT / BM;
+    const int sr = t % BM, sc0 = t / BM;
+    const bool inner = m0 + BM <= M && n0 + BN <= N;
+    double cs[CPT];
+    if (STAGED && !BETA0) {
 #pragma unroll
-        for (int p = 0; p < QN; ++p) {
-            const i64 col = min(n0 + ecol + 8 * p, N - 1);
-#pragma unroll
-            for (int q = 0; q < QM; ++q) cv[q][p] = C[col * ldc + min(m0 + erow + 8 * q, M - 1)];
-        }
-    };
-    if (C_EARLY) load_c();
+        for (int k = 0; k < CPT; ++k)
+            cs[k] = C[min(n0 + sc0 + SCS * k, N - 1) * ldc + min(m0 + sr, M - 1)];
+    }
     store_slab<A_OC, BM>(As[0], ra, t);
     store_slab<B_OC, BN>(Bs[0], rb, t);
     __syncthreads();
@@ -200,7 +200,37 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
         __syncthreads();
     }
 
-    if (!BETA0 && !C_EARLY) load_c();
+    if (STAGED && inner) {
+        // the accumulators through LDS (the A slabs' space, free after the
+        // last barrier: 2 * BK * (BM + 1) >= BN * (BM + 1) doubles), then
+        // one coalesced pass over C
+        static_assert(!STAGED || 2 * BK >= BN, "staging space");
+        double* E = &As[0][0];
+#pragma unroll
+        for (int p = 0; p < QN; ++p)
+#pragma unroll
+            for (int q = 0; q < QM; ++q) E[(ecol + 8 * p) * LDS_A + erow + 8 * q] = acc[q][p];
+        __syncthreads();
+        double* cc = C + (n0 + sc0) * ldc + m0 + sr;
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const double v = alpha * E[(sc0 + SCS * k) * LDS_A + sr];
+            cc[i64(SCS * k) * ldc] = BETA0 ? v : __builtin_fma(beta, cs[k], v);
+        }
+        return;
+    }
+    // edge tiles (and 128 x 128): all loads of C (issued without branches, an
+    // element outside C reading the nearest one inside and not stored)
+    // before the first store
+    double cv[QM][QN];
+    if (!BETA0) {
+#pragma unroll
+        for (int p = 0; p < QN; ++p) {
+            const i64 col = min(n0 + ecol + 8 * p, N - 1);
+#pragma unroll
+            for (int q = 0; q < QM; ++q) cv[q][p] = C[col * ldc + min(m0 + erow + 8 * q, M - 1)];
+        }
+    }
 #pragma unroll
     for (int p = 0; p < QN; ++p) {
         const i64 col = n0 + ecol + 8 * p;
@@ -226,7 +256,7 @@ constexpr int BK16 = 16;
 // One tile; EDGE: the tile crosses an edge of C or K is not a multiple of
 // the slab depth (bounds-checked loads and stores).
 template <bool EDGE, bool TA, bool TB, bool BETA0>
-__device__ __forceinline__ void m16_tile(double (*As)[BK16 * 129], double (*Bs)[BK16 * 129], i64 M, i64 N, i64 K,
+__device__ __forceinline__ void m16_tile(double (*As)[BK16 * 129], double (*Bs)[BK16 * 129], double* E, i64 M, i64 N, i64 K,
                                          double alpha, const double* __restrict__ A, i64 lda,
                                          const double* __restrict__ B, i64 ldb, double beta, double* __restrict__ C,
                                          i64 ldc, i64 m0, i64 n0) {
@@ -285,9 +315,50 @@ __device__ __forceinline__ void m16_tile(double (*As)[BK16 * 129], double (*Bs)[
         __syncthreads();
     }
 
-    // one block column at a time: its C loads (issued without branches; at
-    // an edge an element outside C reads the nearest one inside and is not
-    // stored) before its stores
+    if (!EDGE) {
+        // Interior tile: C in two halves of 64 columns, each staged through
+        // LDS (128 x 64 doubles, row stride 129) by the two waves that hold
+        // it, then combined with C by all threads, a wave covering 64
+        // consecutive rows of one column (512 contiguous bytes) and every
+        // thread issuing its 32 loads before its first store: one memory
+        // round trip per half.
+        constexpr int LDE = 129;
+        const int er = t & 127, ec0 = t >> 7;     // row in the tile, first column (then every 2nd)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (wn == h) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) E[(j * 16 + li) * LDE + wm * 64 + i * 16 + lk + 4 * r] = acc[i][j][r];
+            }
+            __syncthreads();
+            double* cc = C + (n0 + h * 64 + ec0) * ldc + m0 + er;
+            // two batches of 16 columns, the second's loads in flight while
+            // the first is stored
+            double cv[2][16];
+            auto load16 = [&](int b) __attribute__((always_inline)) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) cv[b][k] = cc[i64(2 * (16 * b + k)) * ldc];
+            };
+            if (!BETA0) { load16(0); load16(1); }
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const int c = 16 * b + k;
+                    const double v = alpha * E[(ec0 + 2 * c) * LDE + er];
+                    cc[i64(2 * c) * ldc] = BETA0 ? v : __builtin_fma(beta, cv[b][k], v);
+                }
+            if (h == 0) __syncthreads();
+        }
+        return;
+    }
+    // Edge tile: one block column at a time, its C loads (issued without
+    // branches; an element outside C reads the nearest one inside and is
+    // not stored) before its stores.
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const i64 col = n0 + wn * 64 + j * 16 + li;
@@ -318,15 +389,17 @@ __global__ __launch_bounds__(NT, 2) void dgemm_m16_kernel(i64 M, i64 N, i64 K, d
                                                           const double* __restrict__ A, i64 lda,
                                                           const double* __restrict__ B, i64 ldb, double beta,
                                                           double* __restrict__ C, i64 ldc, int tiles_m, int tiles_n) {
-    __shared__ double As[2][BK16 * 129];
-    __shared__ double Bs[2][BK16 * 129];
+    // the slabs of A and B, and after the K loop the staging of C
+    __shared__ __align__(16) double lds[4 * BK16 * 129];
+    double (*As)[BK16 * 129] = (double (*)[BK16 * 129])lds;
+    double (*Bs)[BK16 * 129] = (double (*)[BK16 * 129])(lds + 2 * BK16 * 129);
     int tm, tn;
     tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const i64 m0 = i64(tm) * 128, n0 = i64(tn) * 128;
     if (m0 + 128 <= M && n0 + 128 <= N && K % BK16 == 0)
-        m16_tile<false, TA, TB, BETA0>(As, Bs, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
+        m16_tile<false, TA, TB, BETA0>(As, Bs, lds, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
     else
-        m16_tile<true, TA, TB, BETA0>(As, Bs, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
+        m16_tile<true, TA, TB, BETA0>(As, Bs, lds, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
 }
 
 __global__ void dscale_kernel(i64 m, i64 n, double beta, double* __restrict__ C, i64 ldc) {
