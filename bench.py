@@ -135,6 +135,60 @@ def cpu_baseline(n=4096, reps=3):
              "sample": f"scipy-OpenBLAS dgetrf+dgetrs n={n}"})
 
 
+def trailing_shapes(n=16384):
+    """(M, N, K) in the factorization's transposed coordinates of the trailing
+    updates that run on the 128 x 128 kernel, in issue order: getrf's tuned
+    block-size schedule (512 while more than 3/8 of the columns remain, then
+    256) and the GEMM's tile rule (128 x 128 tiles when that grid has at least
+    512 workgroups)."""
+    cdiv = lambda a, b: (a + b - 1) // b
+    wide_until = n - (3 * n) // 8
+    nb_at = lambda j: 512 if j < wide_until else 256
+    out, j0 = [], 0
+    while j0 < n:
+        j1 = min(j0 + nb_at(j0), n)
+        if j1 >= n:
+            break
+        j2 = min(j1 + nb_at(j1), n)
+        for (M, N) in ((j2 - j1, n - j1), (n - j2, n - j1)):
+            if M > 0 and cdiv(M, 128) * cdiv(N, 128) >= 512:
+                out.append((M, N, j1 - j0))
+        j0 = j1
+    return out
+
+
+def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, out_json):
+    """HBM bytes per dgemm_m16_kernel launch of the n = 16384 factorization
+    from a FETCH_SIZE run and a WRITE_SIZE run of
+    'bench.py --steps 1 --warmup 0 --no-cpu --no-extra --no-mixed', each
+    scaled by what the same counter reads for a 1 GiB device copy
+    ('bench.py --copy-probe 1024')."""
+    import csv
+
+    def rows(path, counter, name):
+        return [float(r["Counter_Value"]) for r in csv.DictReader(open(path))
+                if r["Counter_Name"] == counter and name in r["Kernel_Name"]]
+    f = rows(fetch_csv, "FETCH_SIZE", "dgemm_m16_kernel<false, false, false>")
+    w = rows(write_csv, "WRITE_SIZE", "dgemm_m16_kernel<false, false, false>")
+    cf = rows(copy_fetch_csv, "FETCH_SIZE", "copyBuffer")
+    cw = rows(copy_write_csv, "WRITE_SIZE", "copyBuffer")
+    kf = (1 << 20) / (sum(cf) / len(cf))          # KiB per KiB the counter reports
+    kw = (1 << 20) / (sum(cw) / len(cw))
+    shapes = trailing_shapes()
+    reps = len(f) // len(shapes)
+    assert reps >= 1 and len(f) == reps * len(shapes) and len(w) == len(f), (len(f), len(w), len(shapes))
+    alg = [8 * (M * K + K * N + 2 * M * N) for (M, N, K) in shapes] * reps
+    meas = [(a * kf + b * kw) * 1024 for a, b in zip(f, w)]
+    out = {"kernel": "dgemm_m16_kernel: getrf trailing updates on 128x128 tiles, n=16384, tuned block sizes",
+           "hbm_bytes_per_launch": round(sum(meas) / len(meas)),
+           "algorithmic_bytes_per_launch": round(sum(alg) / len(alg)),
+           "fetch_scale": round(kf, 4), "write_scale": round(kw, 4),
+           "launches_measured": len(meas), "launches_per_factorization": len(shapes)}
+    with open(out_json, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +200,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--kappa", type=float, default=1e6)
+    ap.add_argument("--traffic-summary", nargs=5, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV",
+                                                           "COPY_WRITE_CSV", "OUT_JSON"),
+                    help="write the HBM traffic per dgemm_m16_kernel launch from counter runs and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 times): the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -156,6 +213,9 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.traffic_summary:
+        traffic_summary(*args.traffic_summary)
+        return
     if args.copy_probe:
         nel = args.copy_probe * (1 << 20) // 8
         src = torch.randn(nel, dtype=torch.float64, device="cuda")
@@ -199,8 +259,9 @@ def main():
     berr = torch.linalg.norm(r).item() / (torch.linalg.matrix_norm(At).item() * torch.linalg.norm(xt).item() * n * eps)
     del F, x
 
-    # roofline of the dominant kernel: the trailing-update GEMM inside getrf,
-    # events around every launch on the stream it runs on
+    # roofline of the dominant kernel: the trailing-update GEMM inside getrf on
+    # its 128 x 128 tiles (dgemm_m16_kernel), events around each of those
+    # launches on the stream it runs on
     h = ox.handle()
     h._L.oxi_profile_enable(h.ptr, 1)
     F, x = step()
@@ -210,7 +271,7 @@ def main():
     h._L.oxi_profile_enable(h.ptr, 0)
     del F, x
     gemm_tflops = fl.value / (ms.value * 1e-3) / 1e12 if ms.value > 0 else 0.0
-    log(f"trailing GEMM: {nl.value} launches, {ms.value:.2f} ms, {gemm_tflops:.1f} TFLOP/s")
+    log(f"trailing GEMM (dgemm_m16_kernel): {nl.value} launches, {ms.value:.2f} ms, {gemm_tflops:.1f} TFLOP/s")
 
     extra = {}
     if not args.no_extra:
@@ -258,7 +319,7 @@ def main():
         "pct_fp64_peak": round(100.0 * value / (PEAK_FP64_TFLOPS * 1e3), 2),
         "roofline": {"bound": "mfma", "achieved": round(gemm_tflops, 3), "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(gemm_tflops / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                     "kernel": "trailing-update dgemm in getrf (all launches of one step)",
+                     "kernel": "dgemm_m16_kernel: getrf trailing updates on 128x128 tiles (all such launches of one step)",
                      "launches": int(nl.value)},
         "cpu_baseline": cpu,
         "cpu_openblas": cpu_ob,
@@ -271,7 +332,7 @@ def main():
         t = json.load(open(tf))
         out["roofline"]["traffic"] = t["hbm_bytes_per_launch"]
         out["roofline"]["traffic_algorithmic"] = t["algorithmic_bytes_per_launch"]
-        out["roofline"]["traffic_launches"] = f"{t['launches_measured']} launches of more than 128 workgroups"
+        out["roofline"]["traffic_launches"] = f"{t['launches_measured']} dgemm_m16_kernel launches"
     if mixed:
         best = max(mixed.values(), key=lambda v: v["speedup_vs_fp64_lu_solve"] if v["converged"] else 0)
         out["mixed"] = {"kappa": args.kappa, "n": n, "best_speedup_vs_fp64_lu_solve":

@@ -478,6 +478,12 @@ void launch_variant(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const doub
 
 }  // namespace
 
+bool oxi_dgemm_large_tiles(i64 m, i64 n) {
+    int bm, bn;
+    choose_tile(m, n, bm, bn);
+    return bm == 16;
+}
+
 int oxi_launch_dscale(hipStream_t s, i64 m, i64 n, double beta, double* C, i64 ldc) {
     if (m <= 0 || n <= 0 || beta == 1.0) return OXI_OK;
     i64 total = m * n;

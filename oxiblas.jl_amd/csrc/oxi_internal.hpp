@@ -77,6 +77,8 @@ int oxi_launch_dgemm(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n
                      const double* A, i64 lda, const double* B, i64 ldb, double beta, double* C, i64 ldc);
 // C <- beta*C (beta == 0 writes zeros without reading C).
 int oxi_launch_dscale(hipStream_t s, i64 m, i64 n, double beta, double* C, i64 ldc);
+// whether oxi_launch_dgemm runs an m x n product on its 128 x 128 (16x16x4) kernel
+bool oxi_dgemm_large_tiles(i64 m, i64 n);
 // Left-side triangular solve, host-recursive with GEMM updates.
 int oxi_dtrsm_left(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 nrhs,
                    const double* A, i64 lda, double* B, i64 ldb);
