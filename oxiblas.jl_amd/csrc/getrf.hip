@@ -233,30 +233,46 @@ __global__ __launch_bounds__(256) void perm_pairs_kernel(const i64* __restrict__
 
 // Apply pairs to columns of W: W[c][dst] <- W[c][src] for every c in
 // [c0, c1) (W column = A row, W row = A column).  A workgroup stages 16 W rows
-// x all pairs through LDS so the reads complete before any write.
+// x all pairs through LDS so the reads complete before any write.  The pair
+// list is copied to LDS first, and each thread issues its loads eight at a
+// time without branches (a padding slot reloads the last pair): a pair index
+// read from global memory, or a load behind a bounds branch, would cost one
+// memory round trip per pair.
 constexpr int PAIR_ROWS = 16;
 template <typename T>
 __global__ __launch_bounds__(256) void apply_pairs_w_kernel(T* __restrict__ Wm, i64 ldw, i64 c0, i64 c1,
                                                             const int* __restrict__ pairs, i64 roff) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char buf_raw[];  // [npairs][PAIR_ROWS]
+    // [npairs][PAIR_ROWS] values, then the pair list (dst, src) per pair
+    extern __shared__ __attribute__((aligned(16))) unsigned char buf_raw[];
     T* buf = (T*)buf_raw;
     const int np = pairs[0];
     if (np == 0) return;
-    const i64 cbase = c0 + i64(blockIdx.x) * PAIR_ROWS;
+    int* pl = (int*)(buf_raw + size_t(np) * PAIR_ROWS * sizeof(T));
     const int t = threadIdx.x;
+    for (int i = t; i < 2 * np; i += blockDim.x) pl[i] = pairs[1 + i];
+    __syncthreads();
+    const i64 cbase = c0 + i64(blockIdx.x) * PAIR_ROWS;
     const int r = t % PAIR_ROWS, q0 = t / PAIR_ROWS, qs = blockDim.x / PAIR_ROWS;
     const i64 c = cbase + r;
     const bool ok = c < c1;
-    for (int q = q0; q < np; q += qs) {
-        const i64 src = pairs[2 + 2 * q] + roff;
-        buf[q * PAIR_ROWS + r] = ok ? Wm[c + src * ldw] : T(0);
+    const i64 cr = ok ? c : c0;                      // a column inside the range to read
+    constexpr int U = 8;
+    for (int q = q0; q < np; q += U * qs) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int qq = min(q + u * qs, np - 1);
+            v[u] = Wm[cr + (i64(pl[2 * qq + 1]) + roff) * ldw];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int qq = q + u * qs;
+            if (qq < np) buf[qq * PAIR_ROWS + r] = v[u];
+        }
     }
     __syncthreads();
     if (ok)
-        for (int q = q0; q < np; q += qs) {
-            const i64 dst = pairs[1 + 2 * q] + roff;
-            Wm[c + dst * ldw] = buf[q * PAIR_ROWS + r];
-        }
+        for (int q = q0; q < np; q += qs) Wm[c + (i64(pl[2 * q]) + roff) * ldw] = buf[q * PAIR_ROWS + r];
 }
 
 // Column-major (LAPACK) form: rows of A (n x ncols, lda) permuted by pairs.
@@ -340,7 +356,7 @@ int swap_w(GetrfCtx<T>& g, i64 k1, i64 k2, i64 c0, i64 c1, int inverse) {
     if (cnt > PERM_MAX_STEPS) return oxi_set_error(g.h, OXI_ERR_UNSUPPORTED, "laswp block of %d steps", cnt);
     hipLaunchKernelGGL(perm_pairs_kernel, dim3(1), dim3(256), 0, g.s, g.ipiv, i64(0), k1, cnt, inverse, g.pairs);
     OXI_TRY(launch_check(g.h, "perm_pairs"));
-    const size_t shm = size_t(2 * cnt) * PAIR_ROWS * sizeof(T);
+    const size_t shm = size_t(2 * cnt) * (PAIR_ROWS * sizeof(T) + 2 * sizeof(int));
     hipLaunchKernelGGL(apply_pairs_w_kernel<T>, dim3(unsigned(cdiv(c1 - c0, PAIR_ROWS))), dim3(256), shm, g.s, g.Wm,
                        g.ldw, c0, c1, g.pairs, i64(0));
     return launch_check(g.h, "apply_pairs_w");
@@ -468,7 +484,7 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
 
     static bool attr_set = false;
     if (!attr_set) {
-        // the pair-staging kernel may use up to 2 * 512 rows * 16 doubles of LDS
+        // the pair-staging kernel may use up to 2 * 512 pairs * (16 doubles + 2 ints) of LDS
         OXI_HIP(h, hipFuncSetAttribute((const void*)apply_pairs_w_kernel<double>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         OXI_HIP(h, hipFuncSetAttribute((const void*)apply_pairs_w_kernel<float>,
