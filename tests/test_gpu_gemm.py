@@ -175,3 +175,35 @@ def test_full_size_shapes(gpu, shape):
             got = Cd.buf.view(n, m)[jj, ii].cpu().numpy()
             sampled_check(rows_a, cols_b, c0, got, 1.0, -1.0)
             del A, B, Cd
+
+
+# Shapes that select each GEMM kernel (the tile rule: 128 x 128 on the 16x16x4
+# kernel when that grid has >= 512 workgroups, 64 x 64 when its grid has
+# >= 384, else 32 x 32), once with every tile inside C and K a multiple of the
+# slab depth (the unchecked interior path) and once with edges in M, N and K.
+TILE_SHAPES = [
+    ("128x128 interior", 2048, 4096, 256), ("128x128 edges", 2100, 4100, 300),
+    ("64x64 interior", 1600, 1088, 96), ("64x64 edges", 1630, 1100, 70),
+    ("32x32 interior", 640, 320, 64), ("32x32 edges", 650, 333, 37),
+]
+
+
+@pytest.mark.parametrize("name,m,n,k", TILE_SHAPES, ids=[s[0] for s in TILE_SHAPES])
+def test_tile_kernels_every_entry(gpu, rng, name, m, n, k):
+    """Every entry of C for each kernel path, all four transpose pairs,
+    beta = 0 (C never read) and beta != 0, against numpy's product within the
+    normwise bound of the shape tests."""
+    ox = gpu
+    for ta in "NT":
+        for tb in "NT":
+            A = rng.standard_normal((m, k) if ta == "N" else (k, m))
+            B = rng.standard_normal((k, n) if tb == "N" else (n, k))
+            for al, be in ((1.0, 0.0), (-1.0, 0.75)):
+                C0 = np.asfortranarray(rng.standard_normal((m, n))) if be else np.full((m, n), np.nan, order="F")
+                got = run(ox, C0, A, B, al, be, ta, tb)
+                ref = al * (op(A, ta) @ op(B, tb)) + (be * C0 if be else 0.0)
+                bound = 2 * (k + 2) * EPS * (abs(al) * np.linalg.norm(A) * np.linalg.norm(B) +
+                                             (abs(be) * np.linalg.norm(C0) if be else 0.0))
+                assert np.isfinite(got).all(), (name, ta, tb, be)
+                assert np.linalg.norm(got - ref) <= bound, (name, ta, tb, be)
+                np.testing.assert_allclose(got, ref, rtol=3e-12, atol=3e-12 * k)
