@@ -478,6 +478,17 @@ void launch_variant(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const doub
 
 }  // namespace
 
+int oxi_dgemm_right64_inplace(hipStream_t s, i64 rows, const double* V, double* X, i64 ldx) {
+    // X[:, 0:64] <- X[:, 0:64] * V (V 64 x 64, ld 64) in place: 32 x 64 tiles,
+    // so each workgroup owns all 64 columns of its rows, and it writes them
+    // only after its last read of them (the epilogue follows the K loop)
+    if (rows <= 0) return OXI_OK;
+    const int tiles_m = int(cdiv(rows, 32));
+    hipLaunchKernelGGL((dgemm_mfma_kernel<32, 64, false, false, true>), dim3(tiles_m), dim3(NT), 0, s, rows, i64(64),
+                       i64(64), 1.0, X, ldx, V, i64(64), 0.0, X, ldx, tiles_m, 1);
+    return hipGetLastError() == hipSuccess ? OXI_OK : OXI_ERR_HIP;
+}
+
 bool oxi_dgemm_large_tiles(i64 m, i64 n) {
     int bm, bn;
     choose_tile(m, n, bm, bn);

@@ -77,7 +77,8 @@ int oxi_create(int device, oxi_handle_t* handle) {
         hipEventCreateWithFlags(&h->ev_a, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_b, hipEventDisableTiming) != hipSuccess ||
         hipMalloc(&h->dev_flags, 64 * sizeof(int)) != hipSuccess ||
-        hipMalloc(&h->dev_info, 8 * sizeof(i64)) != hipSuccess) {
+        hipMalloc(&h->dev_info, 8 * sizeof(i64)) != hipSuccess ||
+        hipMalloc(&h->trsm_inv, size_t(TRSM_INV_BLOCKS) * 4096 * sizeof(double)) != hipSuccess) {
         oxi_destroy(h);
         return OXI_ERR_HIP;
     }
@@ -115,6 +116,7 @@ int oxi_destroy(oxi_handle_t h) {
     if (h->ws2) (void)hipFree(h->ws2);
     if (h->dev_flags) (void)hipFree(h->dev_flags);
     if (h->dev_info) (void)hipFree(h->dev_info);
+    if (h->trsm_inv) (void)hipFree(h->trsm_inv);
     if (h->ev_a) (void)hipEventDestroy(h->ev_a);
     if (h->ev_b) (void)hipEventDestroy(h->ev_b);
     if (h->ev_c) (void)hipEventDestroy(h->ev_c);

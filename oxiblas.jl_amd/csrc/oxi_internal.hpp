@@ -17,6 +17,7 @@ typedef int64_t i64;
 // Handle: one device, one user stream, two internal streams for the panel /
 // trailing-update overlap, and a grow-only device workspace.
 // ---------------------------------------------------------------------------
+constexpr int TRSM_INV_BLOCKS = 256;
 struct oxi_handle_s {
     int device = 0;
     hipStream_t stream = nullptr;       // caller's stream (null = default stream)
@@ -25,6 +26,9 @@ struct oxi_handle_s {
     // (spread evenly over the XCDs) so that the panel running beside them
     // finds free CUs; null when no CU is kept free
     hipStream_t bulk = nullptr;
+    // inverses of 64 x 64 diagonal blocks for tall right-side triangular
+    // solves (trsm.hip), TRSM_INV_BLOCKS blocks of 64 x 64 doubles
+    double* trsm_inv = nullptr;
     int panel_cus = 0;
     hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr;
     void* ws = nullptr;                 // device workspace (kernels' scratch)
@@ -79,6 +83,8 @@ int oxi_launch_dgemm(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n
 int oxi_launch_dscale(hipStream_t s, i64 m, i64 n, double beta, double* C, i64 ldc);
 // whether oxi_launch_dgemm runs an m x n product on its 128 x 128 (16x16x4) kernel
 bool oxi_dgemm_large_tiles(i64 m, i64 n);
+// X[:, 0:64] <- X[:, 0:64] * V in place (V 64 x 64, column-major, ld 64)
+int oxi_dgemm_right64_inplace(hipStream_t s, i64 rows, const double* V, double* X, i64 ldx);
 // Left-side triangular solve, host-recursive with GEMM updates.
 int oxi_dtrsm_left(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 nrhs,
                    const double* A, i64 lda, double* B, i64 ldb);

@@ -126,6 +126,43 @@ void launch_right_leaf(hipStream_t s, int unit, int trans, i64 rows, int n, cons
     }
 }
 
+// Inverses of the 64 x 64 diagonal blocks of an upper op(T): block b's
+// V = U_bb^-1 goes to inv + 4096 b, column-major (ld 64, zeros below the
+// diagonal), by back substitution, thread j computing column j:
+// V[i][j] = (delta_ij - sum_{k=i+1..j} U[i][k] V[k][j]) / U[i][i].
+template <bool UNIT, bool TRANS>
+__global__ __launch_bounds__(64) void trsm_inv64_kernel(const double* __restrict__ T, i64 ldt,
+                                                        double* __restrict__ inv) {
+    __shared__ double U[64][65];
+    __shared__ double V[64][65];
+    const int tj = threadIdx.x;
+    const i64 o = i64(blockIdx.x) * 64;
+    // T's element (o + tj, o + q), the thread index along the contiguous
+    // direction, 64 loads per thread in flight; it is op(T)[tj][q], or
+    // op(T)[q][tj] when TRANS
+    double r[64];
+#pragma unroll
+    for (int q = 0; q < 64; ++q) r[q] = T[(o + tj) + (o + q) * ldt];
+#pragma unroll
+    for (int q = 0; q < 64; ++q) {
+        if (TRANS) U[q][tj] = r[q];   // element (row q, column tj) of op(T)
+        else U[tj][q] = r[q];         // element (row tj, column q)
+    }
+    __syncthreads();
+    const int j = tj;
+    for (int i = 63; i >= 0; --i) {
+        if (i > j) { V[i][j] = 0.0; continue; }
+        double acc = i == j ? 1.0 : 0.0;
+        for (int k = i + 1; k <= j; ++k) acc = __builtin_fma(-U[i][k], V[k][j], acc);
+        V[i][j] = UNIT ? acc : acc / U[i][i];
+    }
+    __syncthreads();
+    // column-major store, the thread index along rows
+    double* out = inv + 4096 * i64(blockIdx.x);
+#pragma unroll 8
+    for (int c = 0; c < 64; ++c) out[tj + 64 * c] = V[tj][c];
+}
+
 // Left side: X (n x nrhs) <- op(T)^-1 X, one wave per right-hand side, lane =
 // row.  LOWER refers to op(T).  Per element: s = b_i; s = fma(-T[i][p], x_p, s)
 // in order of p; s / T[i][i] when not unit (src/blas/level3.jl:309-318,
@@ -194,22 +231,51 @@ int gemm_t<float>(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n, i
 // Right-side triangular solve X <- X * op(T)^-1 where op(T) is upper
 // triangular (T upper and trans == 0, or T lower and trans == 1), recursive
 // halving with GEMM updates (src/blas/level3.jl:358-376).
+// inv: when not null (FP64 only), the inverses of op(T)'s 64 x 64 diagonal
+// blocks (n a multiple of 64), and each 64-column leaf is the in-place
+// product X_b <- X_b V_b on the matrix cores instead of a column sweep.
 template <typename TY>
 static int trsm_right_upper(oxi_handle_t h, hipStream_t s, int trans, int unit, i64 rows, i64 n, const TY* T,
-                            i64 ldt, TY* X, i64 ldx) {
+                            i64 ldt, TY* X, i64 ldx, const double* inv = nullptr) {
     if (rows <= 0 || n <= 0) return OXI_OK;
     if (n <= TRL) {
+        if constexpr (sizeof(TY) == 8) {
+            if (inv) {
+                if (oxi_dgemm_right64_inplace(s, rows, inv, (double*)X, ldx) != OXI_OK)
+                    return oxi_set_error(h, OXI_ERR_HIP, "trsm in-place block product launch failed");
+                return OXI_OK;
+            }
+        }
         launch_right_leaf<TY, false>(s, unit, trans, rows, int(n), T, ldt, X, ldx);
         return launch_check(h, "trsm_right_leaf");
     }
     i64 hh = n / 2;
     if (hh > TRL) hh = (hh / TRL) * TRL;
     // X1 = X[:, :hh], X2 = X[:, hh:]; op(T) = [U11 U12; 0 U22]
-    OXI_TRY(trsm_right_upper(h, s, trans, unit, rows, hh, T, ldt, X, ldx));
+    OXI_TRY(trsm_right_upper(h, s, trans, unit, rows, hh, T, ldt, X, ldx, inv));
     // X2 -= X1 * U12, U12 = op(T)[0:hh, hh:n]
     const TY* U12 = trans ? T + hh : T + hh * ldt;  // op(T)[0][hh]: trans -> T[hh][0]
     OXI_TRY(gemm_t<TY>(h, s, 0, trans, rows, n - hh, hh, TY(-1), X, ldx, U12, ldt, TY(1), X + hh * ldx, ldx));
-    return trsm_right_upper(h, s, trans, unit, rows, n - hh, T + hh + hh * ldt, ldt, X + hh * ldx, ldx);
+    return trsm_right_upper(h, s, trans, unit, rows, n - hh, T + hh + hh * ldt, ldt, X + hh * ldx, ldx,
+                            inv ? inv + (hh / TRL) * 4096 : nullptr);
+}
+
+// Tall right-side solves (FP64, rows >= 1024, n a multiple of 64): the
+// diagonal blocks' inverses first, then the recursion with in-place products
+// for the leaves (a leaf's column sweep is a chain of 64 dependent steps per
+// row; the product runs on the matrix cores).
+static const double* right_upper_inverses(oxi_handle_t h, hipStream_t s, int trans, int unit, i64 rows, i64 n,
+                                          const double* T, i64 ldt) {
+    if (rows < 1024 || n < TRL || n % TRL || n / TRL > TRSM_INV_BLOCKS || !h->trsm_inv) return nullptr;
+    const dim3 grid(unsigned(n / TRL)), block(64);
+    if (unit) {
+        if (trans) hipLaunchKernelGGL((trsm_inv64_kernel<true, true>), grid, block, 0, s, T, ldt, h->trsm_inv);
+        else hipLaunchKernelGGL((trsm_inv64_kernel<true, false>), grid, block, 0, s, T, ldt, h->trsm_inv);
+    } else {
+        if (trans) hipLaunchKernelGGL((trsm_inv64_kernel<false, true>), grid, block, 0, s, T, ldt, h->trsm_inv);
+        else hipLaunchKernelGGL((trsm_inv64_kernel<false, false>), grid, block, 0, s, T, ldt, h->trsm_inv);
+    }
+    return hipGetLastError() == hipSuccess ? h->trsm_inv : nullptr;
 }
 
 // Right-side solve with op(T) lower: X <- X * op(T)^-1 (backward over columns).
@@ -274,7 +340,11 @@ template <typename TY>
 static int trsm_right_t(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 n, const TY* A,
                         i64 lda, TY* B, i64 ldb) {
     const bool upper_op = (upper != 0) != (trans != 0);
-    if (upper_op) return trsm_right_upper(h, s, trans, unit, m, n, A, lda, B, ldb);
+    if (upper_op) {
+        const double* inv = nullptr;
+        if constexpr (sizeof(TY) == 8) inv = right_upper_inverses(h, s, trans, unit, m, n, (const double*)A, lda);
+        return trsm_right_upper(h, s, trans, unit, m, n, A, lda, B, ldb, inv);
+    }
     return trsm_right_lower(h, s, trans, unit, m, n, A, lda, B, ldb);
 }
 

@@ -45,6 +45,22 @@ def test_trsm_right_tall(gpu, rng):
     assert np.abs(X - Xo).max() <= 1e-11 * np.abs(Xo).max()
 
 
+@pytest.mark.parametrize("uplo,trans,diag", list(itertools.product("UL", "NT", "NU")))
+def test_trsm_right_tall_all_variants(gpu, rng, uplo, trans, diag):
+    """Right-side solves with 1500 rows: when op(T) is upper and n a multiple
+    of 64 they invert op(T)'s 64 x 64 diagonal blocks and replace each
+    64-column sweep by an in-place block product; n = 100 and the lower op(T)
+    keep the column sweeps."""
+    for n in (64, 192, 320, 100):
+        T = tri(rng, n, uplo, diag == "U")
+        B = np.asfortranarray(rng.standard_normal((1500, n)))
+        X = gpu.trsm_(B.copy(order="F"), T, -0.5, "R", uplo, trans, diag == "U")
+        Xo = B.copy(order="F")
+        assert O.dtrsm(Xo, T, -0.5, "R", uplo, trans, diag == "U") == 0
+        scale = np.abs(Xo).max() + 1e-300
+        assert np.abs(X - Xo).max() <= 1e-12 * scale * n, (uplo, trans, diag, n)
+
+
 def test_trsm_singular_reported_before_write(gpu, rng):
     T = tri(rng, 20, "U", False)
     T[7, 7] = 0.0
