@@ -534,13 +534,16 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
         const i64 j1 = j0 + jb;
         jn = j1;
         if (panel_on_side) OXI_HIP(h, hipStreamWaitEvent(s, h->ev_b, 0));
-        OXI_TRY(swap_w(g, j0, j1 - 1, j1, n, 0));
+        const bool next = j1 < q;                         // then j1 < m and j1 < n
+        const i64 j2 = next && lookahead ? std::min<i64>(j1 + nb_at(j1), q) : j1;
+        // this panel's interchanges on the look-ahead block first (the next
+        // panel waits for them), on the rest of the trailing columns after
+        // the next panel has been started
+        OXI_TRY(swap_w(g, j0, j1 - 1, j1, j2 > j1 ? j2 : n, 0));
         if (j1 >= n) {
             if (j0 > 0) OXI_TRY(swap_w(g, j0, j1 - 1, 0, j0, 0));
             continue;
         }
-        const bool next = j1 < q;                         // then j1 < m and j1 < n
-        const i64 j2 = next && lookahead ? std::min<i64>(j1 + nb_at(j1), q) : j1;
         // look-ahead block: the next panel's columns [j1, j2)
         if (j2 > j1) {
             OXI_TRY(trsm_ru(h, s, j2 - j1, jb, g.Wm + j0 + j0 * ldw, ldw, g.Wm + j1 + j0 * ldw, ldw));
@@ -555,6 +558,7 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
             panel_on_side = true;
         }
         // the rest of the trailing matrix: columns [j2, n)
+        if (j2 > j1) OXI_TRY(swap_w(g, j0, j1 - 1, j2, n, 0));
         if (j2 < n) {
             OXI_TRY(trsm_ru(h, s, n - j2, jb, g.Wm + j0 + j0 * ldw, ldw, g.Wm + j2 + j0 * ldw, ldw));
             if (j1 < m) {
