@@ -449,9 +449,11 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
 // leaf's w rows and i is its first occurrence.  pos = -1 for unused lanes.
 // The interchanges are then replayed on slot ids: afterwards lane s holds
 // the slot whose original content ends at slot s.
-__device__ __forceinline__ void leaf_slots(const i64* ipiv, i64 c0, int w, int k_lo, int lane, i64& pos, int& src) {
+// pi_all: lane i < w holds step i's pivot position relative to c0 (-1 beyond).
+__device__ __forceinline__ void leaf_slots(i64 pi_all, int w, int k_lo, int lane, i64& pos, int& src) {
     const int n = w - k_lo;
-    const i64 pi = lane < n ? ipiv[c0 + k_lo + lane] - 1 - c0 : -1;
+    const i64 pi_sh = (i64)__shfl((long long)pi_all, min(k_lo + lane, 63));
+    const i64 pi = lane < n ? pi_sh : -1;
     int first = lane;
     for (int i = 0; i < n; ++i) {
         const i64 q = (i64)readlane64(u64(pi), i);
@@ -490,13 +492,16 @@ __device__ __forceinline__ void leaf_slots(const i64* ipiv, i64 c0, int w, int k
 template <typename T>
 __global__ __launch_bounds__(64) void leaf_swaps_kernel(T* Wm, i64 ldw, i64 c0, int w, const i64* ipiv,
                                                         const int* flags, i64 x_lo, i64 x_hi) {
-    if (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     const int lane = threadIdx.x;
+    // the pivots are loaded with the flag, not after its test: one memory
+    // round trip less (ipiv lies in its own allocation, always readable)
+    const i64 pi_all = lane < w ? ipiv[c0 + lane] - 1 - c0 : -1;
+    if (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     const int b = blockIdx.x;
     if (b < w - 1) {
         const int c = b;
         i64 pos; int src;
-        leaf_slots(ipiv, c0, w, c + 1, lane, pos, src);
+        leaf_slots(pi_all, w, c + 1, lane, pos, src);
         const i64 psrc = (i64)__shfl((long long)pos, src);
         T v = T(0);
         if (pos >= 0 && psrc != pos) v = Wm[(c0 + psrc) * ldw + c0 + c];
@@ -510,7 +515,7 @@ __global__ __launch_bounds__(64) void leaf_swaps_kernel(T* Wm, i64 ldw, i64 c0, 
     if (fb < tl) { x = x_lo + fb * 64 + lane; ok = x < c0; }
     else { x = c0 + w + (fb - tl) * 64 + lane; ok = x < x_hi; }
     i64 pos; int src;
-    leaf_slots(ipiv, c0, w, 0, lane, pos, src);
+    leaf_slots(pi_all, w, 0, lane, pos, src);
     // slot s moves when its source differs (at most 2w <= 64 slots).  All
     // loads are issued before any store and without branches: a slot that
     // does not move reads its own position, a lane outside the columns
