@@ -140,7 +140,7 @@ def trailing_shapes(n=16384):
     updates that run on the 128 x 128 kernel, in issue order: getrf's tuned
     block-size schedule (512 while more than 3/8 of the columns remain, then
     256) and the GEMM's tile rule (128 x 128 tiles when that grid has at least
-    512 workgroups)."""
+    448 workgroups)."""
     cdiv = lambda a, b: (a + b - 1) // b
     wide_until = n - (3 * n) // 8
     nb_at = lambda j: 512 if j < wide_until else 256
@@ -151,7 +151,7 @@ def trailing_shapes(n=16384):
             break
         j2 = min(j1 + nb_at(j1), n)
         for (M, N) in ((j2 - j1, n - j1), (n - j2, n - j1)):
-            if M > 0 and cdiv(M, 128) * cdiv(N, 128) >= 512:
+            if M > 0 and cdiv(M, 128) * cdiv(N, 128) >= 448:
                 out.append((M, N, j1 - j0))
         j0 = j1
     return out

@@ -439,8 +439,9 @@ void launch_m16(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* 
 
 // Tile shape of C per workgroup (measured on MI355X over the LU's products
 // and square ones, K = 32 .. 8192): 128 x 128 on the 16x16x4 instruction
-// where its grid has at least two workgroups per CU (8192^3: 62 TF/s against
-// 54 for 64 x 64); else 64 x 64 tiles on the 4x4x4_4b instruction (66 KB of
+// where its grid has at least 1.75 workgroups per CU (8192^3: 64 TF/s against
+// 54 for 64 x 64; 512 x 15872 x 512, 496 tiles: 170 us against 206, but
+// 512 x 12000 x 512, 376 tiles: 191 us against 166); else 64 x 64 tiles on the 4x4x4_4b instruction (66 KB of
 // LDS, two workgroups per CU); 32 x 32 where the 64 x 64 grid has fewer than
 // about 1.5 workgroups per CU (the LU's look-ahead block, its panel updates
 // and the TRSM updates: products of 32..256 rows).  A side no longer than 32
@@ -458,7 +459,7 @@ void choose_tile(i64 m, i64 n, int& bm, int& bn) {
         }
     }
     if (forced) { bm = bn = forced; return; }
-    if (cdiv(m, 128) * cdiv(n, 128) >= 2 * NCU) { bm = bn = 16; return; }
+    if (cdiv(m, 128) * cdiv(n, 128) >= NCU * 7 / 4) { bm = bn = 16; return; }
     bm = m <= 32 ? 32 : 64;
     bn = n <= 32 ? 32 : 64;
     if (cdiv(m, bm) * cdiv(n, bn) < i64(NCU) * 3 / 2) bm = bn = 32;

@@ -178,7 +178,7 @@ def test_full_size_shapes(gpu, shape):
 
 
 # Shapes that select each GEMM kernel (the tile rule: 128 x 128 on the 16x16x4
-# kernel when that grid has >= 512 workgroups, 64 x 64 when its grid has
+# kernel when that grid has >= 448 workgroups, 64 x 64 when its grid has
 # >= 384, else 32 x 32), once with every tile inside C and K a multiple of the
 # slab depth (the unchecked interior path) and once with edges in M, N and K.
 TILE_SHAPES = [
