@@ -4,9 +4,9 @@
 // Float32 path of gemm! (src/blas/gemm.jl:625-703; GemmConfig kc=896 for
 // 4-byte types, :14).
 //
-// Same decomposition as the FP64 kernel: a 256-thread workgroup owns a
-// 128x128 tile, each wave 64x64 = 4x4 blocks of 16x16, BK=16 slabs staged in
-// LDS and double-buffered.  Measured lane layout of v_mfma_f32_16x16x4f32 on
+// A 256-thread workgroup owns a 128 x 128, 64 x 64 or 32 x 32 tile (chosen
+// per product so that the grid fills the CUs), each wave a quarter of it in
+// blocks of 16 x 16, BK=16 slabs staged in LDS and double-buffered.  Measured lane layout of v_mfma_f32_16x16x4f32 on
 // gfx950: A[l%16][l/16] at lane l, B[l/16][l%16], D item r of lane l = row
 // 4*(l/16)+r, column l%16.
 #include "oxi_internal.hpp"
@@ -15,31 +15,29 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
-constexpr int LDS_A = BM + 1, LDS_B = BN + 1;
-constexpr int LOADS = BM * BK / NT;
+constexpr int BK = 16, NT = 256;
 
-template <bool OUTER_CONTIG>
-__device__ __forceinline__ void load_slab(float (&r)[LOADS], const float* __restrict__ X, i64 ldx, i64 o0, i64 k0,
-                                          i64 outer, i64 K, int t) {
+template <bool OUTER_CONTIG, int O>
+__device__ __forceinline__ void load_slab(float (&r)[O * BK / NT], const float* __restrict__ X, i64 ldx, i64 o0,
+                                          i64 k0, i64 outer, i64 K, int t) {
 #pragma unroll
-    for (int i = 0; i < LOADS; ++i) {
+    for (int i = 0; i < O * BK / NT; ++i) {
         int o, kk;
-        if (OUTER_CONTIG) { o = t % BM; kk = t / BM + i * (NT / BM); }
+        if (OUTER_CONTIG) { o = t % O; kk = t / O + i * (NT / O); }
         else { kk = t % BK; o = t / BK + i * (NT / BK); }
         const i64 go = o0 + o, gk = k0 + kk;
         r[i] = (go < outer && gk < K) ? (OUTER_CONTIG ? X[go + gk * ldx] : X[gk + go * ldx]) : 0.0f;
     }
 }
 
-template <bool OUTER_CONTIG>
-__device__ __forceinline__ void store_slab(float* __restrict__ S, int lds, const float (&r)[LOADS], int t) {
+template <bool OUTER_CONTIG, int O>
+__device__ __forceinline__ void store_slab(float* __restrict__ S, const float (&r)[O * BK / NT], int t) {
 #pragma unroll
-    for (int i = 0; i < LOADS; ++i) {
+    for (int i = 0; i < O * BK / NT; ++i) {
         int o, kk;
-        if (OUTER_CONTIG) { o = t % BM; kk = t / BM + i * (NT / BM); }
+        if (OUTER_CONTIG) { o = t % O; kk = t / O + i * (NT / O); }
         else { kk = t % BK; o = t / BK + i * (NT / BK); }
-        S[kk * lds + o] = r[i];
+        S[kk * (O + 1) + o] = r[i];
     }
 }
 
@@ -56,10 +54,13 @@ __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int& 
     tn = in / gm;
 }
 
-template <bool TA, bool TB, bool BETA0>
+// A BM x BN tile (BM, BN in {32, 64, 128}); the four waves split it 2 x 2,
+// each wave (BM/2) x (BN/2) as QI x QJ blocks of 16 x 16.
+template <int BM, int BN, bool TA, bool TB, bool BETA0>
 __global__ __launch_bounds__(NT) void sgemm_mfma_kernel(i64 M, i64 N, i64 K, float alpha, const float* __restrict__ A,
                                                         i64 lda, const float* __restrict__ B, i64 ldb, float beta,
                                                         float* __restrict__ C, i64 ldc, int tiles_m, int tiles_n) {
+    constexpr int LDS_A = BM + 1, LDS_B = BN + 1, QI = BM / 32, QJ = BN / 32;
     __shared__ float As[2][BK * LDS_A];
     __shared__ float Bs[2][BK * LDS_B];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -69,44 +70,44 @@ __global__ __launch_bounds__(NT) void sgemm_mfma_kernel(i64 M, i64 N, i64 K, flo
     const i64 m0 = i64(tm) * BM, n0 = i64(tn) * BN;
     constexpr bool A_OC = !TA, B_OC = TB;
 
-    f4v acc[4][4];
+    f4v acc[QI][QJ];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < QI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < QJ; ++j) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
 
-    float ra[LOADS], rb[LOADS];
+    float ra[BM * BK / NT], rb[BN * BK / NT];
     const int nslab = int((K + BK - 1) / BK);
-    load_slab<A_OC>(ra, A, lda, m0, 0, M, K, t);
-    load_slab<B_OC>(rb, B, ldb, n0, 0, N, K, t);
-    store_slab<A_OC>(As[0], LDS_A, ra, t);
-    store_slab<B_OC>(Bs[0], LDS_B, rb, t);
+    load_slab<A_OC, BM>(ra, A, lda, m0, 0, M, K, t);
+    load_slab<B_OC, BN>(rb, B, ldb, n0, 0, N, K, t);
+    store_slab<A_OC, BM>(As[0], ra, t);
+    store_slab<B_OC, BN>(Bs[0], rb, t);
     __syncthreads();
     const int lrow = lane >> 4, lcol = lane & 15;
     for (int s = 0; s < nslab; ++s) {
         const int cur = s & 1;
         if (s + 1 < nslab) {
-            load_slab<A_OC>(ra, A, lda, m0, i64(s + 1) * BK, M, K, t);
-            load_slab<B_OC>(rb, B, ldb, n0, i64(s + 1) * BK, N, K, t);
+            load_slab<A_OC, BM>(ra, A, lda, m0, i64(s + 1) * BK, M, K, t);
+            load_slab<B_OC, BN>(rb, B, ldb, n0, i64(s + 1) * BK, N, K, t);
         }
         const float* as = As[cur];
         const float* bs = Bs[cur];
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
             const int kr = ks * 4 + lrow;
-            float a[4], b[4];
+            float a[QI], b[QJ];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = as[kr * LDS_A + wm * 64 + i * 16 + lcol];
+            for (int i = 0; i < QI; ++i) a[i] = as[kr * LDS_A + wm * (BM / 2) + i * 16 + lcol];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = bs[kr * LDS_B + wn * 64 + j * 16 + lcol];
+            for (int j = 0; j < QJ; ++j) b[j] = bs[kr * LDS_B + wn * (BN / 2) + j * 16 + lcol];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < QI; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < QJ; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         if (s + 1 < nslab) {
-            store_slab<A_OC>(As[cur ^ 1], LDS_A, ra, t);
-            store_slab<B_OC>(Bs[cur ^ 1], LDS_B, rb, t);
+            store_slab<A_OC, BM>(As[cur ^ 1], ra, t);
+            store_slab<B_OC, BN>(Bs[cur ^ 1], rb, t);
         }
         __syncthreads();
     }
@@ -114,26 +115,26 @@ __global__ __launch_bounds__(NT) void sgemm_mfma_kernel(i64 M, i64 N, i64 K, flo
     // element outside C reads the nearest element inside it and is not
     // stored): a load behind a bounds branch is waited for before the next
     // one is issued, one memory round trip per element.
-    float cv[4][4][4];
+    float cv[QI][QJ][4];
     if (!BETA0) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < QI; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const i64 col = min(n0 + wn * 64 + j * 16 + lcol, N - 1);
+            for (int j = 0; j < QJ; ++j) {
+                const i64 col = min(n0 + wn * (BN / 2) + j * 16 + lcol, N - 1);
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    cv[i][j][r] = C[col * ldc + min(m0 + wm * 64 + i * 16 + lrow * 4 + r, M - 1)];
+                    cv[i][j][r] = C[col * ldc + min(m0 + wm * (BM / 2) + i * 16 + lrow * 4 + r, M - 1)];
             }
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < QI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const i64 col = n0 + wn * 64 + j * 16 + lcol;
+        for (int j = 0; j < QJ; ++j) {
+            const i64 col = n0 + wn * (BN / 2) + j * 16 + lcol;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const i64 row = m0 + wm * 64 + i * 16 + lrow * 4 + r;
+                const i64 row = m0 + wm * (BM / 2) + i * 16 + lrow * 4 + r;
                 const float v = alpha * acc[i][j][r];
                 if (row < M && col < N) C[col * ldc + row] = BETA0 ? v : __builtin_fmaf(beta, cv[i][j][r], v);
             }
@@ -149,17 +150,30 @@ __global__ void sscale_kernel(i64 m, i64 n, float beta, float* __restrict__ C, i
     }
 }
 
-template <bool TA, bool TB>
-void launch_variant(hipStream_t s, i64 m, i64 n, i64 k, float alpha, const float* A, i64 lda, const float* B, i64 ldb,
-                    float beta, float* C, i64 ldc) {
+template <int BM, int BN, bool TA, bool TB>
+void launch_tiles(hipStream_t s, i64 m, i64 n, i64 k, float alpha, const float* A, i64 lda, const float* B, i64 ldb,
+                  float beta, float* C, i64 ldc) {
     const int tiles_m = int(cdiv(m, BM)), tiles_n = int(cdiv(n, BN));
     dim3 grid(tiles_m * tiles_n), block(NT);
     if (beta == 0.0f)
-        hipLaunchKernelGGL((sgemm_mfma_kernel<TA, TB, true>), grid, block, 0, s, m, n, k, alpha, A, lda, B, ldb, beta,
-                           C, ldc, tiles_m, tiles_n);
+        hipLaunchKernelGGL((sgemm_mfma_kernel<BM, BN, TA, TB, true>), grid, block, 0, s, m, n, k, alpha, A, lda, B,
+                           ldb, beta, C, ldc, tiles_m, tiles_n);
     else
-        hipLaunchKernelGGL((sgemm_mfma_kernel<TA, TB, false>), grid, block, 0, s, m, n, k, alpha, A, lda, B, ldb,
-                           beta, C, ldc, tiles_m, tiles_n);
+        hipLaunchKernelGGL((sgemm_mfma_kernel<BM, BN, TA, TB, false>), grid, block, 0, s, m, n, k, alpha, A, lda, B,
+                           ldb, beta, C, ldc, tiles_m, tiles_n);
+}
+
+// Tile shape per product, as for the FP64 GEMM: 128 x 128 where that grid has
+// at least 1.75 workgroups per CU, else 64 x 64 where its grid has at least
+// 1.5, else 32 x 32 (the mixed LU's panel updates, look-ahead block and TRSM
+// updates are products of 32..256 rows).
+template <bool TA, bool TB>
+void launch_variant(hipStream_t s, i64 m, i64 n, i64 k, float alpha, const float* A, i64 lda, const float* B, i64 ldb,
+                    float beta, float* C, i64 ldc) {
+    constexpr i64 NCU = 256;
+    if (cdiv(m, 128) * cdiv(n, 128) >= NCU * 7 / 4) return launch_tiles<128, 128, TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (cdiv(m, 64) * cdiv(n, 64) >= NCU * 3 / 2) return launch_tiles<64, 64, TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    return launch_tiles<32, 32, TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 }  // namespace
