@@ -219,7 +219,7 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
         }
         return;
     }
-    // edge tiles (and 128 x 128): all loads of C (issued without branches, an
+    // edge tiles: all loads of C (issued without branches, an
     // element outside C reading the nearest one inside and not stored)
     // before the first store
     double cv[QM][QN];
@@ -445,8 +445,8 @@ void launch_m16(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* 
 // LDS, two workgroups per CU); 32 x 32 where the 64 x 64 grid has fewer than
 // about 1.5 workgroups per CU (the LU's look-ahead block, its panel updates
 // and the TRSM updates: products of 32..256 rows).  A side no longer than 32
-// takes a 32-wide tile.  OXIBLAS_DGEMM_TILE (m16, 64x64, 32x32, 128x128)
-// fixes the choice (tuning); bm = bn = 16 stands for the 16x16x4 kernel.
+// takes a 32-wide tile.  OXIBLAS_DGEMM_TILE (m16, 64x64, 32x32) fixes the
+// choice (tuning); bm = bn = 16 stands for the 16x16x4 kernel.
 constexpr int NCU = 256;
 void choose_tile(i64 m, i64 n, int& bm, int& bn) {
     static int forced = -1;
@@ -454,7 +454,7 @@ void choose_tile(i64 m, i64 n, int& bm, int& bn) {
         forced = 0;
         if (const char* e = getenv("OXIBLAS_DGEMM_TILE")) {
             int a = 0, b = 0;
-            if (sscanf(e, "%dx%d", &a, &b) == 2 && a == b && (a == 32 || a == 64 || a == 128)) forced = a;
+            if (sscanf(e, "%dx%d", &a, &b) == 2 && a == b && (a == 32 || a == 64)) forced = a;
             if (!strcmp(e, "m16")) forced = 16;
         }
     }
@@ -465,16 +465,18 @@ void choose_tile(i64 m, i64 n, int& bm, int& bn) {
     if (cdiv(m, bm) * cdiv(n, bn) < i64(NCU) * 3 / 2) bm = bn = 32;
 }
 
+// false when no kernel takes the chosen tile shape (nothing launched)
 template <bool TA, bool TB>
-void launch_variant(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda, const double* B,
+bool launch_variant(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda, const double* B,
                     i64 ldb, double beta, double* C, i64 ldc) {
     int bm, bn;
     choose_tile(m, n, bm, bn);
-    if (bm == 16) return launch_m16<TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (bm == 16) { launch_m16<TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc); return true; }
 #define OXI_TILE(X, Y)                                                                                              \
-    if (bm == X && bn == Y) return launch_tiles<X, Y, TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    OXI_TILE(64, 64) OXI_TILE(32, 32) OXI_TILE(64, 32) OXI_TILE(32, 64) OXI_TILE(128, 128)
+    if (bm == X && bn == Y) { launch_tiles<X, Y, TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc); return true; }
+    OXI_TILE(64, 64) OXI_TILE(32, 32) OXI_TILE(64, 32) OXI_TILE(32, 64)
 #undef OXI_TILE
+    return false;
 }
 
 }  // namespace
@@ -511,10 +513,12 @@ int oxi_launch_dgemm(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n
     if (cdiv(m, 128) * cdiv(n, 128) > (i64(1) << 31) - 1)
         return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "dgemm: %lld x %lld has too many tiles", (long long)m,
                              (long long)n);
-    if (!ta && !tb) launch_variant<false, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    else if (!ta && tb) launch_variant<false, true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    else if (ta && !tb) launch_variant<true, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    else launch_variant<true, true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    bool launched;
+    if (!ta && !tb) launched = launch_variant<false, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (!ta && tb) launched = launch_variant<false, true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (ta && !tb) launched = launch_variant<true, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else launched = launch_variant<true, true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (!launched) return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "dgemm: no kernel for the chosen tile shape");
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "dgemm launch: %s", hipGetErrorString(e));
     return OXI_OK;

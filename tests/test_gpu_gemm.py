@@ -179,12 +179,14 @@ def test_full_size_shapes(gpu, shape):
 
 # Shapes that select each GEMM kernel (the tile rule: 128 x 128 on the 16x16x4
 # kernel when that grid has >= 448 workgroups, 64 x 64 when its grid has
-# >= 384, else 32 x 32), once with every tile inside C and K a multiple of the
-# slab depth (the unchecked interior path) and once with edges in M, N and K.
+# >= 384, else 32 x 32; a side of at most 32 takes a 32-wide tile), once with
+# every tile inside C and K a multiple of the slab depth (the unchecked
+# interior path) and once with edges in M, N and K.
 TILE_SHAPES = [
     ("128x128 interior", 2048, 4096, 256), ("128x128 edges", 2100, 4100, 300),
     ("64x64 interior", 1600, 1088, 96), ("64x64 edges", 1630, 1100, 70),
     ("32x32 interior", 640, 320, 64), ("32x32 edges", 650, 333, 37),
+    ("32x64 (32 rows)", 32, 30000, 40), ("64x32 (20 columns)", 30000, 20, 40),
 ]
 
 
