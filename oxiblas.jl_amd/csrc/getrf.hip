@@ -67,14 +67,43 @@ __global__ __launch_bounds__(256) void transpose_kernel(const T* __restrict__ in
 template <typename T>
 __global__ __launch_bounds__(256) void transpose_square_kernel(T* __restrict__ A, i64 lda, i64 n, int tiles) {
     __shared__ T t1[TT][TT + 1], t2[TT][TT + 1];
-    // linear block id -> (bi, bj) with bi <= bj
-    i64 id = blockIdx.x;
-    int bi = 0;
-    i64 rowlen = tiles;
-    while (id >= rowlen) { id -= rowlen; ++bi; --rowlen; }
-    const int bj = bi + int(id);
+    // linear block id -> (bi, bj) with bi <= bj: row bi of the upper triangle
+    // starts at S(bi) = bi*tiles - bi*(bi-1)/2; the root of S(b) = id, then
+    // exact integer corrections
+    const i64 id = blockIdx.x, tl = tiles;
+    auto start = [&](i64 b) { return b * tl - b * (b - 1) / 2; };
+    const double q = double(2 * tl + 1);
+    i64 bi = i64((q - sqrt(q * q - 8.0 * double(id))) * 0.5);
+    if (bi < 0) bi = 0;
+    if (bi > tl - 1) bi = tl - 1;
+    while (bi > 0 && start(bi) > id) --bi;
+    while (bi + 1 < tl && start(bi + 1) <= id) ++bi;
+    const int bj = int(bi + (id - start(bi)));
     const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;
-    const i64 r0 = i64(bi) * TT, c0 = i64(bj) * TT;
+    const i64 r0 = bi * TT, c0 = i64(bj) * TT;
+    if (c0 + TT <= n) {
+        // both tiles inside A (bi <= bj): every load issued before any wait
+        T a1[TT / 8], a2[TT / 8];
+#pragma unroll
+        for (int k = 0; k < TT / 8; ++k) {
+            const int y = ty + 8 * k;
+            a1[k] = A[(r0 + tx) + (c0 + y) * lda];
+            a2[k] = A[(c0 + tx) + (r0 + y) * lda];
+        }
+#pragma unroll
+        for (int k = 0; k < TT / 8; ++k) {
+            t1[ty + 8 * k][tx] = a1[k];
+            t2[ty + 8 * k][tx] = a2[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < TT / 8; ++k) {
+            const int y = ty + 8 * k;
+            A[(c0 + tx) + (r0 + y) * lda] = t1[tx][y];
+            if (bi != bj) A[(r0 + tx) + (c0 + y) * lda] = t2[tx][y];
+        }
+        return;
+    }
     for (int y = ty; y < TT; y += 8) {
         i64 r = r0 + tx, c = c0 + y;
         if (r < n && c < n) t1[y][tx] = A[r + c * lda];       // tile (bi, bj): t1[col][row]
