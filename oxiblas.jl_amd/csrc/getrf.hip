@@ -65,12 +65,13 @@ __global__ __launch_bounds__(256) void transpose_kernel(const T* __restrict__ in
 // In-place transpose of the leading n x n block: tile (bi, bj) with bi < bj is
 // swapped with tile (bj, bi); diagonal tiles are transposed in LDS.
 template <typename T>
-__global__ __launch_bounds__(256) void transpose_square_kernel(T* __restrict__ A, i64 lda, i64 n, int tiles) {
+__global__ __launch_bounds__(256) void transpose_square_kernel(T* __restrict__ A, i64 lda, i64 n, int tiles,
+                                                               i64 id0) {
     __shared__ T t1[TT][TT + 1], t2[TT][TT + 1];
     // linear block id -> (bi, bj) with bi <= bj: row bi of the upper triangle
     // starts at S(bi) = bi*tiles - bi*(bi-1)/2; the root of S(b) = id, then
     // exact integer corrections
-    const i64 id = blockIdx.x, tl = tiles;
+    const i64 id = id0 + blockIdx.x, tl = tiles;
     auto start = [&](i64 b) { return b * tl - b * (b - 1) / 2; };
     const double q = double(2 * tl + 1);
     i64 bi = i64((q - sqrt(q * q - 8.0 * double(id))) * 0.5);
@@ -524,14 +525,40 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, s, g.flags, 1);
     OXI_TRY(launch_check(h, "zero"));
 
-    // A -> W
+    static const bool lookahead = !getenv("OXIBLAS_NO_LOOKAHEAD");
+    GetrfCtx<T> gs = g;               // panel context on the side stream
+    gs.s = h->side;
+    gs.pairs = (int*)(base + off_pairs2);
+    bool panel_on_side = false;
+
+    // A -> W.  The first panel reads only W's first nb0 rows, written by the
+    // tile pairs of the first k0 tile rows: for a square matrix (with the
+    // look-ahead) those pairs go first, the first panel then starts on the
+    // side stream, and the rest of the transpose runs beside it.
     const int tiles = int(cdiv(n, TT));
     const i64 nblk = i64(tiles) * (tiles + 1) / 2;
-    if (square)
-        hipLaunchKernelGGL(transpose_square_kernel<T>, dim3(unsigned(nblk)), dim3(256), 0, s, A, lda, n, tiles);
-    else
+    const i64 nb0 = std::min<i64>(nb_at(0), q);
+    const bool early_panel = square && lookahead && q > 0;
+    if (square) {
+        const i64 k0 = early_panel ? std::min<i64>(cdiv(nb0, TT), tiles) : tiles;
+        const i64 ids1 = k0 * tiles - k0 * (k0 - 1) / 2;      // the pairs of tile rows < k0
+        hipLaunchKernelGGL(transpose_square_kernel<T>, dim3(unsigned(ids1)), dim3(256), 0, s, A, lda, n, tiles,
+                           i64(0));
+        OXI_TRY(launch_check(h, "transpose"));
+        if (early_panel) {
+            OXI_HIP(h, hipEventRecord(h->ev_a, s));
+            OXI_HIP(h, hipStreamWaitEvent(h->side, h->ev_a, 0));
+            OXI_TRY(panel_top(gs, 0, nb0));
+            OXI_HIP(h, hipEventRecord(h->ev_b, h->side));
+            panel_on_side = true;
+        }
+        if (ids1 < nblk)
+            hipLaunchKernelGGL(transpose_square_kernel<T>, dim3(unsigned(nblk - ids1)), dim3(256), 0, s, A, lda, n,
+                               tiles, ids1);
+    } else {
         hipLaunchKernelGGL(transpose_kernel<T>, dim3(unsigned(cdiv(m, TT)), unsigned(cdiv(n, TT))), dim3(256), 0, s, A,
                            lda, g.Wm, ldw, m, n);
+    }
     OXI_TRY(launch_check(h, "transpose"));
 
     // The factorization's main sequence runs on the bulk stream (kept off a
@@ -550,14 +577,9 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     // are updated first and panel j+1 is factored on the side stream while
     // the main stream updates the rest of the trailing matrix.  The
     // interchanges of each panel on the columns to its left touch the L21
-    // block the concurrent update reads, so they are applied at the end, in
-    // panel order (the same per-column order as applying them at once).
-    static const bool lookahead = !getenv("OXIBLAS_NO_LOOKAHEAD");
-    GetrfCtx<T> gs = g;               // panel context on the side stream
-    gs.s = h->side;
-    gs.pairs = (int*)(base + off_pairs2);
-    bool panel_on_side = false;
-    OXI_TRY(panel_top(g, 0, std::min<i64>(nb_at(0), q)));
+    // block that the update running beside the panel reads, so the main
+    // stream applies them after that update, at the end of the step.
+    if (!panel_on_side) OXI_TRY(panel_top(g, 0, nb0));
     for (i64 j0 = 0, jn = 0; j0 < q; j0 = jn) {
         const i64 jb = std::min<i64>(nb_at(j0), q - j0);
         const i64 j1 = j0 + jb;
@@ -617,7 +639,7 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
 
     // W -> A
     if (square)
-        hipLaunchKernelGGL(transpose_square_kernel<T>, dim3(unsigned(nblk)), dim3(256), 0, s, A, lda, n, tiles);
+        hipLaunchKernelGGL(transpose_square_kernel<T>, dim3(unsigned(nblk)), dim3(256), 0, s, A, lda, n, tiles, i64(0));
     else
         hipLaunchKernelGGL(transpose_kernel<T>, dim3(unsigned(cdiv(n, TT)), unsigned(cdiv(m, TT))), dim3(256), 0, s,
                            g.Wm, ldw, A, lda, n, m);
