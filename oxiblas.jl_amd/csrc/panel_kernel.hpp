@@ -124,24 +124,22 @@ __device__ __forceinline__ int first_lane(u64 m) { return int(__builtin_ctzll(m)
 
 // Publish one record by the 16 lanes of a row of a wave: lane L stores fields
 // L, L + 16 and L + 32 (the checksum is field 35 = lane 3's third) with
-// device-scope stores; the checksum is reduced across the row.
-template <typename F>
-__device__ __forceinline__ void publish16(double* rec, u64 tag, int L, F field) {
-    double fv[3];
-    u64 h = 0;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const int f = L + 16 * i;
-        fv[i] = f < REC_CHK ? (f == 2 ? from_bits(tag) : field(f)) : 0.0;
-        if (f < REC_CHK) h ^= rec_term(bits(fv[i]), tag, f);
-    }
+// device-scope stores; the checksum is reduced across the row.  Fields: key
+// k0, row k1, the tag, then the w values row[0 .. 31] (LDS) when has, else
+// zeros.  Branch-free: every lane reads its three row values (clamped into
+// the row) and selects.
+__device__ __forceinline__ void publish16(double* rec, u64 tag, int L, const double* row, bool has, double k0,
+                                          double k1) {
+    const double r0 = row[L < 3 ? 0 : L - 3], r1 = row[L + 13], r2 = row[L < 3 ? L + 29 : OXI_LEAF_W - 1];
+    const double f0 = L == 0 ? k0 : L == 1 ? k1 : L == 2 ? from_bits(tag) : has ? r0 : 0.0;
+    const double f1 = has ? r1 : 0.0;
+    const double f2 = has ? r2 : 0.0;
+    u64 h = rec_term(bits(f0), tag, L) ^ rec_term(bits(f1), tag, L + 16);
+    if (L < 3) h ^= rec_term(bits(f2), tag, L + 32);
     h = row_xor16(h);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const int f = L + 16 * i;
-        if (f < REC_CHK) put_agent(&rec[f], fv[i]);
-        else if (f == REC_CHK) put_agent(&rec[f], from_bits(h));
-    }
+    put_agent(&rec[L], f0);
+    put_agent(&rec[L + 16], f1);
+    if (L < 4) put_agent(&rec[L + 32], L < 3 ? f2 : from_bits(h));
 }
 
 // ---------------------------------------------------------------------------
@@ -173,6 +171,7 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
     constexpr int PASS = NWV * RPW;
     constexpr int NPASS_MAX = (REC_SLOTS + PASS - 1) / PASS;
     constexpr unsigned SPIN_LIMIT = 1u << 22;
+    constexpr int STAGGER_ROUND = 24;
     __shared__ double wkey[2][NWV];
     __shared__ long long wrow[2][NWV];
     // LDS copies of records keep field f at offset f + 1 of a 40-double slot,
@@ -192,6 +191,22 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
     // a wait of an earlier leaf timed out: the factorization is already
     // reported as failed, leave at once
     if (__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    // polling slots of the record exchange (see gather); vmask is first
+    // used after the barrier of the first select_publish
+    __shared__ u64 vmask[2][2];   // validated records of the column, by column parity
+    if (t < 4) vmask[t >> 1][t & 1] = 0;
+    const int P1 = a.P + 1;
+    const u64 full0 = P1 >= 64 ? ~0ull : (1ull << P1) - 1;
+    const u64 full1 = P1 <= 64 ? 0ull : P1 >= 128 ? ~0ull : (1ull << (P1 - 64)) - 1;
+    const int slot = wave * RPW + (lane >> 2);
+    const int copies = P1 <= PASS ? PASS / P1 : 1;
+    int qs[NPASS_MAX];   // records of this lane's slot; P1 = none
+#pragma unroll
+    for (int ps = 0; ps < NPASS_MAX; ++ps)
+        qs[ps] = P1 <= PASS ? (ps == 0 && slot < copies * P1 ? slot % P1 : P1) : min(ps * PASS + slot, P1);
+    // a wave starts polling after this many s_sleep(1): its first slot's copy
+    // times the share of one poll round (about STAGGER_ROUND sleeps) per copy
+    const int stagger = copies > 1 ? min(wave * RPW / P1, copies - 1) * STAGGER_ROUND / copies : 0;
 
     T v[RPT][W];
     T* Wm = (T*)a.Wm;
@@ -283,67 +298,83 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
         double* drec = a.pub + (size_t(w) * a.P + jn) * OXI_PUB_STRIDE;
         if (wave == 0 && lane < 16) {
             const bool has = lrow < a.mrows;
-            publish16(rec, tag, lane, [&](int f) {
-                return f == 0 ? (has ? lkey : -2.0)
-                     : f == 1 ? __longlong_as_double(has ? lrow : (long long)(1ll << 62))
-                     : (has ? wvals[parn][lw][f - 3] : 0.0);
-            });
+            publish16(rec, tag, lane, wvals[parn][lw], has, has ? lkey : -2.0,
+                      __longlong_as_double(has ? lrow : (long long)(1ll << 62)));
         }
         if (p == 0 && wave == NWV - 1 && lane < 16)
-            publish16(drec, tag, lane, [&](int f) { return f < 3 ? 0.0 : wvals[parn][NWV][f - 3]; });
+            publish16(drec, tag, lane, wvals[parn][NWV], true, 0.0, 0.0);
     };
 
-    // ---- the records of column jn into rec_sh (P > 1); a barrier either
-    // way, so that no wave overwrites the buffers of column jn - 1 while
-    // another still reads them.  Returns false on timeout.
+    // ---- the records of column jn into rec_sh (P > 1), then a barrier, so
+    // that no wave overwrites the buffers of column jn - 1 while another
+    // still reads them.  Returns false on timeout.
+    //
+    // Record q (q < P candidates, q == P position jn) is polled by the 4 lanes
+    // of a slot (lane sub holds fields sub, sub+4, ..., sub+32).  With P + 1
+    // <= PASS records every record has PASS / (P + 1) slots, in different
+    // waves; the waves poll on their own (no barrier per round) and start at
+    // staggered times, so a record that lands is seen within a fraction of
+    // one load round trip.  A validated record goes to rec_sh and its bit to
+    // vmask (LDS); every wave stops once all bits are set.  vmask of the
+    // other parity is cleared after the barrier: its readers (column jn - 1)
+    // passed the barrier of select_publish before this one.
     auto gather = [&](int jn) __attribute__((always_inline)) -> bool {
         if (!global_x) { __syncthreads(); return true; }
         const int parn = jn & 1;
         const u64 tag = a.tag0 + u64(a.c0 + jn);
         const double* drec = a.pub + (size_t(w) * a.P + jn) * OXI_PUB_STRIDE;
-        // record q (q < P candidates, q == P position jn) by lanes 4i..4i+3 of
-        // one wave; lane sub holds fields sub, sub+4, ..., sub+32
         double* rs = rec_sh[parn];
-        const int P1 = a.P + 1, npass = (P1 + PASS - 1) / PASS;
+        u64* vm = vmask[parn];
         const int sub = lane & 3;
-        bool okp[NPASS_MAX];
-#pragma unroll
-        for (int ps = 0; ps < NPASS_MAX; ++ps) okp[ps] = false;
+        for (int i = 0; i < stagger; ++i) __builtin_amdgcn_s_sleep(1);
         unsigned spins = 0;
         while (true) {
-            bool all_ok = true;
+            const u64 m0 = __hip_atomic_load(&vm[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const u64 m1 = __hip_atomic_load(&vm[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (__builtin_amdgcn_readfirstlane(int(m0 == full0 && m1 == full1))) break;
 #pragma unroll
             for (int ps = 0; ps < NPASS_MAX; ++ps) {
-                const int q = ps * PASS + wave * RPW + (lane >> 2);
-                if (ps < npass && q < P1 && !okp[ps]) {
+                const int q = qs[ps];
+                const bool have = q >= P1 || (((q < 64 ? m0 >> q : m1 >> (q - 64)) & 1) != 0);
+                if (!have) {
                     const double* src = q < a.P ? a.pub + (size_t(jn) * a.P + q) * OXI_PUB_STRIDE : drec;
                     double fv[REC_N / 4];
                     load9_agent(src + sub, fv);
-                    u64 hx = 0;
-#pragma unroll
-                    for (int i = 0; i < REC_N / 4; ++i) {
-                        const int f = sub + 4 * i;
-                        if (f < REC_CHK) hx ^= rec_term(bits(fv[i]), tag, f);
-                    }
-                    hx = quad_xor(hx);
-                    // the checksum sits in quad lane 3 (field 35), the tag in quad lane 2 (field 2)
-                    const u64 chk = quad_bcast<3>(bits(fv[REC_N / 4 - 1]));
+                    // the tag sits in quad lane 2 (field 2): a record of an
+                    // earlier column or call is passed over without hashing
                     const u64 tg = quad_bcast<2>(bits(fv[0]));
-                    if (hx == chk && tg == tag) {
-                        okp[ps] = true;
+                    if (tg == tag) {
+                        u64 hx = 0;
 #pragma unroll
-                        for (int i = 0; i < REC_N / 4; ++i) rs[q * RS_N + 1 + sub + 4 * i] = fv[i];
+                        for (int i = 0; i < REC_N / 4; ++i) {
+                            const int f = sub + 4 * i;
+                            if (f < REC_CHK) hx ^= rec_term(bits(fv[i]), tag, f);
+                        }
+                        hx = quad_xor(hx);
+                        // the checksum sits in quad lane 3 (field 35)
+                        const u64 chk = quad_bcast<3>(bits(fv[REC_N / 4 - 1]));
+                        if (hx == chk) {
+#pragma unroll
+                            for (int i = 0; i < REC_N / 4; ++i) rs[q * RS_N + 1 + sub + 4 * i] = fv[i];
+                            if (sub == 0)
+                                __hip_atomic_fetch_or(&vm[q >> 6], 1ull << (q & 63), __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
                     }
                 }
-                if (ps < npass && q < P1 && !okp[ps]) all_ok = false;
             }
-            if (__syncthreads_and(all_ok)) return true;
-            if (++spins > SPIN_LIMIT) {   // uniform: every thread leaves
-                if (t == 0) __hip_atomic_store(a.flags, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                return false;
-            }
+            if (++spins > SPIN_LIMIT) break;   // uniform per wave
             __builtin_amdgcn_s_sleep(1);
         }
+        __syncthreads();
+        const bool ok = __hip_atomic_load(&vm[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == full0 &&
+                        __hip_atomic_load(&vm[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == full1;
+        if (!ok) {   // the same in every thread: no wave writes vmask after the barrier
+            if (t == 0) __hip_atomic_store(a.flags, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        if (t < 2) vmask[parn ^ 1][t] = 0;
+        return true;
     };
 
     // column 0: the registers as loaded
@@ -388,14 +419,16 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
         }
 
         // ---- interchange (the registers: columns >= j), store position j,
-        // scale, store the multipliers
+        // scale, store the multipliers.  Only position piv takes a row
+        // (position j's, dp): position j's registers are not read again
+        // (candidates, multipliers and later interchanges all lie below j),
+        // its final row is the pivot row, stored from up.
         if (p == 0 && t == 0) a.ipiv[a.c0 + j] = a.c0 + piv + 1;
 #pragma unroll
         for (int s = 0; s < RPT; ++s) {
-            if (piv != j && (lr[s] == j || lr[s] == piv)) {
-                const double* src = lr[s] == j ? up : dp;
+            if (piv != j && lr[s] == piv) {
 #pragma unroll
-                for (int c = 0; c < W; ++c) v[s][c] = T(src[c]);
+                for (int c = 0; c < W; ++c) v[s][c] = T(dp[c]);
             }
         }
         // position j now holds the pivot row, final in columns j .. w-1
