@@ -171,7 +171,6 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
     constexpr int PASS = NWV * RPW;
     constexpr int NPASS_MAX = (REC_SLOTS + PASS - 1) / PASS;
     constexpr unsigned SPIN_LIMIT = 1u << 22;
-    constexpr int STAGGER_ROUND = 24;
     __shared__ double wkey[2][NWV];
     __shared__ long long wrow[2][NWV];
     // LDS copies of records keep field f at offset f + 1 of a 40-double slot,
@@ -199,14 +198,9 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
     const u64 full0 = P1 >= 64 ? ~0ull : (1ull << P1) - 1;
     const u64 full1 = P1 <= 64 ? 0ull : P1 >= 128 ? ~0ull : (1ull << (P1 - 64)) - 1;
     const int slot = wave * RPW + (lane >> 2);
-    const int copies = P1 <= PASS ? PASS / P1 : 1;
     int qs[NPASS_MAX];   // records of this lane's slot; P1 = none
 #pragma unroll
-    for (int ps = 0; ps < NPASS_MAX; ++ps)
-        qs[ps] = P1 <= PASS ? (ps == 0 && slot < copies * P1 ? slot % P1 : P1) : min(ps * PASS + slot, P1);
-    // a wave starts polling after this many s_sleep(1): its first slot's copy
-    // times the share of one poll round (about STAGGER_ROUND sleeps) per copy
-    const int stagger = copies > 1 ? min(wave * RPW / P1, copies - 1) * STAGGER_ROUND / copies : 0;
+    for (int ps = 0; ps < NPASS_MAX; ++ps) qs[ps] = min(ps * PASS + slot, P1);
 
     T v[RPT][W];
     T* Wm = (T*)a.Wm;
@@ -310,14 +304,15 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
     // still reads them.  Returns false on timeout.
     //
     // Record q (q < P candidates, q == P position jn) is polled by the 4 lanes
-    // of a slot (lane sub holds fields sub, sub+4, ..., sub+32).  With P + 1
-    // <= PASS records every record has PASS / (P + 1) slots, in different
-    // waves; the waves poll on their own (no barrier per round) and start at
-    // staggered times, so a record that lands is seen within a fraction of
-    // one load round trip.  A validated record goes to rec_sh and its bit to
-    // vmask (LDS); every wave stops once all bits are set.  vmask of the
-    // other parity is cleared after the barrier: its readers (column jn - 1)
-    // passed the barrier of select_publish before this one.
+    // of slot q % PASS (lane sub holds fields sub, sub+4, ..., sub+32).  The
+    // waves poll on their own, without a barrier per round, and a record
+    // whose tag is not yet the column's is passed over without hashing.  A
+    // validated record goes to rec_sh and its bit to vmask (LDS); every wave
+    // stops once all bits are set.  (Polling each record from several waves
+    // at staggered times was slower: a wave cannot leave a poll in flight,
+    // so the barrier then waits for the late copies' round trips.)  vmask of
+    // the other parity is cleared after the barrier: its readers (column
+    // jn - 1) passed the barrier of select_publish before this one.
     auto gather = [&](int jn) __attribute__((always_inline)) -> bool {
         if (!global_x) { __syncthreads(); return true; }
         const int parn = jn & 1;
@@ -326,7 +321,6 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
         double* rs = rec_sh[parn];
         u64* vm = vmask[parn];
         const int sub = lane & 3;
-        for (int i = 0; i < stagger; ++i) __builtin_amdgcn_s_sleep(1);
         unsigned spins = 0;
         while (true) {
             const u64 m0 = __hip_atomic_load(&vm[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
