@@ -547,21 +547,22 @@ __global__ __launch_bounds__(64) void leaf_swaps_kernel(T* Wm, i64 ldw, i64 c0, 
     // loads are issued before any store and without branches: a slot that
     // does not move reads its own position, a lane outside the columns
     // reads column c0 (both in bounds), and only moving slots are stored.
+    // Lane s computes slot s's row offsets once; the loops only read them
+    // back (two readlanes and an add per slot and access).
     const i64 psrc = (i64)__shfl((long long)pos, src);
     const bool moves = pos >= 0 && src != lane;
     const u64 mv = __ballot(moves);
+    const i64 offs = (c0 + (moves ? psrc : 0)) * ldw;
+    const i64 offd = (c0 + (moves ? pos : 0)) * ldw;
     const i64 xr = ok ? x : c0;
     T v[64];
 #pragma unroll
-    for (int s = 0; s < 64; ++s) {
-        const i64 from = (i64)readlane64(u64((mv >> s) & 1 ? psrc : 0), s);
-        v[s] = Wm[(c0 + from) * ldw + xr];
-    }
+    for (int s = 0; s < 64; ++s) v[s] = Wm[(i64)readlane64(u64(offs), s) + xr];
 #pragma unroll
     for (int s = 0; s < 64; ++s)
         if ((mv >> s) & 1) {
-            const i64 to = (i64)readlane64(u64(pos), s);
-            if (ok) Wm[(c0 + to) * ldw + x] = v[s];
+            const i64 to = (i64)readlane64(u64(offd), s);
+            if (ok) Wm[to + x] = v[s];
         }
 }
 
