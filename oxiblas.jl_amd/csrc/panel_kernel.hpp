@@ -187,9 +187,6 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
     const i64 R = i64(LT) * RPT;
     const int w = a.w;
     const bool global_x = a.P > 1;
-    // a wait of an earlier leaf timed out: the factorization is already
-    // reported as failed, leave at once
-    if (__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     // polling slots of the record exchange (see gather); vmask is first
     // used after the barrier of the first select_publish
     __shared__ u64 vmask[2][2];   // validated records of the column, by column parity
@@ -214,10 +211,16 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
 #pragma unroll
         for (int c = 0; c < W; ++c) v[s][c] = (ok && c < w) ? wrow_of[s][c] : T(0);
     }
+    // a wait of an earlier leaf timed out: the factorization is already
+    // reported as failed, leave at once.  The flag is read with the rows (one
+    // memory round trip for both; the rows lie inside the panel either way)
+    // and nothing has been stored before the test.
+    const int failed = __hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // the rows have arrived before the loop: otherwise the first use inside a
     // conditional block leaves a wait on all vector memory operations (the
     // stores of the previous column included) at the top of every column
     __builtin_amdgcn_s_waitcnt(0);
+    if (failed != 0) return;
 
     // ---- candidates of column jn and their publication.  x[s]: row s's
     // value in column jn.  pend: the registers still await the update of
@@ -520,10 +523,12 @@ template <typename T>
 __global__ __launch_bounds__(64) void leaf_swaps_kernel(T* Wm, i64 ldw, i64 c0, int w, const i64* ipiv,
                                                         const int* flags, i64 x_lo, i64 x_hi) {
     const int lane = threadIdx.x;
-    // the pivots are loaded with the flag, not after its test: one memory
-    // round trip less (ipiv lies in its own allocation, always readable)
-    const i64 pi_all = lane < w ? ipiv[c0 + lane] - 1 - c0 : -1;
+    // the pivots are loaded with the flag, not after its test, and without a
+    // branch (lanes >= w read entry c0): one memory round trip for both
+    // (ipiv lies in its own allocation, always readable)
+    const i64 pv = ipiv[c0 + (lane < w ? lane : 0)];
     if (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    const i64 pi_all = lane < w ? pv - 1 - c0 : -1;
     const int b = blockIdx.x;
     if (b < w - 1) {
         const int c = b;
