@@ -478,7 +478,10 @@ __global__ __launch_bounds__(LTT) void lu_leaf_kernel(LeafArgs a) {
 // n + i is the pivot position p_i of step k_lo + i when it lies below the
 // leaf's w rows and i is its first occurrence.  pos = -1 for unused lanes.
 // The interchanges are then replayed on slot ids: afterwards lane s holds
-// the slot whose original content ends at slot s.
+// the slot whose original content ends at slot s.  Every lane follows its
+// own slot backwards through the transpositions (step i exchanges slot i
+// with slot sb_i, held by lane i), so the steps carry no cross-lane
+// dependence: one readlane and two selects each.
 // pi_all: lane i < w holds step i's pivot position relative to c0 (-1 beyond).
 __device__ __forceinline__ void leaf_slots(i64 pi_all, int w, int k_lo, int lane, i64& pos, int& src) {
     const int n = w - k_lo;
@@ -495,16 +498,16 @@ __device__ __forceinline__ void leaf_slots(i64 pi_all, int w, int k_lo, int lane
     const i64 pos_out = (i64)__shfl((long long)pi, src_l);
     const int first_out = __shfl(first, src_l);
     if (src_i >= 0 && src_i < n && pos_out >= w && first_out == src_i) pos = pos_out;
-    src = lane;
-    for (int i = 0; i < n; ++i) {
-        const i64 q = (i64)readlane64(u64(pi), i);
-        const int fi = __builtin_amdgcn_readlane(first, i);
-        const int sa = i, sb = q < w ? int(q - k_lo) : n + fi;
-        if (sa == sb) continue;
-        const int ia = __builtin_amdgcn_readlane(src, sa), ib = __builtin_amdgcn_readlane(src, sb);
-        if (lane == sa) src = ib;
-        else if (lane == sb) src = ia;
+    // the slot that step i exchanges with slot i (lane i < n)
+    const int sbv = lane < n ? (pi < w ? int(pi - k_lo) : n + first) : lane;
+    // the content that ends at slot `lane`: the transpositions of steps
+    // n-1 .. 0 applied to it in turn
+    int x = lane;
+    for (int i = n - 1; i >= 0; --i) {
+        const int sb = __builtin_amdgcn_readlane(sbv, i);
+        x = x == i ? sb : x == sb ? i : x;
     }
+    src = x;
 }
 
 // A leaf's interchanges applied where the leaf did not apply them, one wave
