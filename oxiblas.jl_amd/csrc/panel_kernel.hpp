@@ -487,8 +487,12 @@ __device__ __forceinline__ void leaf_slots(i64 pi_all, int w, int k_lo, int lane
     const int n = w - k_lo;
     const i64 pi_sh = (i64)__shfl((long long)pi_all, min(k_lo + lane, 63));
     const i64 pi = lane < n ? pi_sh : -1;
+    // unrolled to the largest leaf without a bound test (the readlanes of
+    // different steps are independent and interleave): lanes i >= n hold
+    // pi = -1, which matches only lanes >= n, whose first is not used
     int first = lane;
-    for (int i = 0; i < n; ++i) {
+#pragma unroll
+    for (int i = 0; i < OXI_LEAF_W; ++i) {
         const i64 q = (i64)readlane64(u64(pi), i);
         if (i < first && q == pi) first = i;
     }
@@ -501,9 +505,11 @@ __device__ __forceinline__ void leaf_slots(i64 pi_all, int w, int k_lo, int lane
     // the slot that step i exchanges with slot i (lane i < n)
     const int sbv = lane < n ? (pi < w ? int(pi - k_lo) : n + first) : lane;
     // the content that ends at slot `lane`: the transpositions of steps
-    // n-1 .. 0 applied to it in turn
+    // n-1 .. 0 applied to it in turn (steps i >= n exchange slot i with
+    // itself: sbv = lane there)
     int x = lane;
-    for (int i = n - 1; i >= 0; --i) {
+#pragma unroll
+    for (int i = OXI_LEAF_W - 1; i >= 0; --i) {
         const int sb = __builtin_amdgcn_readlane(sbv, i);
         x = x == i ? sb : x == sb ? i : x;
     }
