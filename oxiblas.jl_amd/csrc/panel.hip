@@ -34,7 +34,7 @@ int oxi_launch_lu_leaf(oxi_handle_t h, hipStream_t s, const LeafArgs& a, int kin
     // the leaf's interchanges on its stored multipliers and on the rest of its panel
     const unsigned nblk = unsigned(std::max(a.w - 1, 0) + cdiv(a.c0 - a.xlo, 64) + cdiv(a.xhi - a.c0 - a.w, 64));
     if (nblk > 0) {
-        hipLaunchKernelGGL(leaf_swaps_kernel<double>, dim3(nblk), dim3(64), 0, s, (double*)a.Wm, a.ldw, a.c0, a.w,
+        hipLaunchKernelGGL(leaf_swaps_kernel<double>, dim3(nblk), dim3(128), 0, s, (double*)a.Wm, a.ldw, a.c0, a.w,
                            (const i64*)a.ipiv, (const int*)a.flags, a.xlo, a.xhi);
         e = hipGetLastError();
         if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "leaf_swaps launch: %s", hipGetErrorString(e));

@@ -516,22 +516,24 @@ __device__ __forceinline__ void leaf_slots(i64 pi_all, int w, int k_lo, int lane
     src = x;
 }
 
-// A leaf's interchanges applied where the leaf did not apply them, one wave
-// per block:
-//  * its own multiplier columns c < w - 1 (one block each): column c takes
-//    the interchanges of steps c+1 .. w-1 (the multipliers of column c were
-//    stored at the positions of step c);
+// A leaf's interchanges applied where the leaf did not apply them, blocks of
+// two waves:
+//  * its own multiplier columns c < w - 1 (one block each, the first wave):
+//    column c takes the interchanges of steps c+1 .. w-1 (the multipliers of
+//    column c were stored at the positions of step c);
 //  * the other columns of the enclosing panel, [x_lo, c0) and [c0+w, x_hi),
-//    64 per block: all w interchanges.
+//    64 per block (a lane per column): all w interchanges, half of the slots
+//    by each wave.
 // Applying a leaf's interchanges to the rest of its panel at once gives the
 // same columns as the getrf2 recursion, which applies them level by level:
 // a left subtree's multipliers are not read again once its right sibling
 // has started, and the right sibling's columns receive them before any
 // update reads them.
 template <typename T>
-__global__ __launch_bounds__(64) void leaf_swaps_kernel(T* Wm, i64 ldw, i64 c0, int w, const i64* ipiv,
-                                                        const int* flags, i64 x_lo, i64 x_hi) {
-    const int lane = threadIdx.x;
+__global__ __launch_bounds__(128) void leaf_swaps_kernel(T* Wm, i64 ldw, i64 c0, int w, const i64* ipiv,
+                                                         const int* flags, i64 x_lo, i64 x_hi) {
+    const int lane = threadIdx.x & 63;
+    const int half = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     // the pivots are loaded with the flag, not after its test, and without a
     // branch (lanes >= w read entry c0): one memory round trip for both
     // (ipiv lies in its own allocation, always readable)
@@ -540,6 +542,7 @@ __global__ __launch_bounds__(64) void leaf_swaps_kernel(T* Wm, i64 ldw, i64 c0, 
     const i64 pi_all = lane < w ? pv - 1 - c0 : -1;
     const int b = blockIdx.x;
     if (b < w - 1) {
+        if (half != 0) return;
         const int c = b;
         i64 pos; int src;
         leaf_slots(pi_all, w, c + 1, lane, pos, src);
@@ -557,26 +560,31 @@ __global__ __launch_bounds__(64) void leaf_swaps_kernel(T* Wm, i64 ldw, i64 c0, 
     else { x = c0 + w + (fb - tl) * 64 + lane; ok = x < x_hi; }
     i64 pos; int src;
     leaf_slots(pi_all, w, 0, lane, pos, src);
-    // slot s moves when its source differs (at most 2w <= 64 slots).  All
-    // loads are issued before any store and without branches: a slot that
-    // does not move reads its own position, a lane outside the columns
-    // reads column c0 (both in bounds), and only moving slots are stored.
-    // Lane s computes slot s's row offsets once; the loops only read them
-    // back (two readlanes and an add per slot and access).
+    // slot s moves when its source differs (at most 2w <= 64 slots).  Both
+    // waves compute all slots; wave h moves slots 32h .. 32h+31.  All loads
+    // of the block have returned before any store (a wait and a barrier) and
+    // are issued without branches: a slot that does not move reads its own
+    // position, a lane outside the columns reads column c0 (both in bounds),
+    // and only moving slots are stored.  Lane s computes slot s's row
+    // offsets once; the loops only read them back (two readlanes and an add
+    // per slot and access).
     const i64 psrc = (i64)__shfl((long long)pos, src);
     const bool moves = pos >= 0 && src != lane;
     const u64 mv = __ballot(moves);
     const i64 offs = (c0 + (moves ? psrc : 0)) * ldw;
     const i64 offd = (c0 + (moves ? pos : 0)) * ldw;
     const i64 xr = ok ? x : c0;
-    T v[64];
+    const int s0 = 32 * half;
+    T v[32];
 #pragma unroll
-    for (int s = 0; s < 64; ++s) v[s] = Wm[(i64)readlane64(u64(offs), s) + xr];
+    for (int k = 0; k < 32; ++k) v[k] = Wm[(i64)readlane64(u64(offs), s0 + k) + xr];
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
 #pragma unroll
-    for (int s = 0; s < 64; ++s)
-        if ((mv >> s) & 1) {
-            const i64 to = (i64)readlane64(u64(offd), s);
-            if (ok) Wm[to + x] = v[s];
+    for (int k = 0; k < 32; ++k)
+        if ((mv >> (s0 + k)) & 1) {
+            const i64 to = (i64)readlane64(u64(offd), s0 + k);
+            if (ok) Wm[to + x] = v[k];
         }
 }
 
