@@ -128,12 +128,17 @@ void launch_right_leaf(hipStream_t s, int unit, int trans, i64 rows, int n, cons
 
 // Inverses of the 64 x 64 diagonal blocks of an upper op(T): block b's
 // V = U_bb^-1 goes to inv + 4096 b, column-major (ld 64, zeros below the
-// diagonal), by back substitution, thread j computing column j:
-// V[i][j] = (delta_ij - sum_{k=i+1..j} U[i][k] V[k][j]) / U[i][i].
+// diagonal).  Thread j computes column j right-looking, with the column's
+// 64 partial sums in registers: for k = j down to 0, V[k][j] = acc_k /
+// U[k][k], then acc_i -= U[i][k] V[k][j] for every i < k.  The k steps form
+// the only dependent chain (64 divisions); the i updates of a step are
+// independent.  Lanes with k > j take no update at step k (the terms the
+// substitution V[i][j] = (delta_ij - sum_{k=i+1..j} U[i][k] V[k][j]) /
+// U[i][i] does not have), so only the order of each sum differs from it.
 template <bool UNIT, bool TRANS>
 __global__ __launch_bounds__(64) void trsm_inv64_kernel(const double* __restrict__ T, i64 ldt,
                                                         double* __restrict__ inv) {
-    __shared__ double U[64][65];
+    __shared__ double Ut[64][65];   // Ut[c][r] = op(T)[r][c]: columns contiguous
     __shared__ double V[64][65];
     const int tj = threadIdx.x;
     const i64 o = i64(blockIdx.x) * 64;
@@ -145,16 +150,22 @@ __global__ __launch_bounds__(64) void trsm_inv64_kernel(const double* __restrict
     for (int q = 0; q < 64; ++q) r[q] = T[(o + tj) + (o + q) * ldt];
 #pragma unroll
     for (int q = 0; q < 64; ++q) {
-        if (TRANS) U[q][tj] = r[q];   // element (row q, column tj) of op(T)
-        else U[tj][q] = r[q];         // element (row tj, column q)
+        if (TRANS) Ut[tj][q] = r[q];   // element (row q, column tj) of op(T)
+        else Ut[q][tj] = r[q];         // element (row tj, column q)
     }
     __syncthreads();
     const int j = tj;
-    for (int i = 63; i >= 0; --i) {
-        if (i > j) { V[i][j] = 0.0; continue; }
-        double acc = i == j ? 1.0 : 0.0;
-        for (int k = i + 1; k <= j; ++k) acc = __builtin_fma(-U[i][k], V[k][j], acc);
-        V[i][j] = UNIT ? acc : acc / U[i][i];
+    double acc[64];
+#pragma unroll
+    for (int i = 0; i < 64; ++i) acc[i] = i == j ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 63; k >= 0; --k) {
+        const double x = UNIT ? acc[k] : acc[k] / Ut[k][k];
+        V[k][j] = k > j ? 0.0 : x;
+        if (k <= j) {
+#pragma unroll
+            for (int i = 0; i < k; ++i) acc[i] = __builtin_fma(-Ut[k][i], x, acc[i]);
+        }
     }
     __syncthreads();
     // column-major store, the thread index along rows
