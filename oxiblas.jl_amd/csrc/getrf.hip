@@ -395,10 +395,11 @@ int swap_w(GetrfCtx<T>& g, i64 k1, i64 k2, i64 c0, i64 c1, int inverse) {
 template <typename T>
 int leaf(GetrfCtx<T>& g, i64 c0, int w) {
     const i64 mrows = g.m - c0;
-    // rows per workgroup: 256 (one per thread) up to 2048 rows, then 512 (two
-    // per thread: the most FP64 rows that stay in registers without spilling);
-    // FP32 rows are half the size and go to 1024 per workgroup beyond 8192 rows
-    int kind = mrows > 2048 ? 1 : 0;
+    // rows per workgroup: 256 (one per thread) up to 8192 rows (32
+    // workgroups), then 512 (two per thread: the most FP64 rows that stay in
+    // registers without spilling); FP32 rows are half the size and go to 1024
+    // per workgroup beyond 8192 rows
+    int kind = mrows > 8192 ? 1 : 0;
     if (sizeof(T) == 4 && mrows > 8192) kind = 2;
     const i64 R = kind == 0 ? 256 : kind == 1 ? 512 : 1024;
     const i64 P = cdiv(mrows, R);
