@@ -398,9 +398,9 @@ int leaf(GetrfCtx<T>& g, i64 c0, int w) {
     // rows per workgroup: 256 (one per thread) up to 8192 rows (32
     // workgroups), then 512 (two per thread: the most FP64 rows that stay in
     // registers without spilling); FP32 rows are half the size and go to 1024
-    // per workgroup beyond 8192 rows
+    // per workgroup beyond 16384 rows (more than 32 workgroups of 512)
     int kind = mrows > 8192 ? 1 : 0;
-    if (sizeof(T) == 4 && mrows > 8192) kind = 2;
+    if (sizeof(T) == 4 && mrows > 16384) kind = 2;
     const i64 R = kind == 0 ? 256 : kind == 1 ? 512 : 1024;
     const i64 P = cdiv(mrows, R);
     if (P > std::min<i64>(g.h->num_cus, OXI_LEAF_MAX_P))
