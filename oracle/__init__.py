@@ -8,6 +8,6 @@ baseline.  The product library never imports it.
 """
 from .oxo import (  # noqa: F401
     available, build, lib, max_threads,
-    dgemm, sgemm, dtrsm, dtrsv, dgetrf, sgetrf, dgetrs, sgetrs,
+    dgemm, sgemm, dtrsm, dtrsv, dgetrf, sgetrf, dgetrs, sgetrs, dlaswp,
     dequilibrate, refinement_residual, mixed_solve, perm_from_ipiv,
 )

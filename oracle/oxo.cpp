@@ -672,6 +672,24 @@ int oxo_dgetrf(i64 m, i64 n, double* A, i64 lda, i64* ipiv, i64 nb, int check, i
 int oxo_sgetrf(i64 m, i64 n, float* A, i64 lda, i64* ipiv, i64 nb, int check, i64* info, int threads) {
     return getrf<float>(m, n, A, lda, ipiv, nb, check, info, threads);
 }
+// _lu_swap!(F, pivots, k1, k2, cols) (src/lapack/factorizations.jl:80-88) on
+// columns 0..ncols-1; inverse != 0 applies the steps in reverse order as
+// _rowswap!(X, pivots, true) (src/api/linalg_api.jl:100-110).  k1, k2 1-based.
+int oxo_dlaswp(i64 ncols, double* A, i64 lda, i64 k1, i64 k2, const i64* ipiv, int inverse) {
+    if (ncols < 0) return -1;
+    if (lda < 1) return -3;
+    if (k1 < 1) return -4;
+    if (k2 < k1 - 1) return -5;
+    for (i64 col = 0; col < ncols; ++col)
+        for (i64 t = 0; t <= k2 - k1; ++t) {
+            const i64 j = inverse ? k2 - 1 - t : k1 - 1 + t;
+            const i64 p = ipiv[j] - 1;
+            if (p < 0 || p >= lda) return -6;
+            if (p != j) std::swap(A[j + col * lda], A[p + col * lda]);
+        }
+    return 0;
+}
+
 int oxo_dgetrs(char trans, i64 n, i64 nrhs, const double* LU, i64 lda, const i64* ipiv, double* B, i64 ldb,
                int vector_rhs, int threads) {
     return getrs<double>(trans, n, nrhs, LU, lda, ipiv, B, ldb, vector_rhs, threads);

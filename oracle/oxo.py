@@ -42,6 +42,7 @@ def lib():
         L.oxo_dtrsv.argtypes = [ch, ch, ch, i64, dp, i64, dp]
         L.oxo_dgetrf.argtypes = [i64, i64, dp, i64, ip, i64, C.c_int, ip, C.c_int]
         L.oxo_sgetrf.argtypes = [i64, i64, fp, i64, ip, i64, C.c_int, ip, C.c_int]
+        L.oxo_dlaswp.argtypes = [i64, dp, i64, i64, i64, ip, C.c_int]
         L.oxo_dgetrs.argtypes = [ch, i64, i64, dp, i64, ip, dp, i64, C.c_int, C.c_int]
         L.oxo_sgetrs.argtypes = [ch, i64, i64, fp, i64, ip, fp, i64, C.c_int, C.c_int]
         L.oxo_dequilibrate.argtypes = [i64, i64, dp, i64, dp, dp, C.c_int]
@@ -52,7 +53,7 @@ def lib():
                                       ip, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]
         for f in ("oxo_dgemm", "oxo_sgemm", "oxo_dtrsm", "oxo_dtrsv", "oxo_dgetrf", "oxo_sgetrf",
-                  "oxo_dgetrs", "oxo_sgetrs", "oxo_mixed_solve", "oxo_max_threads"):
+                  "oxo_dgetrs", "oxo_sgetrs", "oxo_dlaswp", "oxo_mixed_solve", "oxo_max_threads"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -150,6 +151,17 @@ def sgetrf(A, blocksize=128, check=False, threads=1):
                           C.byref(info), threads)
     _check(rc, "sgetrf")
     return F, ipiv[:min(m, n)], info.value
+
+
+def dlaswp(A, k1, k2, ipiv, inverse=False):
+    """Interchanges k1..k2 (1-based) of ipiv on every column of A, in place
+    (_lu_swap!, src/lapack/factorizations.jl:80-88; reverse order for the
+    inverse as _rowswap!, src/api/linalg_api.jl:100-110)."""
+    assert A.flags.f_contiguous and A.dtype == np.float64 and A.ndim == 2
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int64)
+    rc = lib().oxo_dlaswp(A.shape[1], _ptr(A), max(1, A.shape[0]), k1, k2, _ptr(ipiv, ip), 1 if inverse else 0)
+    _check(rc, "dlaswp")
+    return A
 
 
 def dgetrs(F, ipiv, B, trans="N", vector=None, threads=1):

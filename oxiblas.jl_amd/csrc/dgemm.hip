@@ -17,6 +17,9 @@
 #include "oxi_internal.hpp"
 
 #include <cstring>
+#include <vector>
+#include <algorithm>
+#include <limits>
 
 namespace {
 
@@ -107,11 +110,50 @@ __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int& 
     tn = in / gm;
 }
 
-template <int BM, int BN, bool TA, bool TB, bool BETA0>
+// Launch modes of the tile kernels.  PLAIN: one product.  STRIDED: product
+// blockIdx.y of a batch whose operands sit at fixed strides.  PTRS: product
+// blockIdx.y of a batch given by device arrays of operand pointers.  LOWER /
+// UPPER: one product of which only the lower (row >= column) or upper
+// triangle of C is computed and written (syrk!/herk!): tiles outside the
+// triangle leave at once, tiles across the diagonal mask their stores.
+enum GemmMode { PLAIN = 0, STRIDED = 1, PTRS = 2, LOWER = 3, UPPER = 4 };
+struct BatchArg {
+    i64 sa, sb, sc;                    // STRIDED: element strides between the products' operands
+    const double* const* pa;           // PTRS: device arrays of operand pointers
+    const double* const* pb;
+    double* const* pc;
+};
+template <int MODE>
+__device__ __forceinline__ void batch_operands(const BatchArg& ba, const double* __restrict__& A,
+                                               const double* __restrict__& B, double* __restrict__& C) {
+    if (MODE == STRIDED) {
+        const i64 z = blockIdx.y;
+        A += z * ba.sa; B += z * ba.sb; C += z * ba.sc;
+    } else if (MODE == PTRS) {
+        const int z = blockIdx.y;
+        A = ba.pa[z]; B = ba.pb[z]; C = ba.pc[z];
+    }
+}
+// false when the BM x BN tile at (m0, n0) has no element in the triangle
+template <int MODE>
+__device__ __forceinline__ bool tile_in_triangle(i64 m0, i64 n0, int BM, int BN) {
+    if (MODE == LOWER) return m0 + BM - 1 >= n0;
+    if (MODE == UPPER) return n0 + BN - 1 >= m0;
+    return true;
+}
+template <int MODE>
+__device__ __forceinline__ bool keep(i64 row, i64 col) {
+    if (MODE == LOWER) return row >= col;
+    if (MODE == UPPER) return row <= col;
+    return true;
+}
+
+template <int BM, int BN, bool TA, bool TB, bool BETA0, int MODE = PLAIN>
 __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, double alpha,
                                                         const double* __restrict__ A, i64 lda,
                                                         const double* __restrict__ B, i64 ldb, double beta,
-                                                        double* __restrict__ C, i64 ldc, int tiles_m, int tiles_n) {
+                                                        double* __restrict__ C, i64 ldc, int tiles_m, int tiles_n,
+                                                        BatchArg ba = BatchArg{}) {
     constexpr int LDS_A = BM + PAD, LDS_B = BN + PAD;
     constexpr int QM = BM / 16, QN = BN / 16;       // 4x4 blocks per lane group, down and across
     __shared__ double As[2][BK * LDS_A];
@@ -123,6 +165,8 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
     int tm, tn;
     tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const i64 m0 = i64(tm) * BM, n0 = i64(tn) * BN;
+    if (!tile_in_triangle<MODE>(m0, n0, BM, BN)) return;
+    batch_operands<MODE>(ba, A, B, C);
 
     // op(A) rows contiguous when A is 'N'; op(B) columns contiguous when B is 'T'
     constexpr bool A_OC = !TA, B_OC = TB;
@@ -215,7 +259,8 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
             const double v = alpha * E[(sc0 + SCS * k) * LDS_A + sr];
-            cc[i64(SCS * k) * ldc] = BETA0 ? v : __builtin_fma(beta, cs[k], v);
+            if (keep<MODE>(m0 + sr, n0 + sc0 + SCS * k))
+                cc[i64(SCS * k) * ldc] = BETA0 ? v : __builtin_fma(beta, cs[k], v);
         }
         return;
     }
@@ -238,7 +283,8 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
         for (int q = 0; q < QM; ++q) {
             const i64 row = m0 + erow + 8 * q;
             const double v = alpha * acc[q][p];
-            if (row < M && col < N) C[col * ldc + row] = BETA0 ? v : __builtin_fma(beta, cv[q][p], v);
+            if (row < M && col < N && keep<MODE>(row, col))
+                C[col * ldc + row] = BETA0 ? v : __builtin_fma(beta, cv[q][p], v);
         }
     }
 }
@@ -255,7 +301,7 @@ constexpr int BK16 = 16;
 
 // One tile; EDGE: the tile crosses an edge of C or K is not a multiple of
 // the slab depth (bounds-checked loads and stores).
-template <bool EDGE, bool TA, bool TB, bool BETA0>
+template <bool EDGE, bool TA, bool TB, bool BETA0, int MODE>
 __device__ __forceinline__ void m16_tile(double (*As)[BK16 * 129], double (*Bs)[BK16 * 129], double* E, i64 M, i64 N, i64 K,
                                          double alpha, const double* __restrict__ A, i64 lda,
                                          const double* __restrict__ B, i64 ldb, double beta, double* __restrict__ C,
@@ -350,7 +396,8 @@ __device__ __forceinline__ void m16_tile(double (*As)[BK16 * 129], double (*Bs)[
                 for (int k = 0; k < 16; ++k) {
                     const int c = 16 * b + k;
                     const double v = alpha * E[(ec0 + 2 * c) * LDE + er];
-                    cc[i64(2 * c) * ldc] = BETA0 ? v : __builtin_fma(beta, cv[b][k], v);
+                    if (keep<MODE>(m0 + er, n0 + h * 64 + ec0 + 2 * c))
+                        cc[i64(2 * c) * ldc] = BETA0 ? v : __builtin_fma(beta, cv[b][k], v);
                 }
             if (h == 0) __syncthreads();
         }
@@ -379,16 +426,18 @@ __device__ __forceinline__ void m16_tile(double (*As)[BK16 * 129], double (*Bs)[
             for (int r = 0; r < 4; ++r) {
                 const i64 row = m0 + wm * 64 + i * 16 + lk + 4 * r;
                 const double v = alpha * acc[i][j][r];
-                if (!EDGE || (row < M && col < N)) C[col * ldc + row] = BETA0 ? v : __builtin_fma(beta, cv[i][r], v);
+                if ((!EDGE || (row < M && col < N)) && keep<MODE>(row, col))
+                    C[col * ldc + row] = BETA0 ? v : __builtin_fma(beta, cv[i][r], v);
             }
     }
 }
 
-template <bool TA, bool TB, bool BETA0>
+template <bool TA, bool TB, bool BETA0, int MODE = PLAIN>
 __global__ __launch_bounds__(NT, 2) void dgemm_m16_kernel(i64 M, i64 N, i64 K, double alpha,
                                                           const double* __restrict__ A, i64 lda,
                                                           const double* __restrict__ B, i64 ldb, double beta,
-                                                          double* __restrict__ C, i64 ldc, int tiles_m, int tiles_n) {
+                                                          double* __restrict__ C, i64 ldc, int tiles_m, int tiles_n,
+                                                          BatchArg ba = BatchArg{}) {
     // the slabs of A and B, and after the K loop the staging of C
     __shared__ __align__(16) double lds[4 * BK16 * 129];
     double (*As)[BK16 * 129] = (double (*)[BK16 * 129])lds;
@@ -396,10 +445,12 @@ __global__ __launch_bounds__(NT, 2) void dgemm_m16_kernel(i64 M, i64 N, i64 K, d
     int tm, tn;
     tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const i64 m0 = i64(tm) * 128, n0 = i64(tn) * 128;
+    if (!tile_in_triangle<MODE>(m0, n0, 128, 128)) return;
+    batch_operands<MODE>(ba, A, B, C);
     if (m0 + 128 <= M && n0 + 128 <= N && K % BK16 == 0)
-        m16_tile<false, TA, TB, BETA0>(As, Bs, lds, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
+        m16_tile<false, TA, TB, BETA0, MODE>(As, Bs, lds, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
     else
-        m16_tile<true, TA, TB, BETA0>(As, Bs, lds, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
+        m16_tile<true, TA, TB, BETA0, MODE>(As, Bs, lds, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
 }
 
 __global__ void dscale_kernel(i64 m, i64 n, double beta, double* __restrict__ C, i64 ldc) {
@@ -411,30 +462,41 @@ __global__ void dscale_kernel(i64 m, i64 n, double beta, double* __restrict__ C,
     }
 }
 
-template <int BM, int BN, bool TA, bool TB>
-void launch_tiles(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda, const double* B,
-                  i64 ldb, double beta, double* C, i64 ldc) {
-    const int tiles_m = int(cdiv(m, BM)), tiles_n = int(cdiv(n, BN));
-    dim3 grid(tiles_m * tiles_n), block(NT);
-    if (beta == 0.0)
-        hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, TA, TB, true>), grid, block, 0, s, m, n, k, alpha, A, lda, B,
-                           ldb, beta, C, ldc, tiles_m, tiles_n);
-    else
-        hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, TA, TB, false>), grid, block, 0, s, m, n, k, alpha, A, lda, B,
-                           ldb, beta, C, ldc, tiles_m, tiles_n);
+// C[triangle] <- beta * C[triangle] (beta == 0 writes zeros without reading)
+__global__ void dscale_tri_kernel(i64 n, double beta, double* __restrict__ C, i64 ldc, int upper) {
+    const i64 total = n * n;
+    for (i64 idx = i64(blockIdx.x) * blockDim.x + threadIdx.x; idx < total; idx += i64(gridDim.x) * blockDim.x) {
+        const i64 j = idx / n, i = idx - j * n;
+        if (upper ? i > j : i < j) continue;
+        double& c = C[i + j * ldc];
+        c = (beta == 0.0) ? 0.0 : c * beta;
+    }
 }
 
-template <bool TA, bool TB>
-void launch_m16(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
-                double beta, double* C, i64 ldc) {
-    const int tiles_m = int(cdiv(m, 128)), tiles_n = int(cdiv(n, 128));
-    dim3 grid(tiles_m * tiles_n), block(NT);
+template <int BM, int BN, bool TA, bool TB, int MODE = PLAIN>
+void launch_tiles(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda, const double* B,
+                  i64 ldb, double beta, double* C, i64 ldc, unsigned batch = 1, BatchArg ba = BatchArg{}) {
+    const int tiles_m = int(cdiv(m, BM)), tiles_n = int(cdiv(n, BN));
+    dim3 grid(tiles_m * tiles_n, batch), block(NT);
     if (beta == 0.0)
-        hipLaunchKernelGGL((dgemm_m16_kernel<TA, TB, true>), grid, block, 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
-                           ldc, tiles_m, tiles_n);
+        hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, TA, TB, true, MODE>), grid, block, 0, s, m, n, k, alpha, A, lda,
+                           B, ldb, beta, C, ldc, tiles_m, tiles_n, ba);
     else
-        hipLaunchKernelGGL((dgemm_m16_kernel<TA, TB, false>), grid, block, 0, s, m, n, k, alpha, A, lda, B, ldb, beta,
-                           C, ldc, tiles_m, tiles_n);
+        hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, TA, TB, false, MODE>), grid, block, 0, s, m, n, k, alpha, A,
+                           lda, B, ldb, beta, C, ldc, tiles_m, tiles_n, ba);
+}
+
+template <bool TA, bool TB, int MODE = PLAIN>
+void launch_m16(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
+                double beta, double* C, i64 ldc, unsigned batch = 1, BatchArg ba = BatchArg{}) {
+    const int tiles_m = int(cdiv(m, 128)), tiles_n = int(cdiv(n, 128));
+    dim3 grid(tiles_m * tiles_n, batch), block(NT);
+    if (beta == 0.0)
+        hipLaunchKernelGGL((dgemm_m16_kernel<TA, TB, true, MODE>), grid, block, 0, s, m, n, k, alpha, A, lda, B, ldb,
+                           beta, C, ldc, tiles_m, tiles_n, ba);
+    else
+        hipLaunchKernelGGL((dgemm_m16_kernel<TA, TB, false, MODE>), grid, block, 0, s, m, n, k, alpha, A, lda, B, ldb,
+                           beta, C, ldc, tiles_m, tiles_n, ba);
 }
 
 // Tile shape of C per workgroup (measured on MI355X over the LU's products
@@ -448,7 +510,7 @@ void launch_m16(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* 
 // takes a 32-wide tile.  OXIBLAS_DGEMM_TILE (m16, 64x64, 32x32) fixes the
 // choice (tuning); bm = bn = 16 stands for the 16x16x4 kernel.
 constexpr int NCU = 256;
-void choose_tile(i64 m, i64 n, int& bm, int& bn) {
+void choose_tile(i64 m, i64 n, int& bm, int& bn, i64 batch = 1) {
     static int forced = -1;
     if (forced < 0) {
         forced = 0;
@@ -459,24 +521,37 @@ void choose_tile(i64 m, i64 n, int& bm, int& bn) {
         }
     }
     if (forced) { bm = bn = forced; return; }
-    if (cdiv(m, 128) * cdiv(n, 128) >= NCU * 7 / 4) { bm = bn = 16; return; }
+    // a batch fills the chip with its products together
+    if (cdiv(m, 128) * cdiv(n, 128) * batch >= NCU * 7 / 4) { bm = bn = 16; return; }
     bm = m <= 32 ? 32 : 64;
     bn = n <= 32 ? 32 : 64;
-    if (cdiv(m, bm) * cdiv(n, bn) < i64(NCU) * 3 / 2) bm = bn = 32;
+    if (cdiv(m, bm) * cdiv(n, bn) * batch < i64(NCU) * 3 / 2) bm = bn = 32;
 }
 
 // false when no kernel takes the chosen tile shape (nothing launched)
-template <bool TA, bool TB>
+template <bool TA, bool TB, int MODE = PLAIN>
 bool launch_variant(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda, const double* B,
-                    i64 ldb, double beta, double* C, i64 ldc) {
+                    i64 ldb, double beta, double* C, i64 ldc, unsigned batch = 1, BatchArg ba = BatchArg{}) {
     int bm, bn;
-    choose_tile(m, n, bm, bn);
-    if (bm == 16) { launch_m16<TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc); return true; }
+    choose_tile(m, n, bm, bn, batch);
+    if (bm == 16) { launch_m16<TA, TB, MODE>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, batch, ba); return true; }
 #define OXI_TILE(X, Y)                                                                                              \
-    if (bm == X && bn == Y) { launch_tiles<X, Y, TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc); return true; }
+    if (bm == X && bn == Y) {                                                                                       \
+        launch_tiles<X, Y, TA, TB, MODE>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, batch, ba);              \
+        return true;                                                                                                \
+    }
     OXI_TILE(64, 64) OXI_TILE(32, 32) OXI_TILE(64, 32) OXI_TILE(32, 64)
 #undef OXI_TILE
     return false;
+}
+
+template <int MODE>
+bool launch_any(hipStream_t s, int ta, int tb, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda,
+                const double* B, i64 ldb, double beta, double* C, i64 ldc, unsigned batch, BatchArg ba) {
+    if (!ta && !tb) return launch_variant<false, false, MODE>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, batch, ba);
+    if (!ta && tb) return launch_variant<false, true, MODE>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, batch, ba);
+    if (ta && !tb) return launch_variant<true, false, MODE>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, batch, ba);
+    return launch_variant<true, true, MODE>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, batch, ba);
 }
 
 }  // namespace
@@ -521,6 +596,74 @@ int oxi_launch_dgemm(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n
     if (!launched) return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "dgemm: no kernel for the chosen tile shape");
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "dgemm launch: %s", hipGetErrorString(e));
+    return OXI_OK;
+}
+
+// A batch of equal-shaped products in one launch per 65535 products
+// (gemm_strided_batched!, src/blas/batched.jl:80-100): grid.y is the product.
+int oxi_launch_dgemm_strided(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n, i64 k, double alpha,
+                             const double* A, i64 lda, i64 sa, const double* B, i64 ldb, i64 sb, double beta,
+                             double* C, i64 ldc, i64 sc, i64 batch) {
+    if (m <= 0 || n <= 0 || batch <= 0) return OXI_OK;
+    if (alpha == 0.0 || k <= 0) {
+        for (i64 z = 0; z < batch; ++z) OXI_TRY(oxi_launch_dscale(s, m, n, beta, C + z * sc, ldc));
+        return OXI_OK;
+    }
+    if (cdiv(m, 32) * cdiv(n, 32) > (i64(1) << 31) - 1)
+        return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "dgemm batch: %lld x %lld has too many tiles", (long long)m,
+                             (long long)n);
+    for (i64 z0 = 0; z0 < batch; z0 += 65535) {
+        const unsigned nb = unsigned(std::min<i64>(65535, batch - z0));
+        BatchArg ba{sa, sb, sc, nullptr, nullptr, nullptr};
+        if (!launch_any<STRIDED>(s, ta, tb, m, n, k, alpha, A + z0 * sa, lda, B + z0 * sb, ldb, beta, C + z0 * sc,
+                                 ldc, nb, ba))
+            return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "dgemm batch: no kernel for the chosen tile shape");
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "dgemm batch launch: %s", hipGetErrorString(e));
+    }
+    return OXI_OK;
+}
+
+// A batch given by device arrays of operand pointers (gemm_batched!,
+// src/blas/batched.jl:49-78), all of one shape.
+int oxi_launch_dgemm_ptrs(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n, i64 k, double alpha,
+                          const double* const* A, i64 lda, const double* const* B, i64 ldb, double beta,
+                          double* const* C, i64 ldc, i64 batch) {
+    if (m <= 0 || n <= 0 || batch <= 0) return OXI_OK;
+    if (cdiv(m, 32) * cdiv(n, 32) > (i64(1) << 31) - 1)
+        return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "dgemm batch: %lld x %lld has too many tiles", (long long)m,
+                             (long long)n);
+    for (i64 z0 = 0; z0 < batch; z0 += 65535) {
+        const unsigned nb = unsigned(std::min<i64>(65535, batch - z0));
+        BatchArg ba{0, 0, 0, A + z0, B + z0, C + z0};
+        if (!launch_any<PTRS>(s, ta, tb, m, n, k, alpha, nullptr, lda, nullptr, ldb, beta, nullptr, ldc, nb, ba))
+            return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "dgemm batch: no kernel for the chosen tile shape");
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "dgemm batch launch: %s", hipGetErrorString(e));
+    }
+    return OXI_OK;
+}
+
+// C[triangle] <- alpha * op(A) op(A)^T + beta * C[triangle], the other
+// triangle untouched (syrk!/herk! for Float64, src/blas/level3.jl:546-783).
+// trans 0: op(A) = A (n x k); 1: op(A) = A^T (A k x n).
+int oxi_launch_dsyrk(oxi_handle_t h, hipStream_t s, int upper, int trans, i64 n, i64 k, double alpha,
+                     const double* A, i64 lda, double beta, double* C, i64 ldc) {
+    if (n <= 0) return OXI_OK;
+    if (alpha == 0.0 || k <= 0) {
+        if (beta == 1.0) return OXI_OK;
+        const int blocks = int(std::min<i64>(cdiv(n * n, 256), 4096));
+        hipLaunchKernelGGL(dscale_tri_kernel, dim3(blocks), dim3(256), 0, s, n, beta, C, ldc, upper);
+        return hipGetLastError() == hipSuccess ? OXI_OK : oxi_set_error(h, OXI_ERR_HIP, "syrk scale launch");
+    }
+    if (cdiv(n, 32) * cdiv(n, 32) > (i64(1) << 31) - 1)
+        return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "syrk: n = %lld has too many tiles", (long long)n);
+    const int ta = trans ? 1 : 0, tb = trans ? 0 : 1;
+    const bool ok = upper ? launch_any<UPPER>(s, ta, tb, n, n, k, alpha, A, lda, A, lda, beta, C, ldc, 1, BatchArg{})
+                          : launch_any<LOWER>(s, ta, tb, n, n, k, alpha, A, lda, A, lda, beta, C, ldc, 1, BatchArg{});
+    if (!ok) return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "syrk: no kernel for the chosen tile shape");
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "syrk launch: %s", hipGetErrorString(e));
     return OXI_OK;
 }
 
@@ -579,4 +722,170 @@ extern "C" int oxi_dgemm(oxi_handle_t h, char transA, char transB, int64_t m, in
         }
     }
     return oxi_launch_dgemm(h, s, ka, kb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+extern "C" int oxi_dgemm_strided_batched(oxi_handle_t h, char transA, char transB, int64_t m, int64_t n, int64_t k,
+                                         double alpha, const double* A, int64_t lda, int64_t strideA, const double* B,
+                                         int64_t ldb, int64_t strideB, double beta, double* C, int64_t ldc,
+                                         int64_t strideC, int64_t batch) {
+    if (!h) return -1;
+    const int ka = trans_kind(transA), kb = trans_kind(transB);
+    if (ka < 0) return -2;
+    if (kb < 0) return -3;
+    if (m < 0) return -4;
+    if (n < 0) return -5;
+    if (k < 0) return -6;
+    const i64 ar = ka ? k : m, ac = ka ? m : k, br = kb ? n : k, bc = kb ? k : n;
+    if (lda < std::max<i64>(1, ar)) return -9;
+    if (ldb < std::max<i64>(1, br)) return -12;
+    if (ldc < std::max<i64>(1, m)) return -16;
+    if (batch < 0) return -18;
+    std::lock_guard<std::recursive_mutex> lk(h->guard);
+    if (m == 0 || n == 0 || batch == 0) return OXI_OK;
+    // disjoint outputs (src/blas/batched.jl:104-117): consecutive products'
+    // spans of C may not meet
+    const size_t cspan = span_bytes(m, n, ldc);
+    if (batch > 1 && (strideC < 0 || size_t(strideC) * 8 < cspan)) return -17;
+    if (!C) return -15;
+    const bool active = alpha != 0.0 && k > 0;
+    if (active && (!A || !B)) return !A ? -8 : -11;
+    if (active && batch > 1 && (strideA < 0 || strideB < 0)) return strideA < 0 ? -10 : -13;
+    (void)hipSetDevice(h->device);
+    hipStream_t s = h->stream;
+    if (active) {
+        // an operand overlapping the outputs is copied first (batched.jl:87-88)
+        const size_t call = size_t(batch - 1) * size_t(strideC) * 8 + cspan;
+        const size_t aall = size_t(batch - 1) * size_t(strideA) * 8 + span_bytes(ar, ac, lda);
+        const size_t ball = size_t(batch - 1) * size_t(strideB) * 8 + span_bytes(br, bc, ldb);
+        const bool ca = overlap(C, call, A, aall), cb = overlap(C, call, B, ball);
+        if (ca || cb) {
+            const size_t need = (ca ? ((aall + 255) & ~size_t(255)) : 0) + (cb ? ((ball + 255) & ~size_t(255)) : 0) + 256;
+            void* ws;
+            OXI_TRY(oxi_workspace(h, need, &ws));
+            char* p = (char*)ws;
+            if (ca) {
+                OXI_HIP(h, hipMemcpyAsync(p, A, aall, hipMemcpyDeviceToDevice, s));
+                A = (const double*)p; p += (aall + 255) & ~size_t(255);
+            }
+            if (cb) {
+                OXI_HIP(h, hipMemcpyAsync(p, B, ball, hipMemcpyDeviceToDevice, s));
+                B = (const double*)p;
+            }
+        }
+    }
+    return oxi_launch_dgemm_strided(h, s, ka, kb, m, n, k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
+                                    strideC, batch);
+}
+
+extern "C" int oxi_dgemm_batched(oxi_handle_t h, char transA, char transB, int64_t m, int64_t n, int64_t k,
+                                 double alpha, const double* const* A, int64_t lda, const double* const* B,
+                                 int64_t ldb, double beta, double* const* C, int64_t ldc, int64_t batch) {
+    if (!h) return -1;
+    const int ka = trans_kind(transA), kb = trans_kind(transB);
+    if (ka < 0) return -2;
+    if (kb < 0) return -3;
+    if (m < 0) return -4;
+    if (n < 0) return -5;
+    if (k < 0) return -6;
+    const i64 ar = ka ? k : m, ac = ka ? m : k, br = kb ? n : k, bc = kb ? k : n;
+    if (lda < std::max<i64>(1, ar)) return -9;
+    if (ldb < std::max<i64>(1, br)) return -11;
+    if (ldc < std::max<i64>(1, m)) return -14;
+    if (batch < 0) return -15;
+    std::lock_guard<std::recursive_mutex> lk(h->guard);
+    if (m == 0 || n == 0 || batch == 0) return OXI_OK;
+    if (!C) return -13;
+    const bool active = alpha != 0.0 && k > 0;
+    if (active && (!A || !B)) return !A ? -8 : -10;
+    // the pointer arrays are host arrays of device pointers; every output
+    // is disjoint from the others (src/blas/batched.jl:66-68)
+    const size_t cspan = span_bytes(m, n, ldc), aspan = span_bytes(ar, ac, lda), bspan = span_bytes(br, bc, ldb);
+    std::vector<std::pair<uintptr_t, i64>> cs(static_cast<size_t>(batch));
+    for (i64 z = 0; z < batch; ++z) {
+        if (!C[z]) return -13;
+        if (active && (!A[z] || !B[z])) return !A[z] ? -8 : -10;
+        cs[size_t(z)] = {uintptr_t(C[z]), z};
+    }
+    std::sort(cs.begin(), cs.end());
+    for (size_t i = 1; i < cs.size(); ++i)
+        if (cs[i].first < cs[i - 1].first + cspan) return oxi_set_error(h, -13, "gemm_batched: outputs overlap");
+    auto hits_c = [&](uintptr_t p, size_t bytes) {
+        // any output span [c, c + cspan) meeting [p, p + bytes): the last
+        // output starting before p + bytes is the only candidate (the
+        // outputs are disjoint and sorted)
+        auto it = std::lower_bound(cs.begin(), cs.end(), std::make_pair(p + bytes, std::numeric_limits<i64>::min()));
+        if (it == cs.begin()) return false;
+        --it;
+        return it->first + cspan > p;
+    };
+    (void)hipSetDevice(h->device);
+    hipStream_t s = h->stream;
+    // operands that alias an output are copied first (batched.jl:58-65)
+    std::vector<const double*> pa(A ? A : nullptr, A ? A + batch : nullptr), pb(B ? B : nullptr, B ? B + batch : nullptr);
+    size_t copies = 0;
+    if (active)
+        for (i64 z = 0; z < batch; ++z) {
+            if (hits_c(uintptr_t(pa[z]), aspan)) copies += (aspan + 255) & ~size_t(255);
+            if (hits_c(uintptr_t(pb[z]), bspan)) copies += (bspan + 255) & ~size_t(255);
+        }
+    const size_t parr = (size_t(3 * batch) * 8 + 255) & ~size_t(255);
+    void* ws;
+    OXI_TRY(oxi_workspace(h, parr + copies + 256, &ws));
+    char* snap = (char*)ws + parr;
+    if (active)
+        for (i64 z = 0; z < batch; ++z) {
+            if (hits_c(uintptr_t(pa[z]), aspan)) {
+                OXI_HIP(h, hipMemcpyAsync(snap, pa[z], aspan, hipMemcpyDeviceToDevice, s));
+                pa[z] = (const double*)snap; snap += (aspan + 255) & ~size_t(255);
+            }
+            if (hits_c(uintptr_t(pb[z]), bspan)) {
+                OXI_HIP(h, hipMemcpyAsync(snap, pb[z], bspan, hipMemcpyDeviceToDevice, s));
+                pb[z] = (const double*)snap; snap += (bspan + 255) & ~size_t(255);
+            }
+        }
+    if (!active) {
+        for (i64 z = 0; z < batch; ++z) OXI_TRY(oxi_launch_dscale(s, m, n, beta, C[z], ldc));
+        return OXI_OK;
+    }
+    std::vector<const void*> ptrs(size_t(3 * batch));
+    for (i64 z = 0; z < batch; ++z) {
+        ptrs[size_t(z)] = pa[z];
+        ptrs[size_t(batch + z)] = pb[z];
+        ptrs[size_t(2 * batch + z)] = C[z];
+    }
+    const double* const* dA = (const double* const*)ws;
+    OXI_HIP(h, hipMemcpyAsync(ws, ptrs.data(), ptrs.size() * 8, hipMemcpyHostToDevice, s));
+    OXI_TRY(oxi_launch_dgemm_ptrs(h, s, ka, kb, m, n, k, alpha, dA, lda, dA + batch, ldb, beta,
+                                  (double* const*)(dA + 2 * batch), ldc, batch));
+    // the host pointer array is read by the copy above before this call returns
+    OXI_HIP(h, hipStreamSynchronize(s));
+    return OXI_OK;
+}
+
+extern "C" int oxi_dsyrk(oxi_handle_t h, char uplo, char trans, int64_t n, int64_t k, double alpha, const double* A,
+                         int64_t lda, double beta, double* C, int64_t ldc) {
+    if (!h) return -1;
+    const bool upper = uplo == 'U' || uplo == 'u';
+    if (!upper && !(uplo == 'L' || uplo == 'l')) return -2;
+    const int tk = trans_kind(trans);
+    if (tk < 0) return -3;
+    if (n < 0) return -4;
+    if (k < 0) return -5;
+    if (lda < std::max<i64>(1, tk ? k : n)) return -8;
+    if (ldc < std::max<i64>(1, n)) return -11;
+    std::lock_guard<std::recursive_mutex> lk(h->guard);
+    if (n == 0) return OXI_OK;
+    if (!C) return -10;
+    if (alpha != 0.0 && k > 0 && !A) return -7;
+    (void)hipSetDevice(h->device);
+    hipStream_t s = h->stream;
+    // A overlapping C is copied first (src/blas/level3.jl:737-750)
+    const size_t abytes = span_bytes(tk ? k : n, tk ? n : k, lda);
+    if (alpha != 0.0 && k > 0 && overlap(C, span_bytes(n, n, ldc), A, abytes)) {
+        void* ws;
+        OXI_TRY(oxi_workspace(h, abytes + 256, &ws));
+        OXI_HIP(h, hipMemcpyAsync(ws, A, abytes, hipMemcpyDeviceToDevice, s));
+        A = (const double*)ws;
+    }
+    return oxi_launch_dsyrk(h, s, upper, tk, n, k, alpha, A, lda, beta, C, ldc);
 }

@@ -703,8 +703,21 @@ extern "C" int oxi_dlaswp(oxi_handle_t h, int64_t ncols, double* A, int64_t lda,
     if (k2 < k1 - 1) return -6;
     std::lock_guard<std::recursive_mutex> lk(h->guard);
     if (ncols == 0 || k2 < k1) return OXI_OK;
+    if (!A) return -3;
+    if (!ipiv) return -7;
+    if (k2 > lda) return -6;
     (void)hipSetDevice(h->device);
     hipStream_t s = h->stream;
+    {
+        // every row a step names must lie inside the leading dimension
+        std::vector<i64> piv(size_t(k2 - k1 + 1));
+        OXI_HIP(h, hipMemcpyAsync(piv.data(), ipiv + (k1 - 1), piv.size() * 8, hipMemcpyDeviceToHost, s));
+        OXI_HIP(h, hipStreamSynchronize(s));
+        for (size_t t = 0; t < piv.size(); ++t)
+            if (piv[t] < 1 || piv[t] > lda)
+                return oxi_set_error(h, -7, "laswp: ipiv[%lld] = %lld outside 1..lda", (long long)(k1 + i64(t)),
+                                     (long long)piv[t]);
+    }
     void* ws;
     OXI_TRY(oxi_workspace(h, size_t(4 * PERM_MAX_STEPS + 4) * 4 + 256, &ws));
     int* pairs = (int*)ws;

@@ -144,7 +144,10 @@ int getrs_api(oxi_handle_t h, char trans, i64 n, i64 nrhs, const T* LU, i64 lda,
     i64* dperm = (i64*)((char*)ws + xpad);
     void* ctrl = (char*)ws + xpad + ppad;
     OXI_HIP(h, hipMemcpyAsync(dperm, perm.data(), size_t(n) * 8, hipMemcpyHostToDevice, s));
-    if (chain) OXI_TRY(oxi_trsv_prepare(h, s, tk, n, LU, lda, (T*)nullptr, ctrl));
+    if (chain) {
+        OXI_TRY(oxi_trsv_begin(h, s, ctrl));
+        OXI_TRY(oxi_trsv_prepare(h, s, tk, n, LU, lda, (T*)nullptr, ctrl));
+    }
     OXI_TRY(getrs_perm<T>(h, s, tk, n, nrhs, LU, lda, dperm, B, ldb, Y, ctrl));
     if (chain) return oxi_trsv_check(h, s, ctrl);   // the chain kernel ran: its status
     OXI_HIP(h, hipStreamSynchronize(s));

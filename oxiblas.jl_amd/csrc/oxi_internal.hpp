@@ -132,6 +132,8 @@ int oxi_trsv(oxi_handle_t h, hipStream_t s, int which, i64 n, const double* M, i
 int oxi_trsv(oxi_handle_t h, hipStream_t s, int which, i64 n, const float* M, i64 ld, float* x, void* ctrl);
 int oxi_trsv(oxi_handle_t h, hipStream_t s, int which, i64 n, const float* M, i64 ld, double* x, void* ctrl);
 int oxi_trsv_check(oxi_handle_t h, hipStream_t s, const void* ctrl);
+// clears the status word of ctrl (once per solve, before its chain launches)
+int oxi_trsv_begin(oxi_handle_t h, hipStream_t s, void* ctrl);
 // The trsv workspace (ctrl above): control words and the inverses of the
 // 128 x 128 diagonal blocks of the two triangles of a solve, in the vector
 // type (tv bytes).  oxi_trsv_prepare fills them for the solve kind tk

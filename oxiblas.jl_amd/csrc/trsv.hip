@@ -70,7 +70,14 @@ __device__ __forceinline__ V get_agent(const V* p) {
     return __hip_atomic_load(const_cast<V*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Per launch: the ticket and the ready counter start at zero.  The error word
+// is sticky across the launches of one solve (a timed-out launch leaves x
+// unwritten, and a later launch must not hide that); it is cleared once per
+// solve by trsv_status_clear and read at the end by oxi_trsv_check.
 __global__ void trsv_ctrl_reset(TrsvCtrl* c) {
+    if (threadIdx.x == 0) { c->ticket = 0; c->ready = 0; }
+}
+__global__ void trsv_status_clear(TrsvCtrl* c) {
     if (threadIdx.x == 0) { c->ticket = 0; c->ready = 0; c->err = 0; c->pad = 0; }
 }
 
@@ -413,6 +420,13 @@ int oxi_trsv(oxi_handle_t h, hipStream_t s, int which, i64 n, const float* M, i6
 }
 int oxi_trsv(oxi_handle_t h, hipStream_t s, int which, i64 n, const float* M, i64 ld, double* x, void* ctrl) {
     return trsv_t<float, double>(h, s, which, n, M, ld, x, ctrl);
+}
+
+int oxi_trsv_begin(oxi_handle_t h, hipStream_t s, void* ctrl) {
+    hipLaunchKernelGGL(trsv_status_clear, dim3(1), dim3(64), 0, s, static_cast<TrsvCtrl*>(ctrl));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "trsv status clear: %s", hipGetErrorString(e));
+    return OXI_OK;
 }
 
 int oxi_trsv_check(oxi_handle_t h, hipStream_t s, const void* ctrl) {

@@ -355,15 +355,15 @@ end
 const _REASONS = (:converged, :iteration_limit, :nonfinite_iteration, :working_factor_fallback)
 
 """
-    mixed_solve(A, b; maxiter=20, rtol=16eps(), lowtype=Float32, solver=:ir, gmres_restart=0)
+    mixed_solve(A, b; maxiter=10, rtol=1e-13, lowtype=Float32, solver=:ir, gmres_restart=0)
 
 Radix equilibration, a low-precision LU on the GPU (FP32 panel; trailing
 products in `lowtype`) and FP64 refinement until the componentwise backward
 error is <= rtol; `solver=:gmres` runs GMRES-IR, `:auto` switches to it when
 classical refinement stalls.  Returns the reference's NamedTuple.
 """
-function mixed_solve(A::StridedMatrix{Float64}, b::StridedVecOrMat{Float64}; maxiter::Integer=20,
-                     rtol::Real=16eps(Float64), lowtype=Float32, solver::Symbol=:ir, gmres_restart::Integer=0,
+function mixed_solve(A::StridedMatrix{Float64}, b::StridedVecOrMat{Float64}; maxiter::Integer=10,
+                     rtol::Real=1e-13, lowtype=Float32, solver::Symbol=:ir, gmres_restart::Integer=0,
                      factorization::Symbol=:lu, pivot::Symbol=:partial)
     factorization === :lu || throw(ArgumentError("the GPU mixed solve implements factorization=:lu"))
     n = size(A, 1); size(A, 2) == n || throw(DimensionMismatch("mixed factorization dimensions differ"))

@@ -10,7 +10,7 @@ reference's API used by tests and benchmarks.
 from ._lib import load, handle, Handle, OxiError, LIB_PATH, SIGNATURES  # noqa: F401
 from .device import DMatrix, as_dmatrix  # noqa: F401
 from .linalg import (  # noqa: F401
-    gemm_, trsm_, lu, lu_, lu_factor, LU, ldiv, ldiv_, solve, solve_transposed,
+    gemm_, trsm_, laswp_, lu, lu_, lu_factor, LU, ldiv, ldiv_, solve, solve_transposed,
     transpose, adjoint, SingularException, DimensionMismatch,
 )
 from .mixed import (  # noqa: F401
@@ -20,7 +20,7 @@ from .pullbacks import ldiv_pullback, rrule_ldiv, gemm_pullback, rrule_mul  # no
 
 __all__ = [
     "load", "handle", "Handle", "OxiError", "DMatrix", "as_dmatrix",
-    "gemm_", "trsm_", "lu", "lu_", "lu_factor", "LU", "ldiv", "ldiv_", "solve", "solve_transposed",
+    "gemm_", "trsm_", "laswp_", "lu", "lu_", "lu_factor", "LU", "ldiv", "ldiv_", "solve", "solve_transposed",
     "transpose", "adjoint", "SingularException", "DimensionMismatch",
     "mixed_solve", "equilibrate", "sgemm_", "gemm_lowp_update_", "lu_lowp", "LUFactor32", "MixedSolveResult",
     "precision_code", "ldiv_pullback", "rrule_ldiv", "gemm_pullback", "rrule_mul",

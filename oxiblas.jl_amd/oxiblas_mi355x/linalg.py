@@ -115,6 +115,27 @@ def trsm_(B, A, alpha=1.0, side="L", uplo="U", trans="N", unit=False):
     return Bd
 
 
+def laswp_(A, k1, k2, ipiv, inverse=False):
+    """Interchanges k1..k2 (1-based, inclusive) of ``ipiv`` on every column of
+    the device matrix A, in place; in reverse order when ``inverse``
+    (_lu_swap!, src/lapack/factorizations.jl:80-88; _rowswap!,
+    src/api/linalg_api.jl:100-110).  ``ipiv`` is a device int64 tensor or a
+    host array of 1-based global rows."""
+    import torch
+    if not isinstance(A, DMatrix):
+        raise TypeError("laswp_ permutes a device matrix in place")
+    if isinstance(ipiv, torch.Tensor):
+        pv = ipiv.to(device=A.buf.device, dtype=torch.int64).contiguous()
+    else:
+        pv = torch.from_numpy(np.ascontiguousarray(ipiv, dtype=np.int64)).to(A.buf.device)
+    if k2 > pv.numel():
+        raise ValueError("laswp: k2 beyond the pivot vector")
+    h = _h()
+    rc = h._L.oxi_dlaswp(h.ptr, A.n, A.ptr, A.ld, int(k1), int(k2), pv.data_ptr(), 1 if inverse else 0)
+    h.check(rc, "laswp")
+    return A
+
+
 # ---------------------------------------------------------------------------
 # LU
 # ---------------------------------------------------------------------------

@@ -73,10 +73,11 @@ def _singular_from(h):
     return SingularException(int(m.group(1)) if m else 0)
 
 
-def mixed_solve(A, b, maxiter=20, rtol=16 * EPS, lowtype=None, fallback=True, factorization="lu", solver="ir",
+def mixed_solve(A, b, maxiter=10, rtol=1e-13, lowtype=None, fallback=True, factorization="lu", solver="ir",
                 gmres_restart=0, threaded=True, pivot="partial"):
-    """mixed_solve(A, b; maxiter=20, rtol=16eps, lowtype, fallback=true,
-    factorization=:lu) (src/lapack/extended_lapack.jl:175-226).
+    """mixed_solve(A, b; maxiter=10, rtol=1e-13, lowtype, fallback=true,
+    factorization=:lu) (the Float64 method, src/lapack/factorizations.jl:425-427,
+    over src/lapack/extended_lapack.jl:175-226).
 
     Radix equilibration, a low-precision LU, then refinement in FP64 until the
     componentwise backward error is <= rtol.  solver "ir" is the reference's

@@ -209,3 +209,19 @@ def test_tile_kernels_every_entry(gpu, rng, name, m, n, k):
                 assert np.isfinite(got).all(), (name, ta, tb, be)
                 assert np.linalg.norm(got - ref) <= bound, (name, ta, tb, be)
                 np.testing.assert_allclose(got, ref, rtol=3e-12, atol=3e-12 * k)
+
+
+def test_gemm_512_cube_beta_zero(gpu, rng):
+    """The 512^3 FP64 product C = A*B (beta = 0, C overwritten: its NaN
+    contents are never read), all transposes, against the oracle within the
+    reference's tolerance (test/blas/gemm.jl:13) and k*eps*|A||B|."""
+    n = 512
+    for ta in "NT":
+        for tb in "NT":
+            A = rng.standard_normal((n, n)); B = rng.standard_normal((n, n))
+            got = run(gpu, np.full((n, n), np.nan), A, B, 1.0, 0.0, ta, tb)
+            ref = np.asfortranarray(np.full((n, n), np.nan))
+            O.dgemm(ref, A, B, 1.0, 0.0, ta, tb, threads=8)
+            assert np.all(np.isfinite(got))
+            np.testing.assert_allclose(got, ref, rtol=3e-12, atol=3e-12 * n)
+            assert np.linalg.norm(got - ref) <= n * EPS * np.linalg.norm(A) * np.linalg.norm(B)

@@ -13,6 +13,14 @@ pytestmark = pytest.mark.gpu
 EPS = np.finfo(np.float64).eps
 
 
+def solve16(gpu, A, b, **kw):
+    """mixed_solve run to the internal defaults of _mixed_precision_impl
+    (maxiter 20, rtol 16 eps: src/lapack/extended_lapack.jl:175) unless given."""
+    kw.setdefault("maxiter", 20)
+    kw.setdefault("rtol", 16 * EPS)
+    return gpu.mixed_solve(A, b, **kw)
+
+
 def haar(rng, n):
     Q, R = np.linalg.qr(rng.standard_normal((n, n)))
     return Q * np.sign(np.diag(R))
@@ -95,7 +103,16 @@ def test_fp32_lu_matches_oracle(gpu, rng, n):
     assert F.info == 0 and info == 0
     assert np.array_equal(F.pivots, ipo)
     Fh = F.factors.numpy()
-    assert np.abs(Fh - Fo).max() <= 1e-3 * max(1.0, np.abs(Fo).max())
+    # both factors are Float32 eliminations with the same pivots in different
+    # summation orders: they agree to a few n*eps32 of the factor's size, and
+    # each reconstructs P*A to the bound of test/lapack/lapack_dense.jl:11
+    # taken at Float32 eps
+    e32 = float(np.finfo(np.float32).eps)
+    assert np.abs(Fh.astype(np.float64) - Fo).max() <= 4 * n * e32 * max(1.0, np.abs(Fo).max())
+    L = np.tril(Fh.astype(np.float64), -1) + np.eye(n); U = np.triu(Fh.astype(np.float64))
+    p = O.perm_from_ipiv(F.pivots, n)
+    A64 = A.astype(np.float64)
+    assert np.linalg.norm(L @ U - A64[p - 1]) <= 30 * n * e32 * np.linalg.norm(A64)
     b = rng.standard_normal(n).astype(np.float32)
     x = F.solve(b)
     xo = O.sgetrs(Fo, ipo, b)
@@ -172,11 +189,29 @@ def test_lowp_lu_is_a_good_preconditioner(gpu, rng, prec):
 # ---------------------------------------------------------------------------
 # mixed_solve (src/lapack/extended_lapack.jl:175-226)
 # ---------------------------------------------------------------------------
+def test_mixed_solve_defaults_match_oracle(gpu, rng):
+    # the Float64 method's defaults maxiter = 10, rtol = 1e-13
+    # (src/lapack/factorizations.jl:425), on the device and in the oracle
+    for n in (1, 17, 67, 300):
+        A = rng.standard_normal((n, n)) + n * np.eye(n); b = rng.standard_normal(n)
+        r = gpu.mixed_solve(A, b)
+        o = O.mixed_solve(A, b)
+        assert r.converged == o["converged"] and r.converged and r.backward_error <= 1e-13
+        assert abs(r.iterations - o["iterations"]) <= 1 and r.iterations <= 10
+        np.testing.assert_allclose(r.x, o["x"], rtol=1e-11, atol=1e-13)
+    # an iteration cap the solve cannot meet is reported, as in the oracle
+    A = conditioned(rng, 256, 1e6); b = rng.standard_normal(256)
+    r = gpu.mixed_solve(A, b, maxiter=1, rtol=1e-300)
+    o = O.mixed_solve(A, b, maxiter=1, rtol=1e-300)
+    assert not r.converged and not o["converged"] and r.iterations == o["iterations"] == 1
+    assert r.reason == o["reason"] == "iteration_limit"
+
+
 def test_mixed_solve_contract(gpu, rng):
     # test/lapack/factorizations.jl:57-63
     for n in (1, 17, 67, 300):
         A = rng.standard_normal((n, n)) + n * np.eye(n); b = rng.standard_normal(n)
-        r = gpu.mixed_solve(A, b)
+        r = solve16(gpu, A, b)
         assert r.converged and r.iterations <= 10 and not r.fallback and r.reason == "converged"
         assert r.backward_error <= 16 * EPS and r.finite and r.low_type == "Float32"
         np.testing.assert_allclose(A @ r.x, b, rtol=2e-12, atol=2e-12)
@@ -191,7 +226,7 @@ def test_mixed_solve_contract(gpu, rng):
 def test_mixed_solve_fp32_ir_conditioned(gpu, rng, kappa):
     n = 512
     A = conditioned(rng, n, kappa); xt = rng.standard_normal(n); b = A @ xt
-    r = gpu.mixed_solve(A, b, lowtype="float32", solver="ir")
+    r = solve16(gpu, A, b, lowtype="float32", solver="ir")
     assert r.converged and r.backward_error <= 16 * EPS
     assert np.linalg.norm(r.x - xt) / np.linalg.norm(xt) <= 50 * kappa * EPS
     # the componentwise error agrees with the oracle's evaluation of the same x
@@ -203,7 +238,7 @@ def test_mixed_solve_fp32_ir_conditioned(gpu, rng, kappa):
 def test_mixed_solve_gmres_lowp(gpu, rng, prec, kappa):
     n = 768
     A = conditioned(rng, n, kappa); xt = rng.standard_normal(n); b = A @ xt
-    r = gpu.mixed_solve(A, b, lowtype=prec, solver="gmres", maxiter=20, gmres_restart=50)
+    r = solve16(gpu, A, b, lowtype=prec, solver="gmres", maxiter=20, gmres_restart=50)
     assert r.converged and r.backward_error <= 16 * EPS, r
     assert r.solver == "gmres" and r.inner_iterations >= 1
     assert np.linalg.norm(r.x - xt) / np.linalg.norm(xt) <= 100 * kappa * EPS
@@ -213,10 +248,10 @@ def test_mixed_solve_auto_switches(gpu, rng):
     n = 512
     # FP8 factors (u = 2^-4) at kappa = 10: the classical iteration cannot halve the error
     A = conditioned(rng, n, 1e1); b = rng.standard_normal(n)
-    r = gpu.mixed_solve(A, b, lowtype="float8_e4m3fn", solver="auto", maxiter=30, gmres_restart=50)
+    r = solve16(gpu, A, b, lowtype="float8_e4m3fn", solver="auto", maxiter=30, gmres_restart=50)
     assert r.converged and r.solver == "gmres"
     A = conditioned(rng, n, 1e4)
-    r2 = gpu.mixed_solve(A, b, lowtype="float32", solver="auto")
+    r2 = solve16(gpu, A, b, lowtype="float32", solver="auto")
     assert r2.converged and r2.solver == "ir"
 
 
@@ -225,11 +260,11 @@ def test_mixed_solve_multiple_rhs(gpu, rng):
     A = conditioned(rng, n, 1e3)
     for nrhs in (3, 9):
         B = rng.standard_normal((n, nrhs))
-        r = gpu.mixed_solve(A, B)
+        r = solve16(gpu, A, B)
         assert r.converged and r.x.shape == (n, nrhs)
         _, e = O.refinement_residual(A, r.x, B)
         assert e <= 16 * EPS
-        r = gpu.mixed_solve(A, B, lowtype="bfloat16", solver="gmres", maxiter=20)
+        r = solve16(gpu, A, B, lowtype="bfloat16", solver="gmres", maxiter=20)
         assert r.converged
 
 
@@ -242,7 +277,7 @@ def test_mixed_solve_multiple_rhs(gpu, rng):
 @pytest.mark.parametrize("n", [4, 8, 64, 300, 301, 1024])
 def test_mixed_solve_rbt_contract(gpu, rng, n):
     A = rng.standard_normal((n, n)) + n * np.eye(n); b = rng.standard_normal(n)
-    r = gpu.mixed_solve(A, b, pivot="rbt")
+    r = solve16(gpu, A, b, pivot="rbt")
     assert r.converged and not r.fallback and r.backward_error <= 16 * EPS and r.iterations <= 10, r
     o = O.mixed_solve(A, b, maxiter=20, rtol=16 * EPS)
     np.testing.assert_allclose(r.x, o["x"], rtol=1e-12, atol=1e-14)
@@ -255,7 +290,7 @@ def test_mixed_solve_rbt_contract(gpu, rng, n):
 def test_mixed_solve_rbt_conditioned(gpu, rng, prec, solver, kappa):
     n = 1024
     A = conditioned(rng, n, kappa); xt = rng.standard_normal(n); b = A @ xt
-    r = gpu.mixed_solve(A, b, lowtype=prec, solver=solver, maxiter=30, gmres_restart=50, pivot="rbt")
+    r = solve16(gpu, A, b, lowtype=prec, solver=solver, maxiter=30, gmres_restart=50, pivot="rbt")
     assert r.converged and r.backward_error <= 16 * EPS and not r.fallback, r
     assert np.linalg.norm(r.x - xt) / np.linalg.norm(xt) <= 100 * kappa * EPS
     _, e = O.refinement_residual(A, r.x, b)
@@ -269,7 +304,7 @@ def test_mixed_solve_rbt_zero_diagonal(gpu, rng):
     A = 3.0 * np.roll(np.eye(n), 1, axis=1) + 0.1 * rng.standard_normal((n, n))
     np.fill_diagonal(A, 0.0)
     b = rng.standard_normal(n)
-    r = gpu.mixed_solve(A, b, pivot="rbt", solver="gmres")
+    r = solve16(gpu, A, b, pivot="rbt", solver="gmres")
     assert r.converged and r.backward_error <= 16 * EPS, r
 
 
@@ -278,14 +313,14 @@ def test_mixed_solve_rbt_multiple_rhs_and_determinism(gpu, rng):
     A = conditioned(rng, n, 1e3)
     for nrhs in (3, 9):
         B = rng.standard_normal((n, nrhs))
-        r = gpu.mixed_solve(A, B, pivot="rbt")
+        r = solve16(gpu, A, B, pivot="rbt")
         assert r.converged and r.x.shape == (n, nrhs)
         _, e = O.refinement_residual(A, r.x, B)
         assert e <= 16 * EPS
-        r2 = gpu.mixed_solve(A, B, pivot="rbt")
+        r2 = solve16(gpu, A, B, pivot="rbt")
         assert np.array_equal(r.x, r2.x)
     with pytest.raises(ValueError):
-        gpu.mixed_solve(A, B, pivot="complete")
+        solve16(gpu, A, B, pivot="complete")
 
 
 def test_mixed_solve_fallback_and_errors(gpu, rng):
@@ -294,36 +329,36 @@ def test_mixed_solve_fallback_and_errors(gpu, rng):
     A = np.eye(n) * 3.0
     A[10:12, 10:12] = [[1.0, 1.0], [1.0, 1.0 + 1e-12]]
     b = rng.standard_normal(n)
-    r = gpu.mixed_solve(A, b)
+    r = solve16(gpu, A, b)
     assert r.fallback and r.reason == "working_factor_fallback" and r.converged
     _, e = O.refinement_residual(A, r.x, b)     # kappa ~ 4e12: a backward-error check
     assert e <= 16 * EPS
     # exactly singular in both precisions
     S = rng.standard_normal((n, n)); S[:, 7] = 0
     with pytest.raises(gpu.SingularException):
-        gpu.mixed_solve(S, b)
+        solve16(gpu, S, b)
     bad = rng.standard_normal((n, n)); bad[3, 3] = np.nan
     with pytest.raises(ValueError):
-        gpu.mixed_solve(bad, b)
+        solve16(gpu, bad, b)
     with pytest.raises(ValueError):
-        gpu.mixed_solve(A, np.full(n, np.inf))
+        solve16(gpu, A, np.full(n, np.inf))
     with pytest.raises(ValueError):
-        gpu.mixed_solve(A, b, maxiter=-1)
+        solve16(gpu, A, b, maxiter=-1)
     with pytest.raises(gpu.DimensionMismatch):
-        gpu.mixed_solve(rng.standard_normal((4, 5)), np.ones(4))
+        solve16(gpu, rng.standard_normal((4, 5)), np.ones(4))
     # maxiter = 0: the first low-precision solve only, reported as not converged
-    r0 = gpu.mixed_solve(conditioned(rng, 128, 1e6), np.ones(128), maxiter=0)
+    r0 = solve16(gpu, conditioned(rng, 128, 1e6), np.ones(128), maxiter=0)
     assert r0.iterations == 0 and (r0.converged == (r0.backward_error <= 16 * EPS))
     # empty
-    r = gpu.mixed_solve(np.zeros((0, 0)), np.zeros(0))
+    r = solve16(gpu, np.zeros((0, 0)), np.zeros(0))
     assert r.converged and r.x.shape == (0,)
 
 
 def test_mixed_solve_deterministic(gpu, rng):
     n = 1000
     A = conditioned(rng, n, 1e4); b = rng.standard_normal(n)
-    r1 = gpu.mixed_solve(A, b, lowtype="float16", solver="gmres")
-    r2 = gpu.mixed_solve(A, b, lowtype="float16", solver="gmres")
+    r1 = solve16(gpu, A, b, lowtype="float16", solver="gmres")
+    r2 = solve16(gpu, A, b, lowtype="float16", solver="gmres")
     assert np.array_equal(r1.x, r2.x) and r1.backward_error == r2.backward_error
 
 
@@ -331,7 +366,7 @@ def test_mixed_solve_large(gpu, rng):
     n = 4096
     A = rng.standard_normal((n, n)); xt = rng.standard_normal(n); b = A @ xt
     for prec, solver in (("float32", "ir"), ("bfloat16", "gmres"), ("float16", "auto")):
-        r = gpu.mixed_solve(A, b, lowtype=prec, solver=solver, maxiter=30)
+        r = solve16(gpu, A, b, lowtype=prec, solver=solver, maxiter=30)
         assert r.converged and r.backward_error <= 16 * EPS, (prec, solver, r.backward_error)
 
 

@@ -315,3 +315,119 @@ def test_mixed_solve_contract(rng):
     r = O.mixed_solve(A, b)
     assert r["fallback"] and r["reason"] == "working_factor_fallback"
     np.testing.assert_allclose(A @ r["x"], b, rtol=1e-9)
+
+
+# ---------------------------------------------------------------------------
+# Float32 oracle functions (the checker of the FP32 factor of the mixed solve)
+# ---------------------------------------------------------------------------
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+@pytest.mark.parametrize("m,n,bs", LU_CASES + ((600, 600, 128),))
+def test_sgetrf_matches_lapack_sgetrf(rng, m, n, bs):
+    # Float32 lu_factor (src/lapack/factorizations.jl:17-77 at T = Float32):
+    # the pivots of LAPACK sgetrf (scipy on float32 input) and the
+    # reconstruction bound of test/lapack/lapack_dense.jl:11 at Float32 eps
+    A = rng.standard_normal((m, n)).astype(np.float32)
+    F, ipiv, info = O.sgetrf(A, bs)
+    assert info == 0 and F.dtype == np.float32
+    k = min(m, n)
+    L = np.tril(F[:, :k].astype(np.float64), -1) + np.eye(m, k)
+    U = np.triu(F[:k, :].astype(np.float64))
+    p = O.perm_from_ipiv(ipiv, m)
+    r = np.linalg.norm(L @ U - A[p - 1].astype(np.float64)) / (np.linalg.norm(A) * max(m, n) * EPS32)
+    assert r <= 30
+    lapack_lu, lapack_piv = sl.lu_factor(A, check_finite=False)
+    assert lapack_lu.dtype == np.float32            # really sgetrf
+    assert np.array_equal(ipiv, lapack_piv[:k] + 1)
+    # the factors agree with sgetrf's to the rounding of a Float32 elimination
+    assert np.abs(F.astype(np.float64) - lapack_lu).max() <= 64 * max(m, n) * EPS32 * np.abs(lapack_lu).max()
+
+
+def test_sgetrf_singular_column_info(rng):
+    A = rng.standard_normal((40, 40)).astype(np.float32)
+    A[:, 17] = 0
+    F, ipiv, info = O.sgetrf(A, 16, check=False)
+    assert info == 18
+
+
+@pytest.mark.parametrize("n", [1, 9, 64, 129, 300])
+def test_sgetrs_backward_error(rng, n):
+    # Float32 solve(F, b) (src/lapack/factorizations.jl:177-193): normwise
+    # backward error at Float32 eps, both transposes, vector and matrix forms,
+    # and agreement with LAPACK sgetrs on sgetrf's own factors
+    A = rng.standard_normal((n, n)).astype(np.float32)
+    F, ipiv, _ = O.sgetrf(A, 32)
+    b = rng.standard_normal(n).astype(np.float32)
+    B = rng.standard_normal((n, 4)).astype(np.float32)
+    A64 = A.astype(np.float64)
+    for trans, opA in (("N", A64), ("T", A64.T)):
+        for rhs in (b, B):
+            x = O.sgetrs(F, ipiv, rhs, trans)
+            assert x.dtype == np.float32
+            x64 = x.astype(np.float64)
+            res = np.linalg.norm(opA @ x64 - rhs.astype(np.float64))
+            assert res <= 30 * n * EPS32 * np.linalg.norm(A64) * np.linalg.norm(x64) + 1e-30
+            ref = sl.lu_solve((F, ipiv - 1), rhs, trans=0 if trans == "N" else 1, check_finite=False)
+            assert ref.dtype == np.float32
+            cond = np.linalg.cond(A64)
+            np.testing.assert_allclose(x64, ref.astype(np.float64), rtol=0,
+                                       atol=64 * n * EPS32 * cond * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("ta", "NT")
+@pytest.mark.parametrize("tb", "NT")
+def test_sgemm_vs_float64_product(rng, ta, tb):
+    # Float32 gemm! (src/blas/gemm.jl:625-703) on the 16x16 Float32 tile
+    # (edges at 15, 16, 17, 31, 33 rows/columns) and across the kc = 896 block
+    # boundary of K: normwise error <= k eps32 |A||B| against the exact
+    # (float64) product, and bit-exact on small-integer inputs
+    for (m, n, k) in ((15, 16, 17), (16, 16, 16), (17, 31, 33), (33, 17, 895), (31, 33, 897), (100, 70, 1800)):
+        A = rng.standard_normal((m, k) if ta == "N" else (k, m)).astype(np.float32)
+        B = rng.standard_normal((k, n) if tb == "N" else (n, k)).astype(np.float32)
+        C0 = rng.standard_normal((m, n)).astype(np.float32)
+        C = np.asfortranarray(C0.copy())
+        O.sgemm(C, A, B, 1.25, -0.5, ta, tb)
+        exact = 1.25 * (op(A, ta).astype(np.float64) @ op(B, tb).astype(np.float64)) - 0.5 * C0.astype(np.float64)
+        err = np.linalg.norm(C.astype(np.float64) - exact)
+        bound = (k + 2) * EPS32 * (1.25 * np.linalg.norm(A.astype(np.float64)) * np.linalg.norm(B.astype(np.float64))
+                                   + 0.5 * np.linalg.norm(C0.astype(np.float64)))
+        assert err <= bound
+        Ai = rng.integers(-8, 9, A.shape).astype(np.float32)
+        Bi = rng.integers(-8, 9, B.shape).astype(np.float32)
+        Ci = np.asfortranarray(rng.integers(-8, 9, (m, n)).astype(np.float32))
+        expect = 2.0 * (op(Ai, ta).astype(np.float64) @ op(Bi, tb).astype(np.float64)) - 3.0 * Ci.astype(np.float64)
+        O.sgemm(Ci, Ai, Bi, 2.0, -3.0, ta, tb, threads=3)
+        assert np.array_equal(Ci.astype(np.float64), expect)
+
+
+def test_sgemm_no_read_rules(rng):
+    A = rng.standard_normal((17, 13)).astype(np.float32); B = rng.standard_normal((13, 19)).astype(np.float32)
+    C = np.asfortranarray(np.full((17, 19), np.nan, dtype=np.float32))
+    O.sgemm(C, A, B, 1.0, 0.0)
+    assert np.all(np.isfinite(C))
+    C = np.asfortranarray(np.full((9, 7), 2.0, dtype=np.float32))
+    O.sgemm(C, np.full((9, 3), np.nan, np.float32), np.full((3, 7), np.nan, np.float32), 0.0, 0.5)
+    assert np.array_equal(C, np.full((9, 7), 1.0, dtype=np.float32))
+
+
+def test_dlaswp_closed_form(rng):
+    # interchanges in order give A[p, :] with p = perm_from_ipiv (the
+    # permutation of factorizations.jl:72-75); in reverse order its inverse
+    m, ncols = 700, 5
+    ipiv = np.array([rng.integers(j, m) + 1 for j in range(m)], dtype=np.int64)
+    A = np.asfortranarray(rng.standard_normal((m, ncols)))
+    p = O.perm_from_ipiv(ipiv, m)
+    fwd = O.dlaswp(A.copy(order="F"), 1, m, ipiv)
+    assert np.array_equal(fwd, A[p - 1])
+    back = O.dlaswp(fwd.copy(order="F"), 1, m, ipiv, inverse=True)
+    assert np.array_equal(back, A)
+    # a sub-range k1..k2 equals the permutation of those steps alone
+    sub = ipiv.copy(); sub[:99] = np.arange(1, 100); sub[400:] = np.arange(401, m + 1)
+    part = O.dlaswp(A.copy(order="F"), 100, 400, ipiv)
+    assert np.array_equal(part, A[O.perm_from_ipiv(sub, m) - 1])
+    # a dropped step or a reversed order is caught
+    assert not np.array_equal(O.dlaswp(A.copy(order="F"), 2, m, ipiv), fwd)
+    assert not np.array_equal(O.dlaswp(A.copy(order="F"), 1, m, ipiv, inverse=True), fwd)
+    with pytest.raises(ValueError):
+        O.dlaswp(A.copy(order="F"), 1, 3, np.array([1, 2, m + 1]))
