@@ -43,7 +43,8 @@ enum {
     OXI_ERR_SINGULAR = 4,     /* zero on a non-unit triangular diagonal (trsm/trsv/getrs) */
     OXI_ERR_TIMEOUT = 5,      /* a device-side barrier gave up (never expected) */
     OXI_ERR_NOT_FINITE = 6,   /* non-finite input where finite input is required */
-    OXI_ERR_OVERFLOW = 7      /* low-precision conversion overflowed */
+    OXI_ERR_OVERFLOW = 7,     /* low-precision conversion overflowed */
+    OXI_ERR_NOT_POSDEF = 8    /* not positive definite (PosDefException): the message names the column */
 };
 
 /* ---- handle, stream and memory ----------------------------------------- */
@@ -70,6 +71,37 @@ int oxi_memcpy(oxi_handle_t handle, void* dst, const void* src, uint64_t bytes, 
  * (src/blas/gemm.jl:682-688).  FP64 MFMA (v_mfma_f64_16x16x4). */
 int oxi_dgemm(oxi_handle_t handle, char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
               const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc);
+
+/* gemm_strided_batched!(C, A, B, alpha, beta; transA, transB)
+ * (src/blas/batched.jl:80-100): `batch` independent products
+ * C_z <- alpha*op(A_z)*op(B_z) + beta*C_z with A_z = A + z*strideA (elements),
+ * B_z and C_z likewise, in one launch (grid.y = z).  Outputs must be disjoint
+ * (strideC >= the span of one C; -17 otherwise, batched.jl:104-117); an A or B
+ * overlapping the outputs is copied first (:87-88).  The per-product rules
+ * of oxi_dgemm apply (beta == 0 never reads C). */
+int oxi_dgemm_strided_batched(oxi_handle_t handle, char transA, char transB, int64_t m, int64_t n, int64_t k,
+                              double alpha, const double* A, int64_t lda, int64_t strideA, const double* B,
+                              int64_t ldb, int64_t strideB, double beta, double* C, int64_t ldc, int64_t strideC,
+                              int64_t batch);
+
+/* gemm_batched!(Cs, As, Bs, alpha, beta; transA, transB)
+ * (src/blas/batched.jl:49-78) for a batch of one shape: A, B, C are HOST
+ * arrays of `batch` device pointers.  Outputs must be pairwise disjoint
+ * (-13 otherwise, :66-68); an operand overlapping any output is copied first
+ * (:58-65).  One launch (grid.y = product); synchronises the stream.  A batch
+ * of several shapes is one call per shape. */
+int oxi_dgemm_batched(oxi_handle_t handle, char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
+                      const double* const* A, int64_t lda, const double* const* B, int64_t ldb, double beta,
+                      double* const* C, int64_t ldc, int64_t batch);
+
+/* syrk!(C, A, alpha, beta; uplo, trans) and herk! for Float64
+ * (src/blas/level3.jl:546-783): C[tri] <- alpha*op(A)*op(A)^T + beta*C[tri] on
+ * the triangle `uplo` ('U'/'L') only, the other triangle untouched; trans 'N':
+ * A is n x k, 'T'/'C': A is k x n.  alpha == 0 or k == 0 scales the triangle;
+ * beta == 0 never reads C (_triangle_scale!, :546-553).  FP64 MFMA; tiles
+ * outside the triangle do no work. */
+int oxi_dsyrk(oxi_handle_t handle, char uplo, char trans, int64_t n, int64_t k, double alpha, const double* A,
+              int64_t lda, double beta, double* C, int64_t ldc);
 
 /* trsm!(B, A, alpha; side, uplo, trans, unit)  (src/blas/level3.jl:518-544):
  * B <- alpha*op(A)^-1*B (side 'L') or alpha*B*op(A)^-1 (side 'R').  A zero on a
@@ -105,6 +137,102 @@ int oxi_dgetrf(oxi_handle_t handle, int64_t m, int64_t n, double* A, int64_t lda
 int oxi_dgetrs(oxi_handle_t handle, char trans, int64_t n, int64_t nrhs, const double* LU, int64_t lda,
                const int64_t* ipiv, double* B, int64_t ldb);
 
+/* ---- what is built on the LU factors -------------------------------------- */
+/* LinearAlgebra.logabsdet(F::LU) / logabsdet_factor (src/api/linalg_api.jl:185-191,
+ * src/lapack/dense_utilities.jl:177-187) on the factors of oxi_dgetrf:
+ * *logabs = sum log|u_ii| (-Inf with a zero pivot), *sign = the product of the
+ * pivots' signs, negated per interchange (0 with a zero pivot); det =
+ * sign * exp(logabs).  One device reduction; synchronises. */
+int oxi_dlogabsdet(oxi_handle_t handle, int64_t n, const double* LU, int64_t lda, const int64_t* ipiv,
+                   double* logabs, double* sign);
+
+/* native_inverse(A) from its LU factors (src/lapack/dense_utilities.jl:14-36):
+ * Ainv <- inv(U) inv(L) P, i.e. U^-1 by a left triangular solve, times L^-1 by
+ * a right solve with the unit lower factor, then the column permutation.
+ * Ainv must not overlap the factors (-6).  A zero on U's diagonal returns
+ * OXI_ERR_SINGULAR (SingularException).  Uses n*n doubles of handle workspace. */
+int oxi_dgetri(oxi_handle_t handle, int64_t n, const double* LU, int64_t lda, const int64_t* ipiv, double* Ainv,
+               int64_t ldainv);
+
+/* backward_error(A, x, b)  (src/lapack/dense_utilities.jl:214-228): R = B - A X
+ * (A m x n, X n x nrhs, B m x nrhs) and *error = max_ij |r_ij| / (|A||X| + |B|)_ij
+ * (0/0 = 0, r/0 = Inf, a non-finite denominator Inf; NaN propagates).  R may be
+ * NULL (the residual is then not returned). */
+int oxi_dbackward_error(oxi_handle_t handle, int64_t m, int64_t n, int64_t nrhs, const double* A, int64_t lda,
+                        const double* X, int64_t ldx, const double* B, int64_t ldb, double* R, int64_t ldr,
+                        double* error);
+
+/* inverse_norm_estimate(F, T, n; maxiter)  (src/lapack/expert_refinement.jl:45-60):
+ * the Hager-Higham estimate of ||A^-1||_1 from solves with the factors of
+ * oxi_dgetrf and their transpose (an estimate, not a bound).  maxiter >= 1. */
+int oxi_dinverse_norm_estimate(oxi_handle_t handle, int64_t n, const double* LU, int64_t lda, const int64_t* ipiv,
+                               int32_t maxiter, double* estimate);
+
+typedef struct {
+    int64_t iterations;             /* refinement steps taken */
+    int32_t converged;              /* backward_error <= rtol */
+    int32_t reserved;
+    double backward_error;          /* componentwise, as _refinement_residual! */
+    double residual_norm;           /* nrm2 of the last residual */
+    double rcond_estimate;          /* min(1, 1 / (||A||_1 * estimate of ||A^-1||_1)); refine_solve only */
+    double forward_error_estimate;  /* delta / (1 - delta), delta = est ||A^-1||_1 ||R||_1 / ||X||_1; refine_solve only */
+} oxi_refine_info;
+
+/* refine_solve(A, F, B; maxiter, rtol)  (src/lapack/expert_refinement.jl:89-104):
+ * X = solve(F, B), then X += solve(F, B - A X) until the componentwise backward
+ * error is <= rtol or maxiter steps; then the condition and forward-error
+ * estimates.  F are the factors (LU, ipiv) of oxi_dgetrf of A.  The reference's
+ * defaults are maxiter = 8, rtol = 8 eps. */
+int oxi_drefine_solve(oxi_handle_t handle, int64_t n, int64_t nrhs, const double* A, int64_t lda, const double* LU,
+                      int64_t ldlu, const int64_t* ipiv, const double* B, int64_t ldb, double* X, int64_t ldx,
+                      int32_t maxiter, double rtol, oxi_refine_info* info);
+
+/* expert_solve(A, B; maxiter, rtol, equilibrate)
+ * (src/lapack/advanced_factorizations.jl:591-645): with equilibrate != 0, rows
+ * are divided by their largest |entry| (a zero row returns OXI_ERR_SINGULAR,
+ * SingularException(i)) and then columns by theirs (1 when zero); the scaled
+ * matrix is factored into LU/ipiv (caller storage, the NamedTuple's `factor`),
+ * X = solve(F, B ./ rows) ./ cols, and X is refined against the original A, B
+ * with corrections solve(F, R ./ rows) ./ cols.  rows, cols: device arrays of
+ * n doubles (the returned scales).  Defaults of the reference: maxiter = 8,
+ * rtol = 8 eps, equilibrate = true. */
+int oxi_dexpert_solve(oxi_handle_t handle, int64_t n, int64_t nrhs, const double* A, int64_t lda, const double* B,
+                      int64_t ldb, double* X, int64_t ldx, int32_t maxiter, double rtol, int32_t equilibrate,
+                      double* LU, int64_t ldlu, int64_t* ipiv, double* rows, double* cols, oxi_refine_info* info);
+
+/* ---- Cholesky ----------------------------------------------------------- */
+/* _cholesky_factor!(L) / cholesky_factor(A)  (src/lapack/factorizations.jl:206-286):
+ * A (n x n, device) is overwritten by its lower Cholesky factor L (A = L L^T)
+ * read from the lower triangle only; uplo must be 'L' (the reference factors
+ * the lower triangle).  *info (host) receives 0, or the 1-based column of the
+ * first non-positive or non-finite pivot (PosDefException(info) in the
+ * binding; the factor is then partial).  On success the strict upper
+ * triangle is zeroed (:256-260).  Recursive potrf2: 64-column leaves in LDS,
+ * right-side TRSM and lower SYRK on the matrix cores. */
+int oxi_dpotrf(oxi_handle_t handle, char uplo, int64_t n, double* A, int64_t lda, int64_t* info);
+
+/* solve(F::CholeskyFactor, B)  (src/lapack/factorizations.jl:288-300):
+ * B <- (L L^T)^-1 B in place with the factor of oxi_dpotrf.  A zero on L's
+ * diagonal returns OXI_ERR_SINGULAR with B untouched. */
+int oxi_dpotrs(oxi_handle_t handle, char uplo, int64_t n, int64_t nrhs, const double* L, int64_t ldl, double* B,
+               int64_t ldb);
+
+/* Solve plans: ldiv!(F, B) repeated on the same factors (the LU object keeps
+ * its factors resident, src/api/linalg_api.jl:168-184).  A plan holds what
+ * every solve with these factors prepares -- the row permutation on the
+ * device, the first zero pivot, and the inverses of the 128 x 128 diagonal
+ * blocks of both triangles (4 * n * 128 doubles for 'N' and 'T' solves, each
+ * kind made by its first solve with at most 4 right-hand sides) -- so later
+ * solves only run the solve.  LU and ipiv must stay unchanged while the plan
+ * lives; a plan is used by one handle at a time; oxi_dgetrs_plan_destroy
+ * frees it.  oxi_dgetrs_plan returns what oxi_dgetrs returns
+ * (OXI_ERR_SINGULAR for a zero pivot). */
+typedef struct oxi_getrs_plan_s* oxi_getrs_plan_t;
+int oxi_dgetrs_plan_create(oxi_handle_t handle, int64_t n, const double* LU, int64_t lda, const int64_t* ipiv,
+                           oxi_getrs_plan_t* plan);
+int oxi_dgetrs_plan(oxi_handle_t handle, oxi_getrs_plan_t plan, char trans, int64_t nrhs, double* B, int64_t ldb);
+int oxi_dgetrs_plan_destroy(oxi_handle_t handle, oxi_getrs_plan_t plan);
+
 /* ---- adjoints ------------------------------------------------------------ */
 /* Pullback of X = A \ B given the factors of oxi_dgetrf and the cotangent
  * Xbar: Bbar = A^-T Xbar (the transposed solve ldiv!(transpose(F), .),
@@ -133,6 +261,15 @@ int oxi_sgemm(oxi_handle_t handle, char transA, char transB, int64_t m, int64_t 
  * counterpart: the building block of the mixed form below. */
 int oxi_gemm_lowp_update(oxi_handle_t handle, int prec, int64_t m, int64_t n, int64_t k, float alpha,
                          const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc);
+
+/* gemm! for Float16  (src/blas/extended_blas.jl:39-88): A, B and C hold IEEE
+ * binary16 values (column-major); each element of op(A)*op(B) accumulates in
+ * Float32 over all of K (FP16 MFMA, FP32 accumulation) and is rounded to
+ * Float16 once: C = Float16(a*acc + b*Float32(C)), a and b being alpha and beta
+ * rounded to Float16 first.  beta == 0 never reads C; alpha == 0 or k == 0
+ * scales C only.  A or B overlapping C is read before C is written. */
+int oxi_hgemm(oxi_handle_t handle, char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha,
+              const void* A, int64_t lda, const void* B, int64_t ldb, float beta, void* C, int64_t ldc);
 
 /* FP32 blocked LU with partial pivoting whose trailing updates run in `prec`
  * (the low-precision factor of mixed_solve, src/lapack/extended_lapack.jl:183-198,
@@ -184,6 +321,20 @@ typedef struct {
 int oxi_dgesv_mixed(oxi_handle_t handle, int prec, int solver, int64_t n, int64_t nrhs, const double* A,
                     int64_t lda, const double* B, int64_t ldb, double* X, int64_t ldx, int32_t maxiter, double rtol,
                     int32_t gmres_restart, int32_t pivot, oxi_ir_info* info);
+
+/* mixed_solve(A, B; factorization=:cholesky)  (src/lapack/extended_lapack.jl:175-226,
+ * _mixed_factor :161): A (symmetric positive definite, lower triangle read by
+ * the factorization, all of A by the residual) divided by its largest |entry|
+ * (rows = that scale, cols = 1, :187-189), the FP32 Cholesky of the lower
+ * triangle (oxi_dpotrf's recursion in FP32), falling back to FP64 when the
+ * FP32 factor meets a non-positive pivot, then the reference's correction
+ * step with L and L^T solves and classical refinement in FP64 until the
+ * componentwise backward error is <= rtol.  prec must be OXI_PREC_FP32 (the
+ * reference's low type).  OXI_ERR_NOT_POSDEF when the FP64 factor fails too
+ * (PosDefException), OXI_ERR_NOT_FINITE for non-finite input. */
+int oxi_dposv_mixed(oxi_handle_t handle, int prec, int64_t n, int64_t nrhs, const double* A, int64_t lda,
+                    const double* B, int64_t ldb, double* X, int64_t ldx, int32_t maxiter, double rtol,
+                    oxi_ir_info* info);
 
 /* equilibrate(A; radix)  (src/lapack/dense_utilities.jl:191-212): row scales
  * rows[i] = max_j |A[i,j]| (1 for a zero row), then column scales of A ./ rows,

@@ -9,5 +9,7 @@ baseline.  The product library never imports it.
 from .oxo import (  # noqa: F401
     available, build, lib, max_threads,
     dgemm, sgemm, dtrsm, dtrsv, dgetrf, sgetrf, dgetrs, sgetrs, dlaswp,
+    dsyrk, dpotrf, dpotrs, dgetri, dlogabsdet, backward_error, inverse_norm_estimate, refine_solve,
+    expert_solve, hgemm,
     dequilibrate, refinement_residual, mixed_solve, perm_from_ipiv,
 )

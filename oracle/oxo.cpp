@@ -647,9 +647,285 @@ double refinement_residual(i64 n, i64 nrhs, const double* A, i64 lda, const doub
     return err;
 }
 
+// syrk!/herk! on one triangle (src/blas/level3.jl:546-783) for real data:
+// C[i,j] <- alpha * sum_p op(A)[i,p] op(A)[j,p] + beta * C[i,j] on the
+// triangle, by the small-case formula of _triangle_leaf! (:555-568): the
+// products accumulated in order of p with muladd, then alpha, then beta * C
+// (not read when beta == 0); alpha == 0 or k == 0 scales the triangle
+// (_triangle_scale!, :546-553).
+template <class T>
+int syrk(char uplo, char trans, i64 n, i64 k, T alpha, const T* A, i64 lda, T beta, T* C, i64 ldc) {
+    const bool upper = uplo == 'U' || uplo == 'u';
+    if (!upper && !(uplo == 'L' || uplo == 'l')) return -1;
+    const bool tr = trans == 'T' || trans == 't' || trans == 'C' || trans == 'c';
+    if (!tr && !(trans == 'N' || trans == 'n')) return -2;
+    if (n < 0) return -3;
+    if (k < 0) return -4;
+    if (lda < std::max<i64>(1, tr ? k : n)) return -7;
+    if (ldc < std::max<i64>(1, n)) return -10;
+    auto a = [&](i64 i, i64 p) { return tr ? A[p + i * lda] : A[i + p * lda]; };
+    for (i64 j = 0; j < n; ++j)
+        for (i64 i = upper ? 0 : j; i < (upper ? j + 1 : n); ++i) {
+            if (alpha == T(0) || k == 0) {
+                C[i + j * ldc] = beta == T(0) ? T(0) : beta * C[i + j * ldc];
+                continue;
+            }
+            T acc = 0;
+            for (i64 p = 0; p < k; ++p) acc = std::fma(a(i, p), a(j, p), acc);
+            T v = alpha * acc;
+            if (beta != T(0)) v += beta * C[i + j * ldc];
+            C[i + j * ldc] = v;
+        }
+    return 0;
+}
+
+// _cholesky_recursive! (src/lapack/factorizations.jl:263-286): leaves of at
+// most `leaf` columns right-looking (d = sqrt(a_cc), l_ic = a_ic / d,
+// a_ij = muladd(-l_ic, l_jc, a_ij)), otherwise L11, L21 = A21 L11^-T (trsm!
+// side R, lower, trans C), A22 -= L21 L21^T (herk! lower), L22.  Returns 0
+// or the global 1-based column of the first pivot that is not positive and
+// finite.
+template <class T>
+i64 chol_rec(T* L, i64 ldl, i64 n, i64 offset, i64 leaf, int threads) {
+    if (n <= leaf) {
+        for (i64 col = 0; col < n; ++col) {
+            T d = L[col + col * ldl];
+            if (!(std::isfinite(d) && d > 0)) return offset + col + 1;
+            d = std::sqrt(d);
+            L[col + col * ldl] = d;
+            for (i64 i = col + 1; i < n; ++i) L[i + col * ldl] /= d;
+            for (i64 j = col + 1; j < n; ++j) {
+                const T c = L[j + col * ldl];
+                for (i64 i = j; i < n; ++i) L[i + j * ldl] = std::fma(-L[i + col * ldl], c, L[i + j * ldl]);
+            }
+        }
+        return 0;
+    }
+    const i64 h = n / 2;
+    i64 info = chol_rec(L, ldl, h, offset, leaf, threads);
+    if (info) return info;
+    trsm<T>('R', 'L', 'C', 'N', n - h, h, T(1), L, ldl, L + h, ldl, 8, threads);
+    syrk<T>('L', 'N', n - h, h, T(-1), L + h, ldl, T(1), L + h + h * ldl, ldl);
+    return chol_rec(L + h + h * ldl, ldl, n - h, offset + h, leaf, threads);
+}
+
 }  // namespace
 
 extern "C" {
+
+// native_inverse (src/lapack/dense_utilities.jl:14-36) from the factors:
+// X = inv(U) by the column formula of _trtri!'s leaf (:94-105, applied to
+// the whole triangle: the recursion's TRMM blocking only reorders sums),
+// X <- X inv(L) (trsm! side R, lower, unit), columns placed by the
+// permutation: result[:, p[k]] = X[:, k].  Returns 0 or the 1-based index
+// of a zero pivot (SingularException).
+int oxo_dgetri(i64 n, const double* LU, i64 lda, const i64* ipiv, double* out, i64 ldo, int threads) {
+    if (n < 0) return -1;
+    for (i64 j = 0; j < n; ++j) if (LU[j + j * lda] == 0.0) return int(j + 1);
+    std::vector<double> X(size_t(n) * n, 0.0);
+    for (i64 j = 0; j < n; ++j) for (i64 i = 0; i <= j; ++i) X[i + j * n] = LU[i + j * lda];
+    for (i64 j = 0; j < n; ++j) {
+        const double d = (X[j + j * n] = 1.0 / X[j + j * n]);
+        for (i64 i = 0; i < j; ++i) {
+            double sacc = X[i + i * n] * X[i + j * n];
+            for (i64 k = i + 1; k < j; ++k) sacc += X[i + k * n] * X[k + j * n];
+            X[i + j * n] = -sacc * d;
+        }
+    }
+    if (n > 1) trsm<double>('R', 'L', 'N', 'U', n, n, 1.0, LU, lda, X.data(), n, 8, threads);
+    std::vector<i64> perm(n);
+    for (i64 i = 0; i < n; ++i) perm[i] = i;
+    for (i64 j = 0; j < n; ++j) std::swap(perm[j], perm[ipiv[j] - 1]);
+    for (i64 k = 0; k < n; ++k) for (i64 i = 0; i < n; ++i) out[i + perm[k] * ldo] = X[i + k * n];
+    return 0;
+}
+
+// LinearAlgebra.logabsdet(F::LU) (src/api/linalg_api.jl:185-191): sum of
+// log|u_ii| in order, sign = prod sign(u_ii), negated per interchange.
+void oxo_dlogabsdet(i64 n, const double* LU, i64 lda, const i64* ipiv, double* logabs, double* sign) {
+    double d = 0, sg = 1;
+    for (i64 i = 0; i < n; ++i) {
+        const double u = LU[i + i * lda];
+        d += std::log(std::fabs(u));
+        sg *= u > 0 ? 1.0 : u < 0 ? -1.0 : 0.0;
+        if (ipiv[i] != i + 1) sg = -sg;
+    }
+    *logabs = d;
+    *sign = sg == 0 ? 0.0 : sg;   // no negative zero
+}
+
+// backward_error (src/lapack/dense_utilities.jl:214-228): R = B - A X and the
+// largest componentwise ratio (sums in order of k).
+double oxo_dbackward_error(i64 m, i64 n, i64 nrhs, const double* A, i64 lda, const double* X, i64 ldx,
+                           const double* B, i64 ldb, double* R, i64 ldr) {
+    double err = 0;
+    for (i64 j = 0; j < nrhs; ++j)
+        for (i64 i = 0; i < m; ++i) {
+            double value = 0, den = std::fabs(B[i + j * ldb]);
+            for (i64 k = 0; k < n; ++k) {
+                value += A[i + k * lda] * X[k + j * ldx];
+                den += std::fabs(A[i + k * lda]) * std::fabs(X[k + j * ldx]);
+            }
+            const double r = B[i + j * ldb] - value;
+            R[i + j * ldr] = r;
+            const double rel = !std::isfinite(den) ? INFINITY : den == 0 ? (r == 0 ? 0.0 : INFINITY) : std::fabs(r) / den;
+            if (std::isnan(rel) || std::isnan(err)) err = NAN; else err = std::max(err, rel);
+        }
+    return err;
+}
+
+// inverse_norm_estimate (src/lapack/expert_refinement.jl:45-60) with the LU
+// factors: solve(F, x) and solve_transposed(F, signs).
+double oxo_dinverse_norm_estimate(i64 n, const double* LU, i64 lda, const i64* ipiv, int maxiter) {
+    if (n == 0) return 0.0;
+    std::vector<double> x(n, 1.0 / double(n)), y(n), z(n);
+    double estimate = 0;
+    i64 previous = -1;
+    for (int it = 0; it < maxiter; ++it) {
+        y = x;
+        getrs<double>('N', n, 1, LU, lda, ipiv, y.data(), n, 1, 1);
+        double next = 0;
+        for (i64 i = 0; i < n; ++i) next += std::fabs(y[i]);
+        estimate = std::max(estimate, next);
+        for (i64 i = 0; i < n; ++i) z[i] = y[i] == 0 ? 1.0 : (y[i] > 0 ? 1.0 : -1.0);
+        getrs<double>('C', n, 1, LU, lda, ipiv, z.data(), n, 1, 1);
+        i64 index = 0;
+        double zmax = -1, dot = 0;
+        for (i64 i = 0; i < n; ++i) {
+            if (std::fabs(z[i]) > zmax) { zmax = std::fabs(z[i]); index = i; }
+            dot += z[i] * x[i];
+        }
+        if (zmax <= dot) break;
+        if (index == previous) break;
+        std::fill(x.begin(), x.end(), 0.0);
+        x[index] = 1.0;
+        previous = index;
+    }
+    for (i64 i = 0; i < n; ++i) x[i] = ((i % 2 == 0) ? 1.0 : -1.0) * (1.0 + double(i) / double(std::max<i64>(1, n - 1)));
+    getrs<double>('N', n, 1, LU, lda, ipiv, x.data(), n, 1, 1);
+    double alt = 0;
+    for (i64 i = 0; i < n; ++i) alt += std::fabs(x[i]);
+    return std::max(estimate, 2.0 * alt / (3.0 * double(n)));
+}
+
+// refine_solve (src/lapack/expert_refinement.jl:89-104).  out: [iterations,
+// converged, backward_error, residual_norm, rcond_estimate, forward_error_estimate].
+int oxo_drefine_solve(i64 n, i64 nrhs, const double* A, i64 lda, const double* LU, i64 ldlu, const i64* ipiv,
+                      const double* B, i64 ldb, double* X, i64 ldx, int maxiter, double rtol, double* out) {
+    if (n < 0 || nrhs < 0 || maxiter < 0 || !(rtol >= 0)) return -1;
+    const bool vec = nrhs == 1;
+    for (i64 j = 0; j < nrhs; ++j) for (i64 i = 0; i < n; ++i) X[i + j * ldx] = B[i + j * ldb];
+    getrs<double>('N', n, nrhs, LU, ldlu, ipiv, X, ldx, vec, 1);
+    std::vector<double> R(size_t(n) * nrhs), C(size_t(n) * nrhs);
+    double err = INFINITY;
+    int it = 0;
+    for (int iteration = 0; iteration <= maxiter; ++iteration) {
+        err = oxo_dbackward_error(n, n, nrhs, A, lda, X, ldx, B, ldb, R.data(), n);
+        it = iteration;
+        if (err <= rtol || iteration == maxiter) break;
+        C = R;
+        getrs<double>('N', n, nrhs, LU, ldlu, ipiv, C.data(), n, vec, 1);
+        for (i64 j = 0; j < nrhs; ++j) for (i64 i = 0; i < n; ++i) X[i + j * ldx] += C[i + j * n];
+    }
+    const double inv_norm = oxo_dinverse_norm_estimate(n, LU, ldlu, ipiv, 8);
+    auto norm1 = [&](const double* M, i64 ld, i64 cols) {
+        double best = 0;
+        for (i64 j = 0; j < cols; ++j) {
+            double sacc = 0;
+            for (i64 i = 0; i < n; ++i) sacc += std::fabs(M[i + j * ld]);
+            best = std::max(best, sacc);
+        }
+        return best;
+    };
+    double rn = 0;
+    for (double v : R) rn += v * v;
+    rn = std::sqrt(rn);
+    const double anorm = norm1(A, lda, n), condition = anorm * inv_norm;
+    const double xnorm = norm1(X, ldx, nrhs);
+    const double delta = xnorm == 0 ? (rn == 0 ? 0.0 : INFINITY) : inv_norm * norm1(R.data(), n, nrhs) / xnorm;
+    out[0] = it; out[1] = err <= rtol; out[2] = err; out[3] = rn;
+    out[4] = condition == 0 ? 1.0 : std::min(1.0, 1.0 / condition);
+    out[5] = delta < 1 ? delta / (1 - delta) : INFINITY;
+    return 0;
+}
+
+// expert_solve (src/lapack/advanced_factorizations.jl:591-645) with
+// equilibration and partial pivoting.  out: [iterations, converged,
+// backward_error]; rows, cols: the scales.  Returns 0, or -(i) for a zero row
+// i (SingularException(i)), or the getrf info as a positive value.
+int oxo_dexpert_solve(i64 n, i64 nrhs, const double* A, i64 lda, const double* B, i64 ldb, double* X, i64 ldx,
+                      int maxiter, double rtol, int equil, double* rows, double* cols, double* out, int threads) {
+    std::vector<double> S(size_t(n) * n);
+    for (i64 j = 0; j < n; ++j) for (i64 i = 0; i < n; ++i) S[i + j * n] = A[i + j * lda];
+    for (i64 i = 0; i < n; ++i) { rows[i] = 1; cols[i] = 1; }
+    if (equil) {
+        for (i64 i = 0; i < n; ++i) {
+            double v = 0;
+            for (i64 j = 0; j < n; ++j) v = std::max(v, std::fabs(S[i + j * n]));
+            if (v == 0) return -int(i + 1);
+            rows[i] = v;
+            for (i64 j = 0; j < n; ++j) S[i + j * n] /= v;
+        }
+        for (i64 j = 0; j < n; ++j) {
+            double v = 0;
+            for (i64 i = 0; i < n; ++i) v = std::max(v, std::fabs(S[i + j * n]));
+            cols[j] = v == 0 ? 1.0 : v;
+            for (i64 i = 0; i < n; ++i) S[i + j * n] /= cols[j];
+        }
+    }
+    std::vector<i64> piv(std::max<i64>(n, 1));
+    i64 info = 0;
+    getrf<double>(n, n, S.data(), n, piv.data(), 128, 0, &info, threads);
+    if (info) return int(info);
+    const bool vec = nrhs == 1;
+    std::vector<double> T(size_t(n) * nrhs), R(size_t(n) * nrhs);
+    for (i64 j = 0; j < nrhs; ++j) for (i64 i = 0; i < n; ++i) T[i + j * n] = B[i + j * ldb] / rows[i];
+    getrs<double>('N', n, nrhs, S.data(), n, piv.data(), T.data(), n, vec, threads);
+    for (i64 j = 0; j < nrhs; ++j) for (i64 i = 0; i < n; ++i) X[i + j * ldx] = T[i + j * n] / cols[i];
+    double err = INFINITY;
+    int it = 0;
+    for (int iteration = 0; iteration <= maxiter; ++iteration) {
+        err = oxo_dbackward_error(n, n, nrhs, A, lda, X, ldx, B, ldb, R.data(), n);
+        it = iteration;
+        if (err <= rtol || iteration == maxiter) break;
+        for (i64 j = 0; j < nrhs; ++j) for (i64 i = 0; i < n; ++i) T[i + j * n] = R[i + j * n] / rows[i];
+        getrs<double>('N', n, nrhs, S.data(), n, piv.data(), T.data(), n, vec, threads);
+        for (i64 j = 0; j < nrhs; ++j) for (i64 i = 0; i < n; ++i) X[i + j * ldx] += T[i + j * n] / cols[i];
+    }
+    out[0] = it; out[1] = err <= rtol; out[2] = err;
+    return 0;
+}
+
+int oxo_dsyrk(char uplo, char trans, i64 n, i64 k, double alpha, const double* A, i64 lda, double beta, double* C,
+              i64 ldc) {
+    return syrk<double>(uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
+}
+
+// _cholesky_factor_info!(L; blocksize, check=false) (factorizations.jl:246-261):
+// the lower triangle of L in place; *info = 0 or the failing column; on
+// success the strict upper triangle is zeroed.
+int oxo_dpotrf(i64 n, double* L, i64 ldl, i64 leaf, i64* info, int threads) {
+    if (n < 0) return -1;
+    if (ldl < std::max<i64>(1, n)) return -3;
+    if (leaf <= 0) return -4;
+    *info = chol_rec<double>(L, ldl, n, 0, leaf, threads);
+    if (*info == 0)
+        for (i64 j = 0; j < n; ++j) for (i64 i = 0; i < j; ++i) L[i + j * ldl] = 0.0;
+    return 0;
+}
+
+// solve(F::CholeskyFactor, B) (factorizations.jl:288-300): L then L^T.
+int oxo_dpotrs(i64 n, i64 nrhs, const double* L, i64 ldl, double* B, i64 ldb, int vector_rhs, int threads) {
+    if (n < 0 || nrhs < 0) return -1;
+    if (vector_rhs) {
+        int r = trsv<double>('L', 'N', 'N', n, L, ldl, B);
+        if (r) return r;
+        return trsv<double>('L', 'C', 'N', n, L, ldl, B);
+    }
+    int r = trsm<double>('L', 'L', 'N', 'N', n, nrhs, 1.0, L, ldl, B, ldb, 8, threads);
+    if (r) return r;
+    return trsm<double>('L', 'L', 'C', 'N', n, nrhs, 1.0, L, ldl, B, ldb, 8, threads);
+}
 
 int oxo_dgemm(char ta, char tb, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
               double beta, double* C, i64 ldc, int threads) {
@@ -768,6 +1044,73 @@ int oxo_mixed_solve(i64 n, i64 nrhs, const double* A, i64 lda, const double* B, 
     }
     bool finite = std::isfinite(err);
     for (i64 j = 0; j < nrhs && finite; ++j) for (i64 i = 0; i < n; ++i) if (!std::isfinite(X[i + j * ldx])) { finite = false; break; }
+    *iterations = it; *converged = err <= rtol; *berr = err; *fallback = used_fallback;
+    *reason = !finite ? 2 : used_fallback ? 3 : err <= rtol ? 0 : 1;
+    return 0;
+}
+
+// mixed_solve(A, B; factorization=:cholesky) (src/lapack/extended_lapack.jl:175-226):
+// scale = max |a_ij| (1 for a zero matrix), rows = scale, cols = 1; Float32
+// Cholesky (chol_rec, leaf 64) of round(A / scale), Float64 on
+// PosDefException; the correction step with L and L^T solves.  Returns <0 on
+// invalid input, -3 when the Float64 factor fails too (PosDefException).
+int oxo_mixed_solve_chol(i64 n, i64 nrhs, const double* A, i64 lda, const double* B, i64 ldb, double* X, i64 ldx,
+                         int maxiter, double rtol, int threads, i64* iterations, int* converged, double* berr,
+                         int* fallback, int* reason) {
+    if (n < 0 || nrhs < 0 || maxiter < 0 || !std::isfinite(rtol) || rtol < 0) return -1;
+    double scale = 0;
+    for (i64 j = 0; j < n; ++j) for (i64 i = 0; i < n; ++i) {
+        if (!std::isfinite(A[i + j * lda])) return -2;
+        scale = std::max(scale, std::fabs(A[i + j * lda]));
+    }
+    for (i64 j = 0; j < nrhs; ++j) for (i64 i = 0; i < n; ++i) if (!std::isfinite(B[i + j * ldb])) return -2;
+    if (scale == 0) scale = 1;
+    std::vector<double> E(size_t(n) * n);
+    for (i64 j = 0; j < n; ++j) for (i64 i = 0; i < n; ++i) E[i + j * n] = A[i + j * lda] / scale;
+    bool used_fallback = false, overflow = false;
+    std::vector<float> low(size_t(n) * n);
+    for (size_t t = 0; t < low.size(); ++t) { low[t] = float(E[t]); if (!std::isfinite(low[t])) overflow = true; }
+    i64 info = overflow ? 1 : chol_rec<float>(low.data(), n, n, 0, 64, threads);
+    std::vector<double> Fd;
+    if (info != 0) {
+        used_fallback = true;
+        Fd = E;
+        if (chol_rec<double>(Fd.data(), n, n, 0, 64, threads) != 0) return -3;
+    }
+    auto correction = [&](const double* Rin, i64 ldr, double* out, i64 ldo) -> int {
+        std::vector<double> scaled(size_t(n) * nrhs);
+        double mag = 0;
+        for (i64 j = 0; j < nrhs; ++j) for (i64 i = 0; i < n; ++i) {
+            double sc = Rin[i + j * ldr] / scale; scaled[i + j * n] = sc; mag = std::max(mag, std::fabs(sc));
+        }
+        if (mag == 0) { for (i64 j = 0; j < nrhs; ++j) for (i64 i = 0; i < n; ++i) out[i + j * ldo] = 0; return 0; }
+        if (!std::isfinite(mag)) return -4;
+        int e; std::frexp(mag, &e); mag = std::ldexp(1.0, e - 1);
+        if (!used_fallback) {
+            std::vector<float> rhs(size_t(n) * nrhs);
+            for (size_t t = 0; t < rhs.size(); ++t) { rhs[t] = float(scaled[t] / mag); if (!std::isfinite(rhs[t])) return -4; }
+            trsm<float>('L', 'L', 'N', 'N', n, nrhs, 1.0f, low.data(), n, rhs.data(), n, 8, threads);
+            trsm<float>('L', 'L', 'C', 'N', n, nrhs, 1.0f, low.data(), n, rhs.data(), n, 8, threads);
+            for (i64 j = 0; j < nrhs; ++j) for (i64 i = 0; i < n; ++i) out[i + j * ldo] = double(rhs[i + j * n]) * mag;
+        } else {
+            std::vector<double> rhs(size_t(n) * nrhs);
+            for (size_t t = 0; t < rhs.size(); ++t) rhs[t] = scaled[t] / mag;
+            trsm<double>('L', 'L', 'N', 'N', n, nrhs, 1.0, Fd.data(), n, rhs.data(), n, 8, threads);
+            trsm<double>('L', 'L', 'C', 'N', n, nrhs, 1.0, Fd.data(), n, rhs.data(), n, 8, threads);
+            for (i64 j = 0; j < nrhs; ++j) for (i64 i = 0; i < n; ++i) out[i + j * ldo] = rhs[i + j * n] * mag;
+        }
+        return 0;
+    };
+    if (correction(B, ldb, X, ldx)) return -4;
+    std::vector<double> R(size_t(n) * nrhs), C(size_t(n) * nrhs);
+    double err = INFINITY; i64 it = 0;
+    for (int iteration = 0; iteration <= maxiter; ++iteration) {
+        err = refinement_residual(n, nrhs, A, lda, X, ldx, B, ldb, R.data(), n); it = iteration;
+        if (err <= rtol || iteration == maxiter || !std::isfinite(err)) break;
+        if (correction(R.data(), n, C.data(), n)) return -4;
+        for (i64 j = 0; j < nrhs; ++j) for (i64 i = 0; i < n; ++i) X[i + j * ldx] += C[i + j * n];
+    }
+    bool finite = std::isfinite(err);
     *iterations = it; *converged = err <= rtol; *berr = err; *fallback = used_fallback;
     *reason = !finite ? 2 : used_fallback ? 3 : err <= rtol ? 0 : 1;
     return 0;

@@ -567,6 +567,21 @@ int oxi_dgemm_right64_inplace(hipStream_t s, i64 rows, const double* V, double* 
     return hipGetLastError() == hipSuccess ? OXI_OK : OXI_ERR_HIP;
 }
 
+int oxi_dgemm_left64_inplace(hipStream_t s, int ta, i64 ncols, const double* V, double* X, i64 ldx) {
+    // X[0:64, :] <- op(V) * X[0:64, :] (V 64 x 64, ld 64) in place: 64 x 32
+    // tiles, so each workgroup owns all 64 rows of its columns and writes
+    // them only after its last read of them (the epilogue follows the K loop)
+    if (ncols <= 0) return OXI_OK;
+    const int tiles_n = int(cdiv(ncols, 32));
+    if (ta)
+        hipLaunchKernelGGL((dgemm_mfma_kernel<64, 32, true, false, true>), dim3(tiles_n), dim3(NT), 0, s, i64(64),
+                           ncols, i64(64), 1.0, V, i64(64), X, ldx, 0.0, X, ldx, 1, tiles_n);
+    else
+        hipLaunchKernelGGL((dgemm_mfma_kernel<64, 32, false, false, true>), dim3(tiles_n), dim3(NT), 0, s, i64(64),
+                           ncols, i64(64), 1.0, V, i64(64), X, ldx, 0.0, X, ldx, 1, tiles_n);
+    return hipGetLastError() == hipSuccess ? OXI_OK : OXI_ERR_HIP;
+}
+
 bool oxi_dgemm_large_tiles(i64 m, i64 n) {
     int bm, bn;
     choose_tile(m, n, bm, bn);

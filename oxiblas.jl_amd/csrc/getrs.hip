@@ -192,6 +192,106 @@ extern "C" int oxi_sgetrs(oxi_handle_t h, char trans, int64_t n, int64_t nrhs, c
     return getrs_api<float>(h, trans, n, nrhs, LU, lda, ipiv, B, ldb);
 }
 
+// ---------------------------------------------------------------------------
+// Solve plans: what a solve with given factors prepares -- the permutation on
+// the device and the inverses of the 128 x 128 diagonal blocks of both
+// triangles for both solve kinds -- made once per factor and reused by every
+// later solve with those factors.
+struct oxi_getrs_plan_s {
+    int device = 0;
+    i64 n = 0;
+    const double* LU = nullptr;
+    i64 lda = 0;
+    i64* dperm = nullptr;
+    void* ctrl[2] = {nullptr, nullptr};
+    bool prepared[2] = {false, false};   // the inverses of kind 'N' (0) / 'T' (1), made on first use
+    i64 zero_pivot = 0;       // first zero on U's diagonal (1-based), 0 if none
+};
+
+extern "C" int oxi_dgetrs_plan_create(oxi_handle_t h, int64_t n, const double* LU, int64_t lda, const int64_t* ipiv,
+                                      oxi_getrs_plan_t* plan) {
+    if (!h) return -1;
+    if (n < 0) return -2;
+    if (lda < std::max<i64>(1, n)) return -4;
+    if (!plan) return -6;
+    *plan = nullptr;
+    if (n > 0 && !LU) return -3;
+    if (n > 0 && !ipiv) return -5;
+    std::lock_guard<std::recursive_mutex> lk(h->guard);
+    (void)hipSetDevice(h->device);
+    hipStream_t s = h->stream;
+    auto* p = new oxi_getrs_plan_s;
+    p->device = h->device; p->n = n; p->LU = LU; p->lda = lda;
+    auto fail = [&](int rc) {
+        (void)hipStreamSynchronize(s);
+        if (p->dperm) (void)hipFree(p->dperm);
+        for (auto c : p->ctrl) if (c) (void)hipFree(c);
+        delete p;
+        return rc;
+    };
+    if (n == 0) { *plan = p; return OXI_OK; }
+    int rc = first_zero_diag(h, s, LU, lda, n, &p->zero_pivot);
+    if (rc != OXI_OK) return fail(rc);
+    std::vector<i64> perm;
+    if ((rc = oxi_perm_from_ipiv(h, s, ipiv, n, perm)) != OXI_OK) return fail(rc);
+    const size_t cb = oxi_trsv_ws_bytes(n, sizeof(double));
+    if (hipMalloc(&p->dperm, size_t(n) * 8) != hipSuccess || hipMalloc(&p->ctrl[0], cb) != hipSuccess ||
+        hipMalloc(&p->ctrl[1], cb) != hipSuccess)
+        return fail(oxi_set_error(h, OXI_ERR_ALLOC, "getrs plan: allocation failed"));
+    if (hipMemcpyAsync(p->dperm, perm.data(), size_t(n) * 8, hipMemcpyHostToDevice, s) != hipSuccess)
+        return fail(oxi_set_error(h, OXI_ERR_HIP, "getrs plan: copy failed"));
+    if (hipStreamSynchronize(s) != hipSuccess) return fail(oxi_set_error(h, OXI_ERR_HIP, "getrs plan: sync failed"));
+    *plan = p;
+    return OXI_OK;
+}
+
+extern "C" int oxi_dgetrs_plan(oxi_handle_t h, oxi_getrs_plan_t plan, char trans, int64_t nrhs, double* B,
+                               int64_t ldb) {
+    if (!h) return -1;
+    if (!plan) return -2;
+    const int tk = trans_kind(trans);
+    if (tk < 0) return -3;
+    if (nrhs < 0) return -4;
+    const i64 n = plan->n;
+    if (ldb < std::max<i64>(1, n)) return -6;
+    std::lock_guard<std::recursive_mutex> lk(h->guard);
+    if (n == 0 || nrhs == 0) return OXI_OK;
+    if (!B) return -5;
+    if (plan->device != h->device) return oxi_set_error(h, -2, "getrs plan: made on another device");
+    if (plan->zero_pivot)
+        return oxi_set_error(h, OXI_ERR_SINGULAR, "getrs: U(%lld,%lld) is zero", (long long)plan->zero_pivot,
+                             (long long)plan->zero_pivot);
+    (void)hipSetDevice(h->device);
+    hipStream_t s = h->stream;
+    void* ws;
+    OXI_TRY(oxi_workspace(h, size_t(n) * size_t(nrhs) * 8 + 256, &ws));
+    const bool chain = nrhs <= OXI_TRSV_MAX_RHS;
+    void* ctrl = chain ? plan->ctrl[tk] : nullptr;
+    if (chain) {
+        OXI_TRY(oxi_trsv_begin(h, s, ctrl));
+        if (!plan->prepared[tk]) {
+            OXI_TRY(oxi_trsv_prepare(h, s, tk, n, plan->LU, plan->lda, (double*)nullptr, ctrl));
+            plan->prepared[tk] = true;
+        }
+    }
+    OXI_TRY(getrs_perm<double>(h, s, tk, n, nrhs, plan->LU, plan->lda, plan->dperm, B, ldb, (double*)ws, ctrl));
+    if (chain) return oxi_trsv_check(h, s, ctrl);
+    OXI_HIP(h, hipStreamSynchronize(s));
+    return OXI_OK;
+}
+
+extern "C" int oxi_dgetrs_plan_destroy(oxi_handle_t h, oxi_getrs_plan_t plan) {
+    if (!h) return -1;
+    if (!plan) return OXI_OK;
+    std::lock_guard<std::recursive_mutex> lk(h->guard);
+    (void)hipSetDevice(plan->device);
+    (void)hipStreamSynchronize(h->stream);
+    if (plan->dperm) (void)hipFree(plan->dperm);
+    for (auto c : plan->ctrl) if (c) (void)hipFree(c);
+    delete plan;
+    return OXI_OK;
+}
+
 // Pullback of X = A \ B with the factors of oxi_dgetrf: Bbar = A^-T Xbar
 // (ldiv!(transpose(F), .), src/api/linalg_api.jl:176-184) and
 // Abar = -Bbar X^T (a rank-nrhs FP64 MFMA GEMM).

@@ -162,6 +162,35 @@ __global__ void equil_convert_kernel(const double* A, i64 lda, i64 m, i64 n, con
     if (bad) atomicOr(&flags[1], 1u);
 }
 
+// the largest row maximum (one global scale, the Cholesky form of the mixed
+// solve, src/lapack/extended_lapack.jl:187-189)
+__global__ void global_max_kernel(const double* __restrict__ pmax, i64 total, u64* out) {
+    double mx = 0;
+    for (i64 i = i64(blockIdx.x) * TPB + threadIdx.x; i < total; i += i64(gridDim.x) * TPB) mx = fmax(mx, pmax[i]);
+    wave_atomic_max(out, mx);
+}
+// rows = scale (1 for a zero matrix), cols = 1 and E (TO) = A ./ scale
+template <typename TO>
+__global__ void global_scale_kernel(const double* A, i64 lda, i64 n, const u64* scalebits, double* __restrict__ rows,
+                                    double* __restrict__ rinv, double* __restrict__ cols, double* __restrict__ cinv,
+                                    TO* E, i64 lde, unsigned* __restrict__ flags) {
+    const i64 i = i64(blockIdx.x) * TPB + threadIdx.x;
+    if (i >= n) return;
+    double sc = bitsd(*scalebits);
+    if (sc == 0) sc = 1.0;
+    if (blockIdx.y == 0) {
+        rows[i] = sc; rinv[i] = 1.0 / sc; cols[i] = 1.0; cinv[i] = 1.0;
+    }
+    if (!E) return;
+    bool bad = false;
+    for (i64 j = blockIdx.y; j < n; j += gridDim.y) {
+        const TO v = TO(A[i + j * lda] / sc);
+        bad |= !(fabs(double(v)) <= DBL_MAX);
+        E[i + j * lde] = v;
+    }
+    if (bad) atomicOr(&flags[1], 1u);
+}
+
 __global__ void finite_kernel(const double* __restrict__ B, i64 ldb, i64 n, i64 nrhs, unsigned* flags) {
     const i64 i = i64(blockIdx.x) * TPB + threadIdx.x;
     if (i >= n) return;
@@ -412,6 +441,7 @@ struct Mixed {
     int nseg = 1;
     i64 seglen = 1;
     bool fallback = false;
+    bool chol = false;         // Cholesky factors (factorization=:cholesky): L, then L^T solves
     double anorm = 0;
 
     dim3 vgrid(i64 cols) const {
@@ -494,12 +524,40 @@ int prepare(Mixed& c, int kind) {
     return OXI_OK;
 }
 
+// The correction with Cholesky factors (solve(F::CholeskyFactor, .),
+// src/lapack/factorizations.jl:288-300): L then L^T, in FP32 on the low
+// factor or in FP64 after a fallback.
+int chol_correction(Mixed& c, const double* Rin, i64 ldr, bool init) {
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, c.s, &c.stats[5], 1);
+    hipLaunchKernelGGL(corr_scale_kernel, c.vgrid(c.nrhs), dim3(TPB), 0, c.s, Rin, ldr, c.n, c.nrhs, c.rows, c.S,
+                       &c.stats[5], c.flags);
+    if (!c.fallback) {
+        hipLaunchKernelGGL(corr_low_kernel<float>, c.vgrid(c.nrhs), dim3(TPB), 0, c.s, c.S, c.n, c.nrhs, &c.stats[5],
+                           c.Yf, c.magout, c.flags);
+        OXI_TRY(launch_ok(c.h, "correction"));
+        OXI_TRY(oxi_strsm_left(c.h, c.s, 0, 0, 0, c.n, c.nrhs, c.A32, c.n, c.Yf, c.n));
+        OXI_TRY(oxi_strsm_left(c.h, c.s, 0, 1, 0, c.n, c.nrhs, c.A32, c.n, c.Yf, c.n));
+        hipLaunchKernelGGL(corr_back_kernel<float>, c.vgrid(c.nrhs), dim3(TPB), 0, c.s, c.Yf, c.n, c.nrhs, c.magout,
+                           c.cols, c.X, c.ldx, int(init));
+    } else {
+        hipLaunchKernelGGL(corr_low_kernel<double>, c.vgrid(c.nrhs), dim3(TPB), 0, c.s, c.S, c.n, c.nrhs,
+                           &c.stats[5], c.Yd, c.magout, c.flags);
+        OXI_TRY(launch_ok(c.h, "correction"));
+        OXI_TRY(oxi_dtrsm_left(c.h, c.s, 0, 0, 0, c.n, c.nrhs, c.E64, c.n, c.Yd, c.n));
+        OXI_TRY(oxi_dtrsm_left(c.h, c.s, 0, 1, 0, c.n, c.nrhs, c.E64, c.n, c.Yd, c.n));
+        hipLaunchKernelGGL(corr_back_kernel<double>, c.vgrid(c.nrhs), dim3(TPB), 0, c.s, c.Yd, c.n, c.nrhs, c.magout,
+                           c.cols, c.X, c.ldx, int(init));
+    }
+    return launch_ok(c.h, "correction");
+}
+
 int correction(Mixed& c, const double* Rin, i64 ldr, bool init) {
     // after the butterflies (no pivoting) the few-RHS correction solves run in
     // FP64 arithmetic with the FP32 factors: the triangles of a factor without
     // pivoting are less well conditioned, and FP32 block-inverse products
     // would slow the contraction down
     const bool wide = c.rbt && c.nrhs <= OXI_TRSV_MAX_RHS;
+    if (c.chol) return chol_correction(c, Rin, ldr, init);
     if (c.nrhs <= OXI_TRSV_MAX_RHS) OXI_TRY(prepare(c, c.fallback ? 2 : wide ? 1 : 0));
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, c.s, &c.stats[5], 1);
     hipLaunchKernelGGL(corr_scale_kernel, c.vgrid(c.nrhs), dim3(TPB), 0, c.s, Rin, ldr, c.n, c.nrhs, c.rows, c.S,
@@ -864,6 +922,153 @@ extern "C" int oxi_dgesv_mixed(oxi_handle_t h, int prec, int solver, int64_t n, 
     info->fallback = c.fallback;
     info->reason = !finite ? 2 : c.fallback ? 3 : err <= rtol ? 0 : 1;
     info->solver = used_gmres ? 1 : 0;
+    info->backward_error = err;
+    info->normwise_backward_error = nerr;
+    info->residual_norm = std::sqrt(rss);
+    info->factor_ms = f_ms;
+    info->refine_ms = r_ms;
+    return OXI_OK;
+}
+
+// mixed_solve(A, B; factorization=:cholesky) (src/lapack/extended_lapack.jl:175-226
+// with _mixed_factor's :cholesky, :161): A scaled by its largest |entry|,
+// the FP32 Cholesky of the lower triangle, FP64 fallback on
+// PosDefException, the reference's correction step with L and L^T solves,
+// classical refinement.
+extern "C" int oxi_dposv_mixed(oxi_handle_t h, int prec, int64_t n, int64_t nrhs, const double* A, int64_t lda,
+                               const double* B, int64_t ldb, double* X, int64_t ldx, int32_t maxiter, double rtol,
+                               oxi_ir_info* info) {
+    if (!h) return -1;
+    if (prec != OXI_PREC_FP32) return -2;
+    if (n < 0) return -3;
+    if (nrhs < 0) return -4;
+    if (lda < std::max<i64>(1, n)) return -6;
+    if (ldb < std::max<i64>(1, n)) return -8;
+    if (ldx < std::max<i64>(1, n)) return -10;
+    if (maxiter < 0) return -11;
+    if (!std::isfinite(rtol) || rtol < 0) return -12;
+    if (!info) return -13;
+    std::lock_guard<std::recursive_mutex> lk(h->guard);
+    *info = oxi_ir_info{};
+    info->converged = 1;
+    if (n == 0 || nrhs == 0) return OXI_OK;
+    if (!A) return -5;
+    if (!B) return -7;
+    if (!X) return -9;
+    (void)hipSetDevice(h->device);
+
+    Mixed c;
+    c.h = h; c.s = h->stream; c.n = n; c.nrhs = nrhs; c.chol = true;
+    c.A = A; c.lda = lda; c.B = B; c.ldb = ldb; c.X = X; c.ldx = ldx;
+    plan_residual(c);
+    const int nrowchunk = int(cdiv(n, EQ_KC));
+    const size_t nn = size_t(n) * size_t(n), nr = size_t(n) * size_t(nrhs);
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += pad(bytes); return o; };
+    const size_t oA32 = take(nn * 4);
+    const size_t orows = take(size_t(n) * 8), ocols = take(size_t(n) * 8), orinv = take(size_t(n) * 8),
+                 ocinv = take(size_t(n) * 8), orowsum = take(size_t(n) * 8);
+    const size_t oR = take(nr * 8), oS = take(nr * 8), oYd = take(nr * 8), oYf = take(nr * 4);
+    const size_t partial = std::max(size_t(c.nchunk) * nr, size_t(nrowchunk) * size_t(n));
+    const size_t oP = take(partial * 8), oQ = take(partial * 8);
+    const size_t ostats = take(8 * 8), oflags = take(8 * 4), omag = take(8), oscale = take(8), odinfo = take(8);
+    void* ws;
+    OXI_TRY(oxi_workspace2(h, off, &ws));
+    char* base = static_cast<char*>(ws);
+    c.A32 = (float*)(base + oA32);
+    c.rows = (double*)(base + orows); c.cols = (double*)(base + ocols);
+    c.rinv = (double*)(base + orinv); c.cinv = (double*)(base + ocinv); c.rowsum = (double*)(base + orowsum);
+    c.R = (double*)(base + oR); c.S = (double*)(base + oS); c.Yd = (double*)(base + oYd); c.Yf = (float*)(base + oYf);
+    c.P = (double*)(base + oP); c.Q = (double*)(base + oQ);
+    c.stats = (u64*)(base + ostats); c.flags = (unsigned*)(base + oflags); c.magout = (double*)(base + omag);
+    u64* scalebits = (u64*)(base + oscale);
+    i64* dinfo = (i64*)(base + odinfo);
+
+    EventPair ev;
+    for (auto& e : ev.e) OXI_HIP(h, hipEventCreate(&e));
+    OXI_HIP(h, hipEventRecord(ev.e[0], c.s));
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, c.s, c.stats, 8);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, c.s, (u64*)c.flags, 4);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, c.s, scalebits, 1);
+    // 1. finite check, ||A||_inf, the global scale and the FP32 copy
+    hipLaunchKernelGGL(finite_kernel, c.vgrid(nrhs), dim3(TPB), 0, c.s, B, ldb, n, nrhs, c.flags);
+    hipLaunchKernelGGL(row_stats_kernel, dim3(unsigned(cdiv(n, TPB)), unsigned(nrowchunk)), dim3(TPB), 0, c.s, A, lda,
+                       n, n, EQ_KC, c.P, c.Q, c.flags);
+    hipLaunchKernelGGL(row_scale_kernel, dim3(unsigned(cdiv(n, TPB))), dim3(TPB), 0, c.s, n, nrowchunk, c.P, c.Q, 0,
+                       c.rows, c.rinv, c.rowsum, &c.stats[4]);
+    hipLaunchKernelGGL(global_max_kernel, dim3(unsigned(std::min<i64>(cdiv(i64(nrowchunk) * n, TPB), 4096))), dim3(TPB),
+                       0, c.s, c.P, i64(nrowchunk) * n, scalebits);
+    hipLaunchKernelGGL(global_scale_kernel<float>,
+                       dim3(unsigned(cdiv(n, TPB)), unsigned(std::min<i64>(n, 65535))), dim3(TPB), 0, c.s, A, lda, n,
+                       scalebits, c.rows, c.rinv, c.cols, c.cinv, c.A32, n, c.flags);
+    OXI_TRY(launch_ok(h, "mixed cholesky scaling"));
+    {
+        u64 st[8];
+        unsigned fl[8];
+        OXI_TRY(read_stats(c, st, fl));
+        if (fl[0]) return oxi_set_error(h, OXI_ERR_NOT_FINITE, "mixed_solve: finite mixed input required");
+        c.anorm = from_bits(st[4]);
+        // 2. the FP32 Cholesky; FP64 on PosDefException or overflow
+        i64 finfo = 0;
+        if (!fl[1]) {
+            OXI_TRY(oxi_potrf_device(h, c.s, n, c.A32, n, dinfo));
+            OXI_HIP(h, hipMemcpyAsync(&finfo, dinfo, 8, hipMemcpyDeviceToHost, c.s));
+            OXI_HIP(h, hipStreamSynchronize(c.s));
+        }
+        if (fl[1] || finfo != 0) {
+            c.fallback = true;
+            if (hipMalloc(&c.E64, nn * 8) != hipSuccess) {
+                c.E64 = nullptr;
+                return oxi_set_error(h, OXI_ERR_ALLOC, "mixed_solve: FP64 fallback factor allocation failed");
+            }
+        }
+    }
+    struct FreeE64 { double*& p; ~FreeE64() { if (p) { (void)hipDeviceSynchronize(); (void)hipFree(p); } } } free_e64{c.E64};
+    if (c.fallback) {
+        hipLaunchKernelGGL(global_scale_kernel<double>,
+                           dim3(unsigned(cdiv(n, TPB)), unsigned(std::min<i64>(n, 65535))), dim3(TPB), 0, c.s, A, lda,
+                           n, scalebits, c.rows, c.rinv, c.cols, c.cinv, c.E64, n, c.flags);
+        OXI_TRY(launch_ok(h, "mixed cholesky scaling"));
+        OXI_TRY(oxi_potrf_device(h, c.s, n, c.E64, n, dinfo));
+        i64 finfo = 0;
+        OXI_HIP(h, hipMemcpyAsync(&finfo, dinfo, 8, hipMemcpyDeviceToHost, c.s));
+        OXI_HIP(h, hipStreamSynchronize(c.s));
+        if (finfo != 0)
+            return oxi_set_error(h, OXI_ERR_NOT_POSDEF, "mixed_solve: not positive definite (A(%lld,%lld))",
+                                 (long long)finfo, (long long)finfo);
+    }
+    OXI_HIP(h, hipEventRecord(ev.e[1], c.s));
+    // 3-5. first solve and classical refinement (the reference's loop)
+    OXI_TRY(correction(c, B, ldb, true));
+    double err = INFINITY, nerr = INFINITY;
+    bool xfinite = true;
+    i64 iterations = 0;
+    for (int iteration = 0; iteration <= maxiter; ++iteration) {
+        OXI_TRY(residual(c, &err, &nerr, &xfinite));
+        iterations = iteration;
+        if (err <= rtol || iteration == maxiter || !std::isfinite(err)) break;
+        OXI_TRY(correction(c, c.R, n, false));
+    }
+    OXI_HIP(h, hipEventRecord(ev.e[2], c.s));
+    double rss = 0;
+    {
+        // nrm2 of the last residual: the partial-sum machinery of the residual, by column
+        std::vector<double> hR(nr);
+        OXI_HIP(h, hipMemcpyAsync(hR.data(), c.R, nr * 8, hipMemcpyDeviceToHost, c.s));
+        OXI_HIP(h, hipStreamSynchronize(c.s));
+        for (double v : hR) rss += v * v;
+    }
+    OXI_HIP(h, hipEventSynchronize(ev.e[2]));
+    float f_ms = 0, r_ms = 0;
+    (void)hipEventElapsedTime(&f_ms, ev.e[0], ev.e[1]);
+    (void)hipEventElapsedTime(&r_ms, ev.e[1], ev.e[2]);
+    const bool finite = xfinite && std::isfinite(err);
+    info->iterations = iterations;
+    info->inner_iterations = 0;
+    info->converged = err <= rtol;
+    info->fallback = c.fallback;
+    info->reason = !finite ? 2 : c.fallback ? 3 : err <= rtol ? 0 : 1;
+    info->solver = 0;
     info->backward_error = err;
     info->normwise_backward_error = nerr;
     info->residual_norm = std::sqrt(rss);

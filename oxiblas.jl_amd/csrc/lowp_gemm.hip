@@ -286,7 +286,119 @@ __global__ __launch_bounds__(256) void block_scale_kernel(const float* __restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// Float16 gemm! (src/blas/extended_blas.jl:39-88): Float16 operands and
+// output, Float32 accumulation of each element over all of K, one rounding to
+// Float16 at the end: C = Float16(a * acc + b * Float32(C)) with a, b the
+// Float32 values of alpha and beta rounded to Float16 (beta == 0 never reads C).
+// ---------------------------------------------------------------------------
+// out[o][p] (Kp-row-major) = op(X)(o, p) for o < O, p < K, zero up to Kp;
+// element (o, p) of op(X) at X[o * so + p * sp].  P_CONTIG: sp == 1 (threads
+// run along p), otherwise so == 1 (a 32 x 32 tile transposed through LDS).
+template <bool P_CONTIG>
+__global__ __launch_bounds__(256) void pack_half_kernel(const _Float16* __restrict__ X, i64 so, i64 sp, i64 O, i64 K,
+                                                        i64 Kp, _Float16* __restrict__ out) {
+    __shared__ _Float16 tile[32][34];
+    const i64 o0 = i64(blockIdx.x) * 32, p0 = i64(blockIdx.y) * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    if (P_CONTIG) {
+        for (int y = ty; y < 32; y += 8) {
+            const i64 o = o0 + y, p = p0 + tx;
+            if (o < O && p < Kp) out[o * Kp + p] = p < K ? X[o * so + p] : _Float16(0);
+        }
+        return;
+    }
+    for (int y = ty; y < 32; y += 8) {
+        const i64 o = o0 + tx, p = p0 + y;
+        tile[y][tx] = (o < O && p < K) ? X[o + p * sp] : _Float16(0);
+    }
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8) {
+        const i64 o = o0 + y, p = p0 + tx;
+        if (o < O && p < Kp) out[o * Kp + p] = tile[tx][y];
+    }
+}
+__global__ void zero_f32_kernel(float* __restrict__ C, i64 total) {
+    for (i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += i64(gridDim.x) * blockDim.x) C[i] = 0.0f;
+}
+// C (Float16) <- Float16(aw * acc + bw * Float32(C)); beta == 0 never reads C
+__global__ void hgemm_epilogue_kernel(const float* __restrict__ acc, i64 M, i64 N, float aw, float bw, int beta0,
+                                      _Float16* __restrict__ C, i64 ldc) {
+    const i64 total = M * N;
+    for (i64 idx = i64(blockIdx.x) * blockDim.x + threadIdx.x; idx < total; idx += i64(gridDim.x) * blockDim.x) {
+        const i64 j = idx / M, i = idx - j * M;
+        float v = aw * acc[idx];
+        if (!beta0) v += bw * float(C[i + j * ldc]);
+        C[i + j * ldc] = _Float16(v);
+    }
+}
+__global__ void hscale_kernel(i64 M, i64 N, _Float16 beta, _Float16* __restrict__ C, i64 ldc) {
+    const i64 total = M * N;
+    for (i64 idx = i64(blockIdx.x) * blockDim.x + threadIdx.x; idx < total; idx += i64(gridDim.x) * blockDim.x) {
+        const i64 j = idx / M, i = idx - j * M;
+        C[i + j * ldc] = beta == _Float16(0) ? _Float16(0) : _Float16(float(beta) * float(C[i + j * ldc]));
+    }
+}
+
 }  // namespace
+
+extern "C" int oxi_hgemm(oxi_handle_t h, char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha,
+                         const void* Av, int64_t lda, const void* Bv, int64_t ldb, float beta, void* Cv, int64_t ldc) {
+    if (!h) return -1;
+    const int ta = (transA == 'N' || transA == 'n') ? 0 : (transA == 'T' || transA == 't' || transA == 'C' || transA == 'c') ? 1 : -1;
+    const int tb = (transB == 'N' || transB == 'n') ? 0 : (transB == 'T' || transB == 't' || transB == 'C' || transB == 'c') ? 1 : -1;
+    if (ta < 0) return -2;
+    if (tb < 0) return -3;
+    if (m < 0) return -4;
+    if (n < 0) return -5;
+    if (k < 0) return -6;
+    if (lda < std::max<i64>(1, ta ? k : m)) return -9;
+    if (ldb < std::max<i64>(1, tb ? n : k)) return -11;
+    if (ldc < std::max<i64>(1, m)) return -14;
+    std::lock_guard<std::recursive_mutex> lk(h->guard);
+    if (m == 0 || n == 0) return OXI_OK;
+    if (!Cv) return -13;
+    (void)hipSetDevice(h->device);
+    hipStream_t s = h->stream;
+    const _Float16* A = (const _Float16*)Av;
+    const _Float16* B = (const _Float16*)Bv;
+    _Float16* C = (_Float16*)Cv;
+    // alpha and beta are converted to Float16 first (extended_blas.jl:43)
+    const _Float16 ah = _Float16(alpha), bh = _Float16(beta);
+    const unsigned eg = unsigned(std::min<i64>(cdiv(m * n, 256), 8192));
+    if (ah == _Float16(0) || k == 0) {
+        hipLaunchKernelGGL(hscale_kernel, dim3(eg), dim3(256), 0, s, m, n, bh, C, ldc);
+        hipError_t e = hipGetLastError();
+        return e == hipSuccess ? OXI_OK : oxi_set_error(h, OXI_ERR_HIP, "hgemm launch: %s", hipGetErrorString(e));
+    }
+    if (!A) return -8;
+    if (!B) return -10;
+    const i64 kslab = SLAB_BYTES / 2;
+    const i64 Kp = cdiv(k, kslab) * kslab;
+    const size_t abytes = (size_t(m) * Kp * 2 + 255) & ~size_t(255), bbytes = (size_t(n) * Kp * 2 + 255) & ~size_t(255);
+    void* ws;
+    OXI_TRY(oxi_workspace(h, abytes + bbytes + size_t(m) * size_t(n) * 4 + 256, &ws));
+    _Float16* Ap = (_Float16*)ws;
+    _Float16* Bp = (_Float16*)((char*)ws + abytes);
+    float* acc = (float*)((char*)ws + abytes + bbytes);
+    // op(A)(i, p): A[i + p*lda] ('N', i contiguous) or A[p + i*lda] ('T', p contiguous);
+    // op(B)(p, j) packed as [j][p]: B[p + j*ldb] ('N') or B[j + p*ldb] ('T')
+    const dim3 ga(unsigned(cdiv(m, 32)), unsigned(cdiv(Kp, 32))), gb(unsigned(cdiv(n, 32)), unsigned(cdiv(Kp, 32)));
+    if (ta) hipLaunchKernelGGL(pack_half_kernel<true>, ga, dim3(256), 0, s, A, lda, i64(1), m, k, Kp, Ap);
+    else hipLaunchKernelGGL(pack_half_kernel<false>, ga, dim3(256), 0, s, A, i64(1), lda, m, k, Kp, Ap);
+    if (tb) hipLaunchKernelGGL(pack_half_kernel<false>, gb, dim3(256), 0, s, B, i64(1), ldb, n, k, Kp, Bp);
+    else hipLaunchKernelGGL(pack_half_kernel<true>, gb, dim3(256), 0, s, B, ldb, i64(1), n, k, Kp, Bp);
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(eg), dim3(256), 0, s, acc, m * n);
+    const int tiles_m = int(cdiv(m, BM)), tiles_n = int(cdiv(n, BN));
+    hipLaunchKernelGGL(lowp_update_kernel<1>, dim3(unsigned(tiles_m * tiles_n)), dim3(NT), 0, s, m, n, Kp, 1.0f,
+                       (const unsigned char*)Ap, (const unsigned char*)Bp, (const float*)nullptr, (const float*)nullptr,
+                       acc, m, tiles_m, tiles_n);
+    hipLaunchKernelGGL(hgemm_epilogue_kernel, dim3(eg), dim3(256), 0, s, acc, m, n, float(ah), float(bh),
+                       int(bh == _Float16(0)), C, ldc);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "hgemm launch: %s", hipGetErrorString(e));
+    return OXI_OK;
+}
 
 // Trailing update C (M x N, FP32, ldc) += alpha * X_A (M x K) * X_B (K x N) with
 // both operands rounded to `prec` (OXI_PREC_BF16 / FP16 / FP8) from FP32

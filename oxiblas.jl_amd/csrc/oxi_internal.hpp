@@ -81,10 +81,29 @@ int oxi_launch_dgemm(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n
                      const double* A, i64 lda, const double* B, i64 ldb, double beta, double* C, i64 ldc);
 // C <- beta*C (beta == 0 writes zeros without reading C).
 int oxi_launch_dscale(hipStream_t s, i64 m, i64 n, double beta, double* C, i64 ldc);
+// A batch of equal-shaped products at fixed element strides (grid.y = product).
+int oxi_launch_dgemm_strided(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n, i64 k, double alpha,
+                             const double* A, i64 lda, i64 sa, const double* B, i64 ldb, i64 sb, double beta,
+                             double* C, i64 ldc, i64 sc, i64 batch);
+// A batch of equal-shaped products given by device arrays of operand pointers.
+int oxi_launch_dgemm_ptrs(oxi_handle_t h, hipStream_t s, int ta, int tb, i64 m, i64 n, i64 k, double alpha,
+                          const double* const* A, i64 lda, const double* const* B, i64 ldb, double beta,
+                          double* const* C, i64 ldc, i64 batch);
+// C[triangle] <- alpha op(A) op(A)^T + beta C[triangle] (upper != 0: upper
+// triangle; trans 0: A is n x k, 1: A is k x n); the other triangle untouched.
+int oxi_launch_dsyrk(oxi_handle_t h, hipStream_t s, int upper, int trans, i64 n, i64 k, double alpha,
+                     const double* A, i64 lda, double beta, double* C, i64 ldc);
+// Cholesky of the lower triangle of A in place on stream s; *dinfo (device)
+// receives 0 or the first failing column (1-based).  No synchronisation.
+int oxi_potrf_device(oxi_handle_t h, hipStream_t s, i64 n, double* A, i64 lda, i64* dinfo);
+// the same in FP32 (its trailing updates write both triangles)
+int oxi_potrf_device(oxi_handle_t h, hipStream_t s, i64 n, float* A, i64 lda, i64* dinfo);
 // whether oxi_launch_dgemm runs an m x n product on its 128 x 128 (16x16x4) kernel
 bool oxi_dgemm_large_tiles(i64 m, i64 n);
 // X[:, 0:64] <- X[:, 0:64] * V in place (V 64 x 64, column-major, ld 64)
 int oxi_dgemm_right64_inplace(hipStream_t s, i64 rows, const double* V, double* X, i64 ldx);
+// X[0:64, 0:ncols] <- op(V) X[0:64, 0:ncols] in place (V 64 x 64, ld 64; ta: V^T)
+int oxi_dgemm_left64_inplace(hipStream_t s, int ta, i64 ncols, const double* V, double* X, i64 ldx);
 // Left-side triangular solve, host-recursive with GEMM updates.
 int oxi_dtrsm_left(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 nrhs,
                    const double* A, i64 lda, double* B, i64 ldb);

@@ -294,11 +294,40 @@ template <typename TY>
 static int trsm_right_lower(oxi_handle_t h, hipStream_t s, int trans, int unit, i64 rows, i64 n, const TY* T,
                             i64 ldt, TY* X, i64 ldx);
 
+// Left solves with many right-hand sides (FP64, m a multiple of 64 and at
+// most 64 * TRSM_INV_BLOCKS rows): the inverses of op(A)'s 64 x 64 diagonal
+// blocks first, so that every leaf of the recursion is one in-place product
+// on the matrix cores instead of a column sweep per right-hand side.  For an
+// upper op(A) the kernel gives V_b = op(A)_bb^-1; for a lower op(A) it gives
+// V_b = (op(A)_bb^T)^-1 and the leaf multiplies by V_b^T.
+static const double* left_inverses(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 nrhs,
+                                   const double* A, i64 lda) {
+    if (m < 2 * TRL || m % TRL || m / TRL > TRSM_INV_BLOCKS || nrhs < 16 || !h->trsm_inv) return nullptr;
+    const bool lower_op = (!upper) != (trans != 0);
+    const bool ktrans = lower_op ? !trans : (trans != 0);
+    const dim3 grid(unsigned(m / TRL)), block(64);
+    if (unit) {
+        if (ktrans) hipLaunchKernelGGL((trsm_inv64_kernel<true, true>), grid, block, 0, s, A, lda, h->trsm_inv);
+        else hipLaunchKernelGGL((trsm_inv64_kernel<true, false>), grid, block, 0, s, A, lda, h->trsm_inv);
+    } else {
+        if (ktrans) hipLaunchKernelGGL((trsm_inv64_kernel<false, true>), grid, block, 0, s, A, lda, h->trsm_inv);
+        else hipLaunchKernelGGL((trsm_inv64_kernel<false, false>), grid, block, 0, s, A, lda, h->trsm_inv);
+    }
+    return hipGetLastError() == hipSuccess ? h->trsm_inv : nullptr;
+}
+
 template <typename TY>
 static int trsm_left_t(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 nrhs, const TY* A,
-                       i64 lda, TY* B, i64 ldb) {
+                       i64 lda, TY* B, i64 ldb, const double* inv = nullptr) {
     if (m <= 0 || nrhs <= 0) return OXI_OK;
     const bool lower_op = (!upper) != (trans != 0);  // op(A) lower?
+    if constexpr (sizeof(TY) == 8) {
+        if (inv && m == TRL) {
+            if (oxi_dgemm_left64_inplace(s, lower_op ? 1 : 0, nrhs, inv, (double*)B, ldb) != OXI_OK)
+                return oxi_set_error(h, OXI_ERR_HIP, "trsm in-place block product launch failed");
+            return OXI_OK;
+        }
+    }
     if (m <= TRL) {
         dim3 grid(unsigned(cdiv(nrhs, 4)));
         if (lower_op) {
@@ -315,18 +344,19 @@ static int trsm_left_t(oxi_handle_t h, hipStream_t s, int upper, int trans, int 
     // op(A) = [A11 A12; A21 A22] in op coordinates; op(A)[i][j] = trans ? A[j][i] : A[i][j]
     const TY* A11 = A;
     const TY* A22 = A + hh + hh * lda;
+    const double* inv2 = inv ? inv + (hh / TRL) * 4096 : nullptr;   // the blocks of A22
     if (lower_op) {
-        OXI_TRY(trsm_left_t(h, s, upper, trans, unit, hh, nrhs, A11, lda, B, ldb));
+        OXI_TRY(trsm_left_t(h, s, upper, trans, unit, hh, nrhs, A11, lda, B, ldb, inv));
         // B2 -= op(A)[hh:, :hh] * B1 ; op(A)[hh][0] = trans ? A[0][hh] : A[hh][0]
         const TY* A21 = trans ? A + hh * lda : A + hh;
         OXI_TRY(gemm_t<TY>(h, s, trans, 0, m - hh, nrhs, hh, TY(-1), A21, lda, B, ldb, TY(1), B + hh, ldb));
-        return trsm_left_t(h, s, upper, trans, unit, m - hh, nrhs, A22, lda, B + hh, ldb);
+        return trsm_left_t(h, s, upper, trans, unit, m - hh, nrhs, A22, lda, B + hh, ldb, inv2);
     }
-    OXI_TRY(trsm_left_t(h, s, upper, trans, unit, m - hh, nrhs, A22, lda, B + hh, ldb));
+    OXI_TRY(trsm_left_t(h, s, upper, trans, unit, m - hh, nrhs, A22, lda, B + hh, ldb, inv2));
     // B1 -= op(A)[:hh, hh:] * B2 ; op(A)[0][hh] = trans ? A[hh][0] : A[0][hh]
     const TY* A12 = trans ? A + hh : A + hh * lda;
     OXI_TRY(gemm_t<TY>(h, s, trans, 0, hh, nrhs, m - hh, TY(-1), A12, lda, B + hh, ldb, TY(1), B, ldb));
-    return trsm_left_t(h, s, upper, trans, unit, hh, nrhs, A11, lda, B, ldb);
+    return trsm_left_t(h, s, upper, trans, unit, hh, nrhs, A11, lda, B, ldb, inv);
 }
 
 // Right-side solve with op(T) lower: X <- X * op(T)^-1.  op(T) = [L11 0; L21 L22]:
@@ -362,7 +392,8 @@ static int trsm_right_t(oxi_handle_t h, hipStream_t s, int upper, int trans, int
 
 int oxi_dtrsm_left(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 nrhs, const double* A,
                    i64 lda, double* B, i64 ldb) {
-    return trsm_left_t<double>(h, s, upper, trans, unit, m, nrhs, A, lda, B, ldb);
+    const double* inv = left_inverses(h, s, upper, trans, unit, m, nrhs, A, lda);
+    return trsm_left_t<double>(h, s, upper, trans, unit, m, nrhs, A, lda, B, ldb, inv);
 }
 int oxi_dtrsm_right(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 n, const double* A,
                     i64 lda, double* B, i64 ldb) {

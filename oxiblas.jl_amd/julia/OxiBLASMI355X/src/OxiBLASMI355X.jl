@@ -10,11 +10,13 @@
 module OxiBLASMI355X
 
 using LinearAlgebra
-import LinearAlgebra: ldiv!, issuccess, Factorization, SingularException
+import LinearAlgebra: ldiv!, issuccess, Factorization, SingularException, PosDefException
 import ChainRulesCore
 import ChainRulesCore: rrule, NoTangent, unthunk, @thunk
 
-export DeviceMatrix, gemm!, lu, lu!, ldiv, solve, solve_transposed, mixed_solve, equilibrate
+export DeviceMatrix, gemm!, lu, lu!, ldiv, solve, solve_transposed, mixed_solve, equilibrate,
+       syrk!, herk!, gemm_batched!, gemm_strided_batched!, Cholesky, cholesky, cholesky!, cholesky_factor,
+       native_inverse, backward_error, inverse_norm_estimate, reciprocal_condition, refine_solve, expert_solve
 
 const LIB = get(ENV, "OXIBLAS_MI355X_LIB",
                 normpath(joinpath(@__DIR__, "..", "..", "..", "liboxiblas_mi355x.so")))
@@ -232,7 +234,9 @@ function _getrs!(F::LU, B::DeviceMatrix, trans::Char)
     B
 end
 
-# ldiv!(F, B) and its transpose/adjoint (src/api/linalg_api.jl:168-184)
+# ldiv!(F, B) and its transpose/adjoint (src/api/linalg_api.jl:168-184); a
+# complex right-hand side on the real factors is solved as its real and
+# imaginary parts (_apply_split!, :67-78)
 ldiv!(F::LU, B::DeviceMatrix) = _getrs!(F, B, 'N')
 ldiv!(F::LinearAlgebra.TransposeFactorization{Float64,LU}, B::DeviceMatrix) = _getrs!(parent(F), B, 'T')
 ldiv!(F::LinearAlgebra.AdjointFactorization{Float64,LU}, B::DeviceMatrix) = _getrs!(parent(F), B, 'C')
@@ -248,7 +252,15 @@ end
 
 A new solution of F \\ B (OxiBLAS.ldiv, src/api/linalg_api_solve.jl:188-199).
 """
+function ldiv!(F::Union{LU,LinearAlgebra.TransposeFactorization{Float64,LU},LinearAlgebra.AdjointFactorization{Float64,LU}},
+               B::StridedVecOrMat{ComplexF64})
+    re = ldiv!(F, Matrix{Float64}(real(reshape(B, size(B, 1), :))))
+    im_ = ldiv!(F, Matrix{Float64}(imag(reshape(B, size(B, 1), :))))
+    copyto!(B, vec(complex.(re, im_)))
+end
+
 ldiv(F, B::StridedVecOrMat{Float64}) = ldiv!(F, copy(B))
+ldiv(F, B::StridedVecOrMat{ComplexF64}) = ldiv!(F, copy(B))
 ldiv(F, B::DeviceMatrix) = ldiv!(F, _copy(B))
 Base.:\(F::LU, B::StridedVecOrMat{Float64}) = ldiv(F, B)
 Base.:\(F::LU, B::DeviceMatrix) = ldiv(F, B)
@@ -261,16 +273,12 @@ function solve_transposed(F::LU, b; trans=:C)                           # src/la
 end
 
 function LinearAlgebra.logabsdet(F::LU)                                 # src/api/linalg_api.jl:185-191
-    _ready(F) === nothing
-    d = diag(Array(F.factors)); p = F.p
-    s = prod(sign, d)
-    cycles = 0; seen = falses(length(p))
-    for i in eachindex(p)
-        seen[i] && continue
-        cycles += 1; j = i
-        while !seen[j]; seen[j] = true; j = p[j]; end
-    end
-    (sum(log ∘ abs, d), isodd(length(p) - cycles) ? -s : s)
+    m, n = size(F); m == n || throw(DimensionMismatch("matrix is not square"))
+    la = Ref{Float64}(0); sg = Ref{Float64}(0); h = handle()
+    rc = ccall((:oxi_dlogabsdet, LIB), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}, Int64, Ptr{Int64}, Ref{Float64}, Ref{Float64}),
+               h, n, F.factors.ptr, F.factors.ld, getfield(F, :ipiv), la, sg)
+    _check(h, rc, "logabsdet")
+    (la[], sg[])          # a singular factor gives (-Inf, 0.0), no exception
 end
 LinearAlgebra.det(F::LU) = F.info != 0 ? 0.0 : (la = logabsdet(F); la[2] * exp(la[1]))
 
@@ -365,7 +373,8 @@ classical refinement stalls.  Returns the reference's NamedTuple.
 function mixed_solve(A::StridedMatrix{Float64}, b::StridedVecOrMat{Float64}; maxiter::Integer=10,
                      rtol::Real=1e-13, lowtype=Float32, solver::Symbol=:ir, gmres_restart::Integer=0,
                      factorization::Symbol=:lu, pivot::Symbol=:partial)
-    factorization === :lu || throw(ArgumentError("the GPU mixed solve implements factorization=:lu"))
+    factorization === :cholesky && return _mixed_cholesky(A, b; maxiter, rtol, lowtype)
+    factorization === :lu || throw(ArgumentError("the GPU mixed solve implements factorization=:lu and :cholesky"))
     n = size(A, 1); size(A, 2) == n || throw(DimensionMismatch("mixed factorization dimensions differ"))
     B = reshape(b, size(b, 1), :); k = size(B, 2)
     Ad = DeviceMatrix(Matrix{Float64}(A)); Bd = DeviceMatrix(Matrix{Float64}(B)); Xd = DeviceMatrix(n, k)
@@ -400,6 +409,259 @@ function equilibrate(A::StridedMatrix{Float64}; radix::Bool=false)
                h, m, n, Ad.ptr, Ad.ld, Ed.ptr, Ed.ld, r.ptr, c.ptr, Cint(radix))
     _check(h, rc, "equilibrate")
     (; matrix=Array(Ed), row_scale=vec(Array(r))[1:m], column_scale=vec(Array(c))[1:n])
+end
+
+# ---------------------------------------------------------------------------
+# what is built on the LU factors (src/lapack/dense_utilities.jl:14-36,
+# src/api/linalg_api.jl:1101-1143)
+# ---------------------------------------------------------------------------
+function Base.inv(F::LU)                                                 # native_inverse from the factors
+    _ready(F)
+    n = size(F, 1); X = DeviceMatrix(n, n); h = handle()
+    rc = ccall((:oxi_dgetri, LIB), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}, Int64, Ptr{Int64}, Ptr{Float64}, Int64),
+               h, n, F.factors.ptr, F.factors.ld, getfield(F, :ipiv), X.ptr, X.ld)
+    rc == 4 && throw(SingularException(F.info))
+    _check(h, rc, "inv")
+    Array(X)
+end
+native_inverse(A::StridedMatrix{Float64}) = inv(lu(A))
+LinearAlgebra.det(A::DeviceMatrix) = det(lu(A; check=false))
+LinearAlgebra.logabsdet(A::DeviceMatrix) = logabsdet(lu(A; check=false))
+
+# ---------------------------------------------------------------------------
+# syrk!/herk! (src/blas/level3.jl:779-781), batched GEMM (src/blas/batched.jl)
+# ---------------------------------------------------------------------------
+function _rankk!(C::DeviceMatrix, A::DeviceMatrix, α, β, uplo, t)
+    n = C.m; C.n == n || throw(DimensionMismatch("C must be square"))
+    k = t == 'N' ? A.n : A.m
+    (t == 'N' ? A.m : A.n) == n || throw(DimensionMismatch("rank-k dimensions differ"))
+    h = handle()
+    rc = ccall((:oxi_dsyrk, LIB), Cint,
+               (Ptr{Cvoid}, Cchar, Cchar, Int64, Int64, Float64, Ptr{Float64}, Int64, Float64, Ptr{Float64}, Int64),
+               h, uplo === :U ? 'U' : 'L', t, n, k, Float64(α), A.ptr, A.ld, Float64(β), C.ptr, C.ld)
+    _check(h, rc, "syrk!")
+    C
+end
+function syrk!(C::DeviceMatrix, A::DeviceMatrix, α=1.0, β=0.0; uplo=:U, trans=:N)
+    t = _trans(trans); t == 'C' && throw(ArgumentError("symmetric rank-k accepts N or T"))
+    _rankk!(C, A, α, β, uplo, t)
+end
+function herk!(C::DeviceMatrix, A::DeviceMatrix, α=1.0, β=0.0; uplo=:U, trans=:N)
+    t = _trans(trans); t == 'T' && throw(ArgumentError("Hermitian rank-k accepts N or C"))
+    _rankk!(C, A, α, β, uplo, t)
+end
+for f in (:syrk!, :herk!)
+    @eval function $f(C::StridedMatrix{Float64}, A::StridedMatrix{Float64}, α=1.0, β=0.0; kwargs...)
+        Cd = DeviceMatrix(C); $f(Cd, DeviceMatrix(A), α, β; kwargs...); copyto!(C, Array(Cd))
+    end
+end
+
+function gemm_strided_batched!(C::Array{Float64,3}, A::Array{Float64,3}, B::Array{Float64,3}, α=1.0, β=0.0;
+                               transA=:N, transB=:N)
+    ta, tb = _trans(transA), _trans(transB)
+    size(C, 3) == size(A, 3) == size(B, 3) || throw(DimensionMismatch("batch counts differ"))
+    am, ak = ta == 'N' ? (size(A, 1), size(A, 2)) : (size(A, 2), size(A, 1))
+    bk, bn = tb == 'N' ? (size(B, 1), size(B, 2)) : (size(B, 2), size(B, 1))
+    ak == bk && (size(C, 1), size(C, 2)) == (am, bn) || throw(DimensionMismatch("batched GEMM dimensions differ"))
+    nb = size(C, 3)
+    up(X) = DeviceMatrix(reshape(X, size(X, 1), :))            # the batch as one device block
+    Ad, Bd, Cd = up(A), up(B), up(C)
+    h = handle()
+    rc = ccall((:oxi_dgemm_strided_batched, LIB), Cint,
+               (Ptr{Cvoid}, Cchar, Cchar, Int64, Int64, Int64, Float64, Ptr{Float64}, Int64, Int64, Ptr{Float64}, Int64,
+                Int64, Float64, Ptr{Float64}, Int64, Int64, Int64),
+               h, ta, tb, size(C, 1), size(C, 2), ak, Float64(α), Ad.ptr, max(1, size(A, 1)), size(A, 1) * size(A, 2),
+               Bd.ptr, max(1, size(B, 1)), size(B, 1) * size(B, 2), Float64(β), Cd.ptr, max(1, size(C, 1)),
+               size(C, 1) * size(C, 2), nb)
+    _check(h, rc, "gemm_strided_batched!")
+    copyto!(C, reshape(Array(Cd), size(C)))
+end
+
+function gemm_batched!(Cs::Vector{DeviceMatrix}, As::Vector{DeviceMatrix}, Bs::Vector{DeviceMatrix}, α=1.0, β=0.0;
+                       transA=:N, transB=:N)
+    ta, tb = _trans(transA), _trans(transB)
+    length(Cs) == length(As) == length(Bs) || throw(DimensionMismatch("batch counts differ"))
+    groups = Dict{NTuple{6,Int},Vector{Int}}()
+    for i in eachindex(Cs)
+        A, B, C = As[i], Bs[i], Cs[i]
+        ak = ta == 'N' ? A.n : A.m
+        push!(get!(groups, (C.m, C.n, ak, A.ld, B.ld, C.ld), Int[]), i)
+    end
+    h = handle()
+    for ((m, n, k, lda, ldb, ldc), idx) in groups                 # one launch per shape
+        pa = [As[i].ptr for i in idx]; pb = [Bs[i].ptr for i in idx]; pc = [Cs[i].ptr for i in idx]
+        rc = ccall((:oxi_dgemm_batched, LIB), Cint,
+                   (Ptr{Cvoid}, Cchar, Cchar, Int64, Int64, Int64, Float64, Ptr{Ptr{Float64}}, Int64, Ptr{Ptr{Float64}},
+                    Int64, Float64, Ptr{Ptr{Float64}}, Int64, Int64),
+                   h, ta, tb, m, n, k, Float64(α), pa, lda, pb, ldb, Float64(β), pc, ldc, length(idx))
+        _check(h, rc, "gemm_batched!")
+    end
+    Cs
+end
+
+# ---------------------------------------------------------------------------
+# Cholesky (src/lapack/factorizations.jl:206-300, src/api/linalg_api.jl:221-296)
+# ---------------------------------------------------------------------------
+mutable struct Cholesky <: Factorization{Float64}
+    factors::DeviceMatrix      # L in the lower triangle, zero above
+    info::Int
+end
+Base.size(F::Cholesky) = size(F.factors)
+issuccess(F::Cholesky) = F.info == 0
+function Base.getproperty(F::Cholesky, s::Symbol)
+    (s === :L || s === :UL) && return LowerTriangular(Array(getfield(F, :factors)))
+    s === :U && return UpperTriangular(Matrix(transpose(Array(getfield(F, :factors)))))
+    s === :uplo && return 'L'
+    getfield(F, s)
+end
+function _potrf!(A::DeviceMatrix, check::Bool)
+    A.m == A.n || throw(DimensionMismatch("matrix is not square"))
+    info = Ref{Int64}(0); h = handle()
+    rc = ccall((:oxi_dpotrf, LIB), Cint, (Ptr{Cvoid}, Cchar, Int64, Ptr{Float64}, Int64, Ref{Int64}),
+               h, 'L', A.n, A.ptr, A.ld, info)
+    _check(h, rc, "cholesky")
+    check && info[] != 0 && throw(PosDefException(Int(info[])))
+    Cholesky(A, Int(info[]))
+end
+function cholesky(A::StridedMatrix{Float64}; check::Bool=true)
+    LinearAlgebra.checksquare(A)
+    if !ishermitian(A)
+        check && throw(PosDefException(-1))
+        return Cholesky(DeviceMatrix(A), -1)
+    end
+    _potrf!(DeviceMatrix(A), check)
+end
+cholesky(A::DeviceMatrix; check::Bool=true) = _potrf!(_copy(A), check)
+cholesky!(A::DeviceMatrix; check::Bool=true) = _potrf!(A, check)
+cholesky_factor(A::StridedMatrix{Float64}) = _potrf!(DeviceMatrix(A), true)
+function ldiv!(F::Cholesky, B::DeviceMatrix)
+    F.info == 0 || throw(PosDefException(F.info))
+    B.m == F.factors.m || throw(DimensionMismatch("Cholesky solve dimensions differ"))
+    h = handle()
+    rc = ccall((:oxi_dpotrs, LIB), Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64),
+               h, 'L', F.factors.m, B.n, F.factors.ptr, F.factors.ld, B.ptr, B.ld)
+    _check(h, rc, "ldiv!")
+    B
+end
+function ldiv!(F::Cholesky, B::StridedVecOrMat{Float64})
+    Bd = DeviceMatrix(reshape(B, size(B, 1), :)); ldiv!(F, Bd); copyto!(B, vec(Array(Bd)))
+end
+Base.:\(F::Cholesky, B::StridedVecOrMat{Float64}) = ldiv!(F, copy(B))
+solve(F::Cholesky, b) = F \ b
+function LinearAlgebra.logabsdet(F::Cholesky)                          # linalg_api.jl:247-251
+    F.info == 0 || throw(PosDefException(F.info))
+    (2 * sum(log, diag(Array(F.factors))), 1.0)
+end
+
+# ---------------------------------------------------------------------------
+# expert drivers and estimates (src/lapack/expert_refinement.jl,
+# src/lapack/advanced_factorizations.jl:591-645, dense_utilities.jl:214-228)
+# ---------------------------------------------------------------------------
+struct RefineInfo
+    iterations::Int64
+    converged::Int32
+    reserved::Int32
+    backward_error::Float64
+    residual_norm::Float64
+    rcond_estimate::Float64
+    forward_error_estimate::Float64
+end
+RefineInfo() = RefineInfo(0, 0, 0, 0.0, 0.0, 0.0, 0.0)
+
+function backward_error(A::StridedMatrix{Float64}, x::StridedVecOrMat{Float64}, b::StridedVecOrMat{Float64})
+    X = reshape(x, size(x, 1), :); B = reshape(b, size(b, 1), :)
+    size(A, 2) == size(X, 1) && size(B) == (size(A, 1), size(X, 2)) || throw(DimensionMismatch("residual dimensions differ"))
+    Ad, Xd, Bd = DeviceMatrix(Matrix(A)), DeviceMatrix(Matrix(X)), DeviceMatrix(Matrix(B))
+    Rd = DeviceMatrix(size(B)...); e = Ref{Float64}(0); h = handle()
+    rc = ccall((:oxi_dbackward_error, LIB), Cint,
+               (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64,
+                Ptr{Float64}, Int64, Ref{Float64}),
+               h, size(A, 1), size(A, 2), size(B, 2), Ad.ptr, Ad.ld, Xd.ptr, Xd.ld, Bd.ptr, Bd.ld, Rd.ptr, Rd.ld, e)
+    _check(h, rc, "backward_error")
+    R = Array(Rd)
+    (; error=e[], residual=ndims(b) == 1 ? vec(R) : R)
+end
+
+function inverse_norm_estimate(F::LU, ::Type{Float64}, n::Integer; maxiter=8)
+    n == size(F, 1) || throw(DimensionMismatch("n differs from the factorization"))
+    est = Ref{Float64}(0); h = handle()
+    rc = ccall((:oxi_dinverse_norm_estimate, LIB), Cint,
+               (Ptr{Cvoid}, Int64, Ptr{Float64}, Int64, Ptr{Int64}, Int32, Ref{Float64}),
+               h, n, F.factors.ptr, F.factors.ld, getfield(F, :ipiv), Int32(maxiter), est)
+    _check(h, rc, "inverse_norm_estimate")
+    est[]
+end
+function reciprocal_condition(A::StridedMatrix{Float64}; maxiter=8)
+    n = LinearAlgebra.checksquare(A); n == 0 && return 1.0
+    F = lu(A; check=false); F.info != 0 && return 0.0
+    v = opnorm(A, 1) * inverse_norm_estimate(F, Float64, n; maxiter)
+    iszero(v) ? 1.0 : min(1.0, 1 / v)
+end
+
+function refine_solve(A::StridedMatrix{Float64}, F::LU, b::StridedVecOrMat{Float64}; maxiter=8, rtol=8eps())
+    n = LinearAlgebra.checksquare(A); B = reshape(b, n, :)
+    Ad, Bd, Xd = DeviceMatrix(Matrix(A)), DeviceMatrix(Matrix(B)), DeviceMatrix(n, size(B, 2))
+    info = Ref(RefineInfo()); h = handle()
+    rc = ccall((:oxi_drefine_solve, LIB), Cint,
+               (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Int64}, Ptr{Float64}, Int64,
+                Ptr{Float64}, Int64, Int32, Float64, Ref{RefineInfo}),
+               h, n, size(B, 2), Ad.ptr, Ad.ld, F.factors.ptr, F.factors.ld, getfield(F, :ipiv), Bd.ptr, Bd.ld, Xd.ptr,
+               Xd.ld, Int32(maxiter), Float64(rtol), info)
+    rc == 4 && throw(SingularException(F.info))
+    _check(h, rc, "refine_solve")
+    r = info[]; X = Array(Xd)
+    (; x=ndims(b) == 1 ? vec(X) : X, converged=r.converged != 0, iterations=Int(r.iterations),
+     backward_error=r.backward_error, residual_norm=r.residual_norm, rcond_estimate=r.rcond_estimate,
+     forward_error_estimate=r.forward_error_estimate)
+end
+
+function expert_solve(A::StridedMatrix{Float64}, b::StridedVecOrMat{Float64}; maxiter=8, rtol=8eps(),
+                      equilibrate=true, pivot=:partial)
+    pivot === :partial || throw(ArgumentError("expert_solve implements pivot=:partial"))
+    n = LinearAlgebra.checksquare(A); B = reshape(b, n, :)
+    Ad, Bd, Xd = DeviceMatrix(Matrix(A)), DeviceMatrix(Matrix(B)), DeviceMatrix(n, size(B, 2))
+    LUd = DeviceMatrix(n, n); rows = DeviceMatrix(max(n, 1), 1); cols = DeviceMatrix(max(n, 1), 1)
+    h = handle(); piv = Ref{Ptr{Cvoid}}(C_NULL)
+    _check(h, ccall((:oxi_malloc, LIB), Cint, (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, UInt64), h, piv, 8 * max(n, 1)), "malloc")
+    info = Ref(RefineInfo())
+    rc = ccall((:oxi_dexpert_solve, LIB), Cint,
+               (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Int32, Float64,
+                Int32, Ptr{Float64}, Int64, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ref{RefineInfo}),
+               h, n, size(B, 2), Ad.ptr, Ad.ld, Bd.ptr, Bd.ld, Xd.ptr, Xd.ld, Int32(maxiter), Float64(rtol),
+               Int32(equilibrate), LUd.ptr, LUd.ld, Ptr{Int64}(piv[]), rows.ptr, cols.ptr, info)
+    if rc == 4
+        m = match(r"U\((\d+),", lasterror(h))
+        throw(SingularException(m === nothing ? 0 : parse(Int, m.captures[1])))
+    end
+    _check(h, rc, "expert_solve")
+    F = LU(LUd, Ptr{Int64}(piv[]), 0)
+    finalizer(f -> ccall((:oxi_free, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), handle(), getfield(f, :ipiv)), F)
+    r = info[]; X = Array(Xd)
+    (; x=ndims(b) == 1 ? vec(X) : X, converged=r.converged != 0, iterations=Int(r.iterations),
+     backward_error=r.backward_error, row_scale=vec(Array(rows))[1:n], column_scale=vec(Array(cols))[1:n], factor=F)
+end
+
+# mixed_solve(A, b; factorization=:cholesky): oxi_dposv_mixed
+function _mixed_cholesky(A, b; maxiter, rtol, lowtype)
+    lowtype === Float32 || throw(ArgumentError("the Cholesky mixed solve takes Float32 low factors"))
+    n = LinearAlgebra.checksquare(A); B = reshape(b, n, :); k = size(B, 2)
+    Ad, Bd, Xd = DeviceMatrix(Matrix{Float64}(A)), DeviceMatrix(Matrix{Float64}(B)), DeviceMatrix(n, k)
+    info = Ref(IRInfo()); h = handle()
+    rc = ccall((:oxi_dposv_mixed, LIB), Cint,
+               (Ptr{Cvoid}, Cint, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Int32,
+                Float64, Ref{IRInfo}),
+               h, Cint(3), n, k, Ad.ptr, Ad.ld, Bd.ptr, Bd.ld, Xd.ptr, Xd.ld, Int32(maxiter), Float64(rtol), info)
+    rc == 6 && throw(ArgumentError("finite mixed input required"))
+    if rc == 8
+        m = match(r"A\((\d+),", lasterror(h))
+        throw(PosDefException(m === nothing ? 0 : parse(Int, m.captures[1])))
+    end
+    _check(h, rc, "mixed_solve")
+    r = info[]; X = Array(Xd)
+    (; x=ndims(b) == 1 ? vec(X) : X, converged=r.converged != 0, iterations=Int(r.iterations),
+     residual=r.residual_norm, backward_error=r.backward_error, metric=:componentwise_backward_error,
+     low_type=Float32, fallback=r.fallback != 0, factorization=:cholesky, finite=r.reason != 2,
+     reason=_REASONS[r.reason + 1])
 end
 
 end # module

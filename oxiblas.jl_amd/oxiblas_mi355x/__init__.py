@@ -11,11 +11,16 @@ from ._lib import load, handle, Handle, OxiError, LIB_PATH, SIGNATURES  # noqa: 
 from .device import DMatrix, as_dmatrix  # noqa: F401
 from .linalg import (  # noqa: F401
     gemm_, trsm_, laswp_, lu, lu_, lu_factor, LU, ldiv, ldiv_, solve, solve_transposed,
-    transpose, adjoint, SingularException, DimensionMismatch,
+    transpose, adjoint, SingularException, DimensionMismatch, det, logdet, logabsdet, inv, factorize,
+)
+from .expert import (  # noqa: F401
+    backward_error, inverse_norm_estimate, reciprocal_condition, refine_solve, expert_solve,
 )
 from .mixed import (  # noqa: F401
     mixed_solve, equilibrate, sgemm_, gemm_lowp_update_, lu_lowp, LUFactor32, MixedSolveResult, precision_code,
 )
+from .blas3 import DBatch, gemm_strided_batched_, gemm_batched_, syrk_, herk_, hgemm_  # noqa: F401
+from .cholesky import Cholesky, PosDefException, cholesky, cholesky_, cholesky_factor  # noqa: F401
 from .pullbacks import ldiv_pullback, rrule_ldiv, gemm_pullback, rrule_mul  # noqa: F401
 
 __all__ = [
@@ -24,4 +29,8 @@ __all__ = [
     "transpose", "adjoint", "SingularException", "DimensionMismatch",
     "mixed_solve", "equilibrate", "sgemm_", "gemm_lowp_update_", "lu_lowp", "LUFactor32", "MixedSolveResult",
     "precision_code", "ldiv_pullback", "rrule_ldiv", "gemm_pullback", "rrule_mul",
+    "DBatch", "gemm_strided_batched_", "gemm_batched_", "syrk_", "herk_", "hgemm_",
+    "Cholesky", "PosDefException", "cholesky", "cholesky_", "cholesky_factor",
+    "det", "logdet", "logabsdet", "inv", "factorize",
+    "backward_error", "inverse_norm_estimate", "reciprocal_condition", "refine_solve", "expert_solve",
 ]

@@ -18,6 +18,7 @@ OXI_ERR_SINGULAR = 4
 OXI_ERR_TIMEOUT = 5
 OXI_ERR_NOT_FINITE = 6
 OXI_ERR_OVERFLOW = 7
+OXI_ERR_NOT_POSDEF = 8
 
 _i64 = C.c_int64
 _vp = C.c_void_p
@@ -47,6 +48,20 @@ class IRInfo(C.Structure):
         ("refine_ms", C.c_double),
     ]
 
+class RefineInfo(C.Structure):
+    """oxi_refine_info: what refine_solve / expert_solve report
+    (src/lapack/expert_refinement.jl:103, advanced_factorizations.jl:644)."""
+    _fields_ = [
+        ("iterations", C.c_int64),
+        ("converged", C.c_int32),
+        ("reserved", C.c_int32),
+        ("backward_error", C.c_double),
+        ("residual_norm", C.c_double),
+        ("rcond_estimate", C.c_double),
+        ("forward_error_estimate", C.c_double),
+    ]
+
+
 # name -> (restype, argtypes); every symbol the header declares
 SIGNATURES = {
     "oxi_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
@@ -59,17 +74,39 @@ SIGNATURES = {
     "oxi_free": (C.c_int, [_vp, _vp]),
     "oxi_memcpy": (C.c_int, [_vp, _vp, _vp, C.c_uint64, C.c_int]),
     "oxi_dgemm": (C.c_int, [_vp, _ch, _ch, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64, _dbl, _vp, _i64]),
+    "oxi_dgemm_strided_batched": (C.c_int, [_vp, _ch, _ch, _i64, _i64, _i64, _dbl, _vp, _i64, _i64, _vp, _i64, _i64,
+                                            _dbl, _vp, _i64, _i64, _i64]),
+    "oxi_dgemm_batched": (C.c_int, [_vp, _ch, _ch, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64, _dbl, _vp, _i64,
+                                    _i64]),
+    "oxi_dsyrk": (C.c_int, [_vp, _ch, _ch, _i64, _i64, _dbl, _vp, _i64, _dbl, _vp, _i64]),
+    "oxi_dlogabsdet": (C.c_int, [_vp, _i64, _vp, _i64, _vp, C.POINTER(_dbl), C.POINTER(_dbl)]),
+    "oxi_dgetri": (C.c_int, [_vp, _i64, _vp, _i64, _vp, _vp, _i64]),
+    "oxi_dbackward_error": (C.c_int, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64,
+                                      C.POINTER(_dbl)]),
+    "oxi_dinverse_norm_estimate": (C.c_int, [_vp, _i64, _vp, _i64, _vp, C.c_int32, C.POINTER(_dbl)]),
+    "oxi_drefine_solve": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _i64, C.c_int32,
+                                    _dbl, C.POINTER(RefineInfo)]),
+    "oxi_dexpert_solve": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, C.c_int32, _dbl, C.c_int32,
+                                    _vp, _i64, _vp, _vp, _vp, C.POINTER(RefineInfo)]),
+    "oxi_dpotrf": (C.c_int, [_vp, _ch, _i64, _vp, _i64, C.POINTER(_i64)]),
+    "oxi_dpotrs": (C.c_int, [_vp, _ch, _i64, _i64, _vp, _i64, _vp, _i64]),
     "oxi_dtrsm": (C.c_int, [_vp, _ch, _ch, _ch, _ch, _i64, _i64, _dbl, _vp, _i64, _vp, _i64, C.POINTER(_i64)]),
     "oxi_dlaswp": (C.c_int, [_vp, _i64, _vp, _i64, _i64, _i64, _vp, C.c_int]),
     "oxi_dgetrf": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, C.POINTER(_i64)]),
     "oxi_dgetrs": (C.c_int, [_vp, _ch, _i64, _i64, _vp, _i64, _vp, _vp, _i64]),
+    "oxi_dgetrs_plan_create": (C.c_int, [_vp, _i64, _vp, _i64, _vp, C.POINTER(_vp)]),
+    "oxi_dgetrs_plan": (C.c_int, [_vp, _vp, _ch, _i64, _vp, _i64]),
+    "oxi_dgetrs_plan_destroy": (C.c_int, [_vp, _vp]),
     "oxi_dgesv_adjoint": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64]),
     "oxi_sgemm": (C.c_int, [_vp, _ch, _ch, _i64, _i64, _i64, _flt, _vp, _i64, _vp, _i64, _flt, _vp, _i64]),
+    "oxi_hgemm": (C.c_int, [_vp, _ch, _ch, _i64, _i64, _i64, _flt, _vp, _i64, _vp, _i64, _flt, _vp, _i64]),
     "oxi_gemm_lowp_update": (C.c_int, [_vp, C.c_int, _i64, _i64, _i64, _flt, _vp, _i64, _vp, _i64, _vp, _i64]),
     "oxi_sgetrf_lowp": (C.c_int, [_vp, C.c_int, _i64, _i64, _vp, _i64, _vp, _i64, C.POINTER(_i64)]),
     "oxi_sgetrs": (C.c_int, [_vp, _ch, _i64, _i64, _vp, _i64, _vp, _vp, _i64]),
     "oxi_dgesv_mixed": (C.c_int, [_vp, C.c_int, C.c_int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, C.c_int32,
                                   _dbl, C.c_int32, C.c_int32, C.POINTER(IRInfo)]),
+    "oxi_dposv_mixed": (C.c_int, [_vp, C.c_int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, C.c_int32, _dbl,
+                                  C.POINTER(IRInfo)]),
     "oxi_dequilibrate": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, C.c_int]),
     "oxi_profile_enable": (C.c_int, [_vp, C.c_int]),
     "oxi_profile_read": (C.c_int, [_vp, C.POINTER(_dbl), C.POINTER(_dbl), C.POINTER(_i64)]),

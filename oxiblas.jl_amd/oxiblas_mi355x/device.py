@@ -114,7 +114,7 @@ class DMatrix:
     @property
     def np_dtype(self):
         torch = _torch()
-        return np.float64 if self.buf.dtype == torch.float64 else np.float32
+        return {torch.float64: np.float64, torch.float32: np.float32, torch.float16: np.float16}[self.buf.dtype]
 
     def __repr__(self):
         return f"DMatrix({self.m}x{self.n}, ld={self.ld}, {self.buf.dtype})"

@@ -151,6 +151,27 @@ class LU:
         self.ipiv_dev = ipiv_dev    # torch int64 tensor, 1-based global rows
         self.info = int(info)
         self._ipiv = None
+        self._plan = None           # solve plan of these factors, made by the first solve
+
+    def _solve_plan(self):
+        """The factors' solve plan (device permutation and the inverses of
+        the diagonal blocks of both triangles), made once."""
+        if self._plan is None:
+            h = _h()
+            p = C.c_void_p()
+            rc = h._L.oxi_dgetrs_plan_create(h.ptr, self.factors.m, self.factors.ptr, self.factors.ld,
+                                             self.ipiv_dev.data_ptr(), C.byref(p))
+            h.check(rc, "ldiv!")
+            self._plan = (h, p)
+        return self._plan[1]
+
+    def __del__(self):
+        plan = getattr(self, "_plan", None)
+        if plan is not None:
+            try:
+                plan[0]._L.oxi_dgetrs_plan_destroy(plan[0].ptr, plan[1])
+            except Exception:
+                pass
 
     # properties of the reference (propertynames: L, U, p, P; private: factors, ipiv, info)
     @property
@@ -206,17 +227,25 @@ class LU:
             raise SingularException(self.info)  # src/api/linalg_api.jl:163-167
 
     def ldiv_(self, B, trans="N"):
-        """In-place solve op(A) X = B (ldiv!(F, B))."""
+        """In-place solve op(A) X = B (ldiv!(F, B)).  A complex host B on these
+        real factors is solved as its real and imaginary parts
+        (_apply_split!, src/api/linalg_api.jl:67-78)."""
         self._ready()
+        if isinstance(B, np.ndarray) and np.iscomplexobj(B):
+            X = self.solve(B.real, trans) + 1j * self.solve(B.imag, trans)
+            if B.flags.writeable:
+                B[...] = X
+                return B
+            return X
         t = _trans(trans)
         host = not isinstance(B, DMatrix)
         vec = host and np.ndim(B) == 1
         Bd = as_dmatrix(B)
         if Bd.m != self.factors.m:
             raise DimensionMismatch(f"factorization has {self.factors.m} rows, right-hand side has {Bd.m}")
+        plan = self._solve_plan()
         h = _h()
-        rc = h._L.oxi_dgetrs(h.ptr, t.encode(), self.factors.m, Bd.n, self.factors.ptr, self.factors.ld,
-                             self.ipiv_dev.data_ptr(), Bd.ptr, Bd.ld)
+        rc = h._L.oxi_dgetrs_plan(h.ptr, plan, t.encode(), Bd.n, Bd.ptr, Bd.ld)
         h.check(rc, "ldiv!")
         if host:
             X = Bd.numpy()
@@ -231,6 +260,8 @@ class LU:
         """``F \\ B``: a new solution, B untouched."""
         if isinstance(B, DMatrix):
             return self.ldiv_(B.copy(), trans)
+        if np.iscomplexobj(B):
+            return self.ldiv_(np.array(B, dtype=np.complex128, copy=True), trans)
         return self.ldiv_(np.array(B, dtype=np.float64, copy=True), trans)
 
     def transpose(self):
@@ -244,32 +275,43 @@ class LU:
         return self.transpose()
 
     def logabsdet(self):
-        """(log|det|, sign) (src/api/linalg_api.jl:185-191)."""
+        """(log|det|, sign) (src/api/linalg_api.jl:185-191): one device
+        reduction over U's diagonal and the interchanges.  A singular factor
+        gives (-inf, 0.0), as the reference (no exception)."""
         m, n = self.factors.shape
         if m != n:
             raise DimensionMismatch("matrix is not square")
-        d = np.diag(self.factors.numpy())
-        s = np.prod(np.sign(d))
-        # parity of the permutation
-        p = self.p - 1
-        seen = np.zeros(n, dtype=bool)
-        cycles = 0
-        for i in range(n):
-            if not seen[i]:
-                cycles += 1
-                j = i
-                while not seen[j]:
-                    seen[j] = True
-                    j = p[j]
-        if (n - cycles) % 2:
-            s = -s
-        return float(np.sum(np.log(np.abs(d)))), float(s)
+        h = _h()
+        la, sg = C.c_double(0), C.c_double(0)
+        rc = h._L.oxi_dlogabsdet(h.ptr, n, self.factors.ptr, self.factors.ld, self.ipiv_dev.data_ptr(), C.byref(la),
+                                 C.byref(sg))
+        h.check(rc, "logabsdet")
+        return float(la.value), float(sg.value)
+
+    def logdet(self):
+        la, sg = self.logabsdet()
+        if sg < 0:
+            raise ValueError("logdet: negative determinant")   # DomainError
+        return la if sg > 0 else -np.inf
 
     def det(self):
         if self.info != 0:
             return 0.0
         la, s = self.logabsdet()
         return s * np.exp(la)
+
+    def inv(self):
+        """inv(A) from these factors: inv(U) inv(L) P (native_inverse,
+        src/lapack/dense_utilities.jl:14-36) as a new device matrix."""
+        self._ready()
+        n = self.factors.n
+        out = DMatrix.empty(n, n, device=self.factors.buf.device)
+        h = _h()
+        rc = h._L.oxi_dgetri(h.ptr, n, self.factors.ptr, self.factors.ld, self.ipiv_dev.data_ptr(), out.ptr, out.ld)
+        if rc == _lib.OXI_ERR_SINGULAR:
+            raise SingularException(self.info or 0)
+        h.check(rc, "inv")
+        return out
 
 
 class _TransposedLU:
@@ -361,3 +403,49 @@ def solve_transposed(F, b, trans="C"):
     if t == "N":
         return F.solve(b)
     return F.solve(b, t)
+
+
+# ---------------------------------------------------------------------------
+# what is built on lu (src/api/linalg_api.jl:1101-1143, dense_utilities.jl)
+# ---------------------------------------------------------------------------
+def det(A):
+    """OxiBLAS.det(A) = det(lu(A; check=false))."""
+    return A.det() if isinstance(A, LU) else lu(A, check=False).det()
+
+
+def logabsdet(A):
+    """OxiBLAS.logabsdet(A) = logabsdet(lu(A; check=false))."""
+    return A.logabsdet() if isinstance(A, LU) else lu(A, check=False).logabsdet()
+
+
+def logdet(A):
+    return A.logdet() if isinstance(A, LU) else lu(A, check=False).logdet()
+
+
+def inv(A):
+    """OxiBLAS.inv(A) = native_inverse(A) (dense_utilities.jl:14-36); a host
+    matrix gives a host matrix."""
+    if isinstance(A, LU):
+        return A.inv()
+    F = lu(A, check=True)
+    X = F.inv()
+    return X if isinstance(A, DMatrix) else X.numpy()
+
+
+def factorize(A):
+    """factorize(A) (src/api/linalg_api.jl:1101-1113) for a square matrix: a
+    symmetric matrix with a positive real diagonal is tried with Cholesky
+    (check=false) and kept when that succeeds; otherwise LU.  (The reference
+    turns a symmetric indefinite matrix into Bunch-Kaufman and a rectangular
+    one into pivoted QR, which are outside this library: those raise or give
+    LU.)"""
+    from .cholesky import cholesky
+    a = A.numpy() if isinstance(A, DMatrix) else np.asarray(A, dtype=np.float64)
+    m, n = a.shape
+    if m != n:
+        raise NotImplementedError("factorize of a rectangular matrix (pivoted QR) is not implemented")
+    if np.array_equal(a, a.T) and np.all(np.diag(a) > 0) and np.all(np.isfinite(np.diag(a))):
+        Fc = cholesky(A if isinstance(A, DMatrix) else a, check=False)
+        if Fc.issuccess():
+            return Fc
+    return lu(A)

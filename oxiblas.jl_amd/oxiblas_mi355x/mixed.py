@@ -92,12 +92,14 @@ def mixed_solve(A, b, maxiter=10, rtol=1e-13, lowtype=None, fallback=True, facto
     residual, backward_error, metric, low_type, fallback, factorization,
     finite, reason) plus the GPU's normwise error, GMRES iteration count and
     device timings."""
-    if factorization != "lu":
-        raise NotImplementedError("the GPU mixed solve implements factorization=:lu")
+    if factorization not in ("lu", "cholesky"):
+        raise NotImplementedError("the GPU mixed solve implements factorization=:lu and :cholesky")
     if not fallback:
         raise NotImplementedError("the GPU mixed solve always falls back to the working-precision factor")
     if maxiter < 0 or not np.isfinite(rtol) or rtol < 0:
         raise ValueError("invalid mixed solve options")
+    if factorization == "cholesky":
+        return _mixed_cholesky(A, b, maxiter, rtol, lowtype)
     if solver not in _SOLVERS:
         raise ValueError("solver must be ir, gmres or auto")
     if pivot not in _PIVOTS:
@@ -136,6 +138,48 @@ def mixed_solve(A, b, maxiter=10, rtol=1e-13, lowtype=None, fallback=True, facto
         reason=_REASONS[info.reason], normwise_backward_error=info.normwise_backward_error,
         inner_iterations=int(info.inner_iterations), solver="gmres" if info.solver else "ir",
         factor_ms=info.factor_ms, refine_ms=info.refine_ms)
+
+
+def _mixed_cholesky(A, b, maxiter, rtol, lowtype):
+    """mixed_solve(A, b; factorization=:cholesky): one global scale, the FP32
+    Cholesky (FP64 on PosDefException), classical refinement
+    (src/lapack/extended_lapack.jl:175-226)."""
+    from .cholesky import PosDefException
+    prec = precision_code(lowtype)
+    if prec != _lib.OXI_PREC_FP32:
+        raise ValueError("the Cholesky mixed solve takes Float32 low factors")
+    vec = not isinstance(b, DMatrix) and np.ndim(b) == 1
+    Ad, Bd = as_dmatrix(A), as_dmatrix(b)
+    if Ad.m != Ad.n:
+        raise DimensionMismatch("mixed factorization dimensions differ")
+    if Bd.m != Ad.m:
+        raise ValueError("invalid mixed solve options")
+    n, nrhs = Ad.n, Bd.n
+    Xd = DMatrix.empty(n, nrhs, device=Ad.buf.device) if n and nrhs else DMatrix.zeros(n, nrhs)
+    info = _lib.IRInfo()
+    h = _h()
+    rc = h._L.oxi_dposv_mixed(h.ptr, prec, n, nrhs, Ad.ptr, Ad.ld, Bd.ptr, Bd.ld, Xd.ptr, Xd.ld, int(maxiter),
+                              float(rtol), C.byref(info))
+    if rc == _lib.OXI_ERR_NOT_FINITE:
+        raise ValueError("finite mixed input required")
+    if rc == _lib.OXI_ERR_NOT_POSDEF:
+        m = re.search(r"A\((\d+),", h.last_error())
+        raise PosDefException(int(m.group(1)) if m else 0)
+    if rc == _lib.OXI_ERR_OVERFLOW:
+        raise OverflowError(h.last_error())
+    h.check(rc, "mixed_solve")
+    if isinstance(b, DMatrix):
+        x = Xd
+    else:
+        x = Xd.numpy()
+        if vec:
+            x = x[:, 0].copy()
+    return MixedSolveResult(
+        x=x, converged=bool(info.converged), iterations=int(info.iterations), residual=info.residual_norm,
+        backward_error=info.backward_error, metric="componentwise_backward_error", low_type=_PREC_LABEL[prec],
+        fallback=bool(info.fallback), factorization="cholesky", finite=info.reason != 2,
+        reason=_REASONS[info.reason], normwise_backward_error=info.normwise_backward_error,
+        inner_iterations=0, solver="ir", factor_ms=info.factor_ms, refine_ms=info.refine_ms)
 
 
 def equilibrate(A, radix=False):

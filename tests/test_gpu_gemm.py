@@ -225,3 +225,121 @@ def test_gemm_512_cube_beta_zero(gpu, rng):
             assert np.all(np.isfinite(got))
             np.testing.assert_allclose(got, ref, rtol=3e-12, atol=3e-12 * n)
             assert np.linalg.norm(got - ref) <= n * EPS * np.linalg.norm(A) * np.linalg.norm(B)
+
+
+# ---------------------------------------------------------------------------
+# batched products (src/blas/batched.jl) and the rank-k update (level3.jl:546-783)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("ta", "NT")
+@pytest.mark.parametrize("tb", "NT")
+def test_strided_batched_vs_oracle(gpu, rng, ta, tb):
+    for (m, n, k, b) in ((17, 19, 23, 5), (64, 64, 64, 33), (130, 70, 257, 3), (1, 300, 40, 2), (200, 200, 16, 40)):
+        A = rng.standard_normal(((m, k) if ta == "N" else (k, m)) + (b,))
+        B = rng.standard_normal(((k, n) if tb == "N" else (n, k)) + (b,))
+        C0 = rng.standard_normal((m, n, b))
+        got = gpu.gemm_strided_batched_(C0.copy(), A, B, 0.7, -0.4, ta, tb)
+        for z in range(b):
+            ref = np.asfortranarray(C0[:, :, z].copy())
+            O.dgemm(ref, A[:, :, z], B[:, :, z], 0.7, -0.4, ta, tb)
+            np.testing.assert_allclose(got[:, :, z], ref, rtol=3e-12, atol=3e-12 * k)
+
+
+def test_strided_batched_rules(gpu, rng):
+    ox = gpu
+    A = rng.standard_normal((9, 7, 4)); B = rng.standard_normal((7, 5, 4))
+    got = ox.gemm_strided_batched_(np.full((9, 5, 4), np.nan), A, B, 1.0, 0.0)   # beta 0 never reads C
+    assert np.all(np.isfinite(got))
+    got = ox.gemm_strided_batched_(np.full((9, 5, 4), 3.0), np.full((9, 7, 4), np.nan), np.full((7, 5, 4), np.nan),
+                                   0.0, 0.5)
+    assert np.array_equal(got, np.full((9, 5, 4), 1.5))                       # alpha 0 never reads A, B
+    # overlapping outputs are rejected before any write
+    Cd = ox.DBatch.from_numpy(np.zeros((9, 5, 4)))
+    Ad, Bd = ox.DBatch.from_numpy(A), ox.DBatch.from_numpy(B)
+    bad = ox.DBatch(Cd.buf, 9, 5, 4, stride=20)
+    with pytest.raises(ValueError):
+        ox.gemm_strided_batched_(bad, Ad, Bd)
+    assert np.array_equal(Cd.numpy(), np.zeros((9, 5, 4)))
+    with pytest.raises(ox.DimensionMismatch):
+        ox.gemm_strided_batched_(np.zeros((9, 5, 3)), A, B)
+
+
+def test_strided_batched_many_products(gpu, rng):
+    # more products than one grid dimension holds, 4 x 4 products
+    b = 70000
+    A = rng.standard_normal((4, 4, b)); B = rng.standard_normal((4, 4, b))
+    got = gpu.gemm_strided_batched_(np.zeros((4, 4, b)), A, B)
+    np.testing.assert_allclose(got, np.einsum("ikz,kjz->ijz", A, B), rtol=1e-12, atol=1e-12)
+
+
+def test_pointer_batched_mixed_shapes(gpu, rng):
+    ox = gpu
+    shapes = [(17, 19, 23), (64, 64, 64), (17, 19, 23), (5, 300, 7), (64, 64, 64), (64, 64, 64)]
+    As = [rng.standard_normal((m, k)) for (m, n, k) in shapes]
+    Bs = [rng.standard_normal((n, k)) for (m, n, k) in shapes]
+    Cs = [rng.standard_normal((m, n)) for (m, n, k) in shapes]
+    Ad = [ox.DMatrix.from_numpy(a) for a in As]
+    Bd = [ox.DMatrix.from_numpy(b) for b in Bs]
+    Cd = [ox.DMatrix.from_numpy(c) for c in Cs]
+    ox.gemm_batched_(Cd, Ad, Bd, 1.5, 0.5, "N", "T")
+    for i, (m, n, k) in enumerate(shapes):
+        ref = np.asfortranarray(Cs[i].copy())
+        O.dgemm(ref, As[i], Bs[i], 1.5, 0.5, "N", "T")
+        np.testing.assert_allclose(Cd[i].numpy(), ref, rtol=3e-12, atol=3e-12 * k)
+    # an input aliasing another member's output is read before that output is written
+    X = ox.DMatrix.from_numpy(rng.standard_normal((8, 8)))
+    Y = ox.DMatrix.from_numpy(rng.standard_normal((8, 8)))
+    x0, y0 = X.numpy(), Y.numpy()
+    ox.gemm_batched_([X, Y], [Y, X], [Y, X])
+    np.testing.assert_allclose(X.numpy(), y0 @ y0, rtol=1e-12)
+    np.testing.assert_allclose(Y.numpy(), x0 @ x0, rtol=1e-12)
+    with pytest.raises(ValueError):
+        ox.gemm_batched_([X, X], [Y, Y], [Y, Y])
+
+
+@pytest.mark.parametrize("uplo", "UL")
+@pytest.mark.parametrize("trans", "NT")
+def test_syrk_vs_oracle(gpu, rng, uplo, trans):
+    for (n, k) in ((1, 1), (17, 9), (64, 64), (130, 257), (300, 40), (1000, 128)):
+        A = rng.standard_normal((n, k) if trans == "N" else (k, n))
+        C0 = np.asfortranarray(rng.standard_normal((n, n)))
+        got = gpu.syrk_(C0.copy(order="F"), A, 0.7, -0.3, uplo, trans)
+        ref = C0.copy(order="F")
+        O.dsyrk(ref, A, 0.7, -0.3, uplo, trans)
+        np.testing.assert_allclose(got, ref, rtol=3e-12, atol=3e-12 * k)
+        mask = np.tril(np.ones((n, n), bool), -1) if uplo == "U" else np.triu(np.ones((n, n), bool), 1)
+        assert np.array_equal(got[mask], C0[mask])         # the other triangle is untouched
+    A = rng.standard_normal((40, 10))
+    got = gpu.syrk_(np.full((40, 40), np.nan), A, 1.0, 0.0, uplo)   # beta 0 never reads C
+    tri = np.triu_indices(40) if uplo == "U" else np.tril_indices(40)
+    assert np.all(np.isfinite(got[tri]))
+    got = gpu.syrk_(np.full((6, 6), 2.0), np.full((6, 3), np.nan), 0.0, 0.5, uplo)   # alpha 0 never reads A
+    expect = np.full((6, 6), 2.0)
+    t6 = np.triu_indices(6) if uplo == "U" else np.tril_indices(6)
+    expect[t6] = 1.0
+    assert np.array_equal(got, expect)
+    with pytest.raises(ValueError):
+        gpu.syrk_(np.zeros((4, 4)), np.zeros((4, 2)), trans="C")
+
+
+@pytest.mark.parametrize("ta", "NT")
+@pytest.mark.parametrize("tb", "NT")
+def test_hgemm_vs_oracle(gpu, rng, ta, tb):
+    # gemm! for Float16 (src/blas/extended_blas.jl:39-88)
+    for (m, n, k) in ((1, 1, 1), (17, 9, 40), (130, 70, 257), (256, 256, 64)):
+        A = (rng.standard_normal((m, k) if ta == "N" else (k, m)) / 4).astype(np.float16)
+        B = (rng.standard_normal((k, n) if tb == "N" else (n, k)) / 4).astype(np.float16)
+        C = rng.standard_normal((m, n)).astype(np.float16)
+        got = gpu.hgemm_(C.copy(), A, B, 0.75, -0.5, ta, tb)
+        ref = O.hgemm(C, A, B, 0.75, -0.5, ta, tb)
+        # both round once to Float16 from Float32 sums that differ in order:
+        # at most one Float16 ulp apart, plus the Float32 reassociation
+        tol = np.abs(ref).astype(np.float64) * 2.0 ** -10 + k * 2.0 ** -23 * (
+            np.abs(op(A, ta)).astype(np.float64) @ np.abs(op(B, tb)).astype(np.float64)) + 2.0 ** -24
+        assert np.all(np.abs(got.astype(np.float64) - ref.astype(np.float64)) <= tol)
+        Ai = rng.integers(-4, 5, A.shape).astype(np.float16)
+        Bi = rng.integers(-4, 5, B.shape).astype(np.float16)
+        Ci = rng.integers(-8, 9, C.shape).astype(np.float16)
+        assert np.array_equal(gpu.hgemm_(Ci.copy(), Ai, Bi, 0.5, -2.0, ta, tb), O.hgemm(Ci, Ai, Bi, 0.5, -2.0, ta, tb))
+    got = gpu.hgemm_(np.full((7, 5), np.nan, np.float16), A[:7] if ta == "N" else A[:, :7],
+                     B[:, :5] if tb == "N" else B[:5], 1.0, 0.0, ta, tb)
+    assert np.all(np.isfinite(got))                       # beta == 0 never reads C

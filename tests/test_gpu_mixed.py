@@ -412,3 +412,53 @@ def test_gemm_pullback(gpu, rng, ta, tb):
     np.testing.assert_allclose(op(Bbar, tb), 1.5 * op(A, ta).T @ Cbar, rtol=1e-12, atol=1e-12)
     Ab, Bb, C0b = gpu.gemm_pullback(A, B, Cbar, 1.5, -0.5, ta, tb)
     np.testing.assert_allclose(C0b, -0.5 * Cbar, rtol=0, atol=0)
+
+
+# ---------------------------------------------------------------------------
+# mixed_solve(A, b; factorization=:cholesky) (src/lapack/extended_lapack.jl:175-226)
+# ---------------------------------------------------------------------------
+def spd_conditioned(rng, n, kappa):
+    U = haar(rng, n)
+    s = kappa ** (-np.arange(n) / max(n - 1, 1))
+    A = (U * s) @ U.T
+    return (A + A.T) / 2
+
+
+@pytest.mark.parametrize("n", [1, 17, 300, 1024])
+def test_mixed_cholesky_vs_oracle(gpu, rng, n):
+    G = rng.standard_normal((n, n))
+    A = G @ G.T + n * np.eye(n); b = rng.standard_normal(n)
+    r = gpu.mixed_solve(A, b, factorization="cholesky")
+    o = O.mixed_solve(A, b, factorization="cholesky", threads=8)
+    assert r.converged and o["converged"] and r.factorization == "cholesky" and not r.fallback
+    assert abs(r.iterations - o["iterations"]) <= 1 and r.backward_error <= 1e-13
+    np.testing.assert_allclose(r.x, o["x"], rtol=1e-11, atol=1e-13 * np.abs(o["x"]).max())
+
+
+@pytest.mark.parametrize("kappa", [1e2, 1e5])
+def test_mixed_cholesky_conditioned(gpu, rng, kappa):
+    n = 2048
+    A = spd_conditioned(rng, n, kappa); xt = rng.standard_normal(n); b = A @ xt
+    r = solve16(gpu, A, b, factorization="cholesky")
+    assert r.converged and r.backward_error <= 16 * EPS
+    _, e = O.refinement_residual(A, r.x, b)
+    assert e <= 16 * EPS
+    assert np.linalg.norm(r.x - xt) / np.linalg.norm(xt) <= 50 * kappa * EPS
+
+
+def test_mixed_cholesky_fallback_and_errors(gpu, rng):
+    n = 64
+    A = np.eye(n) * 3.0
+    A[10:12, 10:12] = [[1.0, 1.0], [1.0, 1.0 + 1e-12]]     # not positive definite once rounded to Float32
+    b = rng.standard_normal(n)
+    r = solve16(gpu, A, b, factorization="cholesky")
+    assert r.fallback and r.reason == "working_factor_fallback" and r.converged
+    I = np.eye(n); I[5, 5] = -1.0                           # indefinite in both precisions
+    with pytest.raises(gpu.PosDefException) as e:
+        gpu.mixed_solve(I, b, factorization="cholesky")
+    assert e.value.info == 6
+    bad = A.copy(); bad[3, 3] = np.nan
+    with pytest.raises(ValueError):
+        gpu.mixed_solve(bad, b, factorization="cholesky")
+    with pytest.raises(ValueError):
+        gpu.mixed_solve(A, b, factorization="cholesky", lowtype="bfloat16")

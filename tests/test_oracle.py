@@ -431,3 +431,199 @@ def test_dlaswp_closed_form(rng):
     assert not np.array_equal(O.dlaswp(A.copy(order="F"), 1, m, ipiv, inverse=True), fwd)
     with pytest.raises(ValueError):
         O.dlaswp(A.copy(order="F"), 1, 3, np.array([1, 2, m + 1]))
+
+
+# ---------------------------------------------------------------------------
+# syrk / Cholesky (src/blas/level3.jl:546-783, src/lapack/factorizations.jl:206-300)
+# ---------------------------------------------------------------------------
+def spd(rng, n, shift=None):
+    G = rng.standard_normal((n, n))
+    return G @ G.T + (n if shift is None else shift) * np.eye(n)
+
+
+@pytest.mark.parametrize("uplo", "UL")
+@pytest.mark.parametrize("trans", "NT")
+def test_syrk_triangle_vs_numpy(rng, uplo, trans):
+    for (n, k) in ((0, 3), (1, 1), (5, 0), (17, 9), (40, 65)):
+        A = rng.standard_normal((n, k) if trans == "N" else (k, n))
+        C0 = np.asfortranarray(rng.standard_normal((n, n)))
+        C = C0.copy(order="F")
+        O.dsyrk(C, A, 0.7, -0.3, uplo, trans)
+        P = A @ A.T if trans == "N" else A.T @ A
+        full = 0.7 * P - 0.3 * C0
+        tri = np.triu if uplo == "U" else np.tril
+        other = np.tril(np.ones((n, n)), -1) if uplo == "U" else np.triu(np.ones((n, n)), 1)
+        np.testing.assert_allclose(tri(C), tri(full), rtol=1e-12, atol=1e-12 * max(1, k))
+        assert np.array_equal(C[other.astype(bool)], C0[other.astype(bool)])   # the other triangle untouched
+    # beta == 0 never reads C; integers are exact
+    A = rng.integers(-5, 6, (30, 12)).astype(float)
+    C = np.asfortranarray(np.full((30, 30), np.nan))
+    O.dsyrk(C, A, 2.0, 0.0, "L")
+    assert np.array_equal(np.tril(C), np.tril(2.0 * A @ A.T))
+    assert np.all(np.isnan(C[np.triu_indices(30, 1)]))
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 64, 65, 130, 257])
+def test_potrf_vs_lapack(rng, n):
+    A = spd(rng, n)
+    L, info = O.dpotrf(A)
+    assert info == 0
+    assert np.array_equal(np.triu(L, 1), np.zeros((n, n)))
+    # LAPACK dpotrf (scipy) and the reconstruction bound
+    Lref = sl.cholesky(A, lower=True)
+    np.testing.assert_allclose(L, Lref, rtol=0, atol=64 * n * EPS * np.abs(Lref).max())
+    assert np.linalg.norm(L @ L.T - A) <= 30 * n * EPS * np.linalg.norm(A)
+    # only the lower triangle is read
+    Au = A.copy(); Au[np.triu_indices(n, 1)] = np.nan
+    L2, info2 = O.dpotrf(Au)
+    assert info2 == 0 and np.array_equal(L2, L)
+
+
+def test_potrf_info_first_failing_column(rng):
+    # a non-positive leading minor at column j: info == j (LAPACK's order k);
+    # scipy names the same order
+    for n, j in ((10, 4), (100, 70), (200, 1), (300, 150)):
+        A = spd(rng, n)
+        A[j - 1, j - 1] = -1e3 * n
+        L, info = O.dpotrf(A)
+        assert info == j
+        with pytest.raises(np.linalg.LinAlgError, match=f"^{j}-th leading minor"):
+            sl.cholesky(A, lower=True)
+    A = spd(rng, 9); A[5, 5] = np.nan
+    assert O.dpotrf(A)[1] == 6
+
+
+def test_potrs_vs_scipy(rng):
+    for n in (1, 9, 70, 200):
+        A = spd(rng, n)
+        L, _ = O.dpotrf(A)
+        b = rng.standard_normal(n); B = rng.standard_normal((n, 5))
+        for rhs in (b, B):
+            x = O.dpotrs(L, rhs)
+            ref = sl.cho_solve((L, True), rhs)
+            np.testing.assert_allclose(x, ref, rtol=1e-10, atol=1e-12)
+            assert np.linalg.norm(A @ x - rhs) <= 30 * n * EPS * np.linalg.norm(A) * np.linalg.norm(x)
+
+
+# ---------------------------------------------------------------------------
+# what is built on LU: inverse, log|det|, backward error, estimates, drivers
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 17, 64, 150])
+def test_getri_vs_numpy_inv(rng, n):
+    A = rng.standard_normal((n, n))
+    F, ipiv, info = O.dgetrf(A)
+    X = O.dgetri(F, ipiv)
+    np.testing.assert_allclose(X, np.linalg.inv(A), rtol=0, atol=1e3 * n * EPS * np.linalg.cond(A) * np.abs(X).max())
+    assert np.linalg.norm(A @ X - np.eye(n)) <= 30 * n * EPS * np.linalg.cond(A)
+    # integer unimodular matrix: the inverse is an exact integer matrix
+    U = np.triu(rng.integers(-2, 3, (6, 6)).astype(float), 1) + np.eye(6)
+    L = np.tril(rng.integers(-2, 3, (6, 6)).astype(float), -1) + np.eye(6)
+    M = (L @ U)[[3, 1, 5, 0, 2, 4]]
+    F, ipiv, _ = O.dgetrf(M)
+    np.testing.assert_allclose(O.dgetri(F, ipiv), np.round(np.linalg.inv(M)), rtol=0, atol=1e-9)
+    S = A.copy(); S[:, 0] = 0
+    F, ipiv, info = O.dgetrf(S)
+    with pytest.raises(np.linalg.LinAlgError):
+        O.dgetri(F, ipiv)
+
+
+def test_logabsdet_vs_slogdet(rng):
+    for n in (1, 5, 40, 200):
+        A = rng.standard_normal((n, n))
+        F, ipiv, _ = O.dgetrf(A)
+        la, s = O.dlogabsdet(F, ipiv)
+        sn, ln = np.linalg.slogdet(A)
+        assert s == sn and la == pytest.approx(ln, rel=1e-12, abs=1e-12)
+    # a permutation matrix: log|det| 0, sign the parity
+    P = np.eye(7)[[1, 0, 3, 2, 4, 6, 5]]
+    F, ipiv, _ = O.dgetrf(P)
+    assert O.dlogabsdet(F, ipiv) == (0.0, -1.0)
+    S = rng.standard_normal((9, 9)); S[:, 4] = 0
+    F, ipiv, _ = O.dgetrf(S)
+    la, s = O.dlogabsdet(F, ipiv)
+    assert la == -np.inf and s == 0.0
+
+
+def test_backward_error_closed_form(rng):
+    A = rng.standard_normal((30, 20)); X = rng.standard_normal((20, 3)); B = rng.standard_normal((30, 3))
+    e, R = O.backward_error(A, X, B)
+    np.testing.assert_allclose(R, B - A @ X, rtol=1e-12, atol=1e-13)
+    assert e == pytest.approx(np.max(np.abs(B - A @ X) / (np.abs(A) @ np.abs(X) + np.abs(B))), rel=1e-12)
+    # exact solution of an integer system: zero; 0/0 = 0; a non-finite denominator: Inf
+    Ai = np.diag(np.arange(1.0, 6.0)); xi = np.arange(1.0, 6.0)
+    assert O.backward_error(Ai, xi, Ai @ xi)[0] == 0.0
+    assert O.backward_error(np.zeros((2, 2)), np.zeros(2), np.zeros(2))[0] == 0.0
+    assert O.backward_error(np.array([[np.inf, 0.0], [0.0, 1.0]]), np.ones(2), np.ones(2))[0] == np.inf
+    assert O.backward_error(np.array([[np.nan]]), np.ones(1), np.ones(1))[0] == np.inf   # NaN denominator
+
+
+def test_inverse_norm_estimate_vs_exact(rng):
+    # Hager's estimate is a lower bound of ||A^-1||_1 that is exact on most
+    # small matrices; it is never above the true norm
+    exact_hits = 0
+    for n in (1, 4, 20, 60, 120):
+        for _ in range(3):
+            A = rng.standard_normal((n, n))
+            F, ipiv, _ = O.dgetrf(A)
+            est = O.inverse_norm_estimate(F, ipiv)
+            true = np.abs(np.linalg.inv(A)).sum(axis=0).max()
+            assert est <= true * (1 + 1e-10) and est >= true / 10
+            exact_hits += est == pytest.approx(true, rel=1e-10)
+    assert exact_hits >= 8
+
+
+def test_refine_and_expert_solve_contract(rng):
+    for n in (1, 10, 80):
+        A = rng.standard_normal((n, n)) * np.logspace(-6, 6, n)[:, None]   # badly scaled rows
+        b = rng.standard_normal(n)
+        F, ipiv, _ = O.dgetrf(A)
+        r = O.refine_solve(A, F, ipiv, b)
+        assert r["converged"] and r["backward_error"] <= 8 * EPS and r["iterations"] <= 8
+        c1 = np.linalg.cond(A, 1)
+        xs = np.linalg.solve(A, b)
+        np.testing.assert_allclose(r["x"], xs, rtol=0, atol=1e-14 * c1 * np.abs(xs).max())
+        assert 0.1 / c1 <= r["rcond_estimate"] <= 1.0 / c1 * (1 + 1e-8)
+        e = O.expert_solve(A, b)
+        assert e["converged"] and e["backward_error"] <= 8 * EPS
+        np.testing.assert_allclose(e["x"], xs, rtol=0, atol=1e-14 * c1 * np.abs(xs).max())
+        np.testing.assert_allclose(e["row_scale"], np.abs(A).max(axis=1), rtol=0, atol=0)
+    Z = rng.standard_normal((5, 5)); Z[2] = 0
+    with pytest.raises(np.linalg.LinAlgError):
+        O.expert_solve(Z, np.ones(5))
+
+
+def test_mixed_solve_cholesky_contract(rng):
+    # mixed_solve(A, b; factorization=:cholesky): test/lapack/factorizations.jl:57-63
+    for n in (1, 17, 80):
+        A = spd(rng, n); b = rng.standard_normal(n)
+        r = O.mixed_solve(A, b, factorization="cholesky")
+        assert r["converged"] and r["iterations"] <= 10 and not r["fallback"]
+        np.testing.assert_allclose(A @ r["x"], b, rtol=2e-12, atol=2e-12 * np.abs(b).max())
+    # positive definite in Float64, not after rounding to Float32: the Float64 factor
+    A = np.array([[1.0, 1.0], [1.0, 1.0 + 1e-12]]); b = np.array([1.0, 2.0])
+    r = O.mixed_solve(A, b, factorization="cholesky")
+    assert r["fallback"] and r["reason"] == "working_factor_fallback"
+    np.testing.assert_allclose(A @ r["x"], b, rtol=1e-9)
+    with pytest.raises(ValueError):
+        O.mixed_solve(np.array([[1.0, 2.0], [2.0, 1.0]]), b, factorization="cholesky")   # indefinite
+
+
+def test_hgemm_restatement(rng):
+    # small integers: every product and partial sum is exact in Float32, the
+    # single rounding to Float16 is the only one, so the result equals the
+    # exact product rounded once -- a dropped beta term or a transposed
+    # operand cannot hide
+    for ta in "NT":
+        for tb in "NT":
+            m, n, k = 17, 9, 40
+            A = rng.integers(-4, 5, (m, k) if ta == "N" else (k, m)).astype(np.float16)
+            B = rng.integers(-4, 5, (k, n) if tb == "N" else (n, k)).astype(np.float16)
+            C = rng.integers(-8, 9, (m, n)).astype(np.float16)
+            got = O.hgemm(C, A, B, 0.5, -2.0, ta, tb)
+            exact = 0.5 * (op(A, ta).astype(np.float64) @ op(B, tb).astype(np.float64)) - 2.0 * C.astype(np.float64)
+            assert np.array_equal(got, exact.astype(np.float16))
+    # alpha, beta rounded to Float16 first; beta == 0 never reads C
+    A = rng.standard_normal((5, 3)).astype(np.float16); B = rng.standard_normal((3, 4)).astype(np.float16)
+    got = O.hgemm(np.full((5, 4), np.nan, np.float16), A, B, 1.0 / 3.0, 0.0)
+    exact = np.float32(np.float16(1.0 / 3.0)) * (A.astype(np.float32) @ B.astype(np.float32))
+    assert np.all(np.isfinite(got)) and np.allclose(got.astype(np.float32), exact, rtol=2e-3, atol=1e-3)
