@@ -9,9 +9,11 @@ Also reported (extra keys): the trailing-update GEMM roofline measured with
 HIP events on the stream it runs on, the standalone GEMM shapes, the 64-RHS
 solve, the mixed-precision LU + refinement against the FP64 LU-solve
 (A = U diag(sigma) V^T with Haar U, V, generated on the GPU and not timed;
-speedup = FP64 LU-solve time / mixed solve time, both reaching backward error
-<= 16 eps), the oracle's (and scipy-OpenBLAS's) LU-solve on the host cores, and
-a parity check of the timed factorization.
+speedup = FP64 LU-solve time / mixed solve time, the mixed solve run to a
+componentwise backward error <= 1e-15 n), the oracle's (n = 8192) and
+scipy-OpenBLAS's (n = 16384) LU-solve on the host cores, rocSOLVER's getrf
+and the Cholesky factorization at the same size, and a parity check of the
+timed factorization.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 16384] [--nb 0]
                   [--no-cpu] [--no-extra] [--no-mixed] [--kappa 1e6]
@@ -70,7 +72,9 @@ def haar_conditioned(torch, n, kappa, seed):
 
 def mixed_extra(ox, torch, n, kappa, t_fp64, seed=0x0B1A5):
     """Mixed LU + refinement on a kappa-conditioned system, per low type and
-    solver: time, speedup over the FP64 LU-solve, iterations, errors."""
+    solver: time, speedup over the FP64 LU-solve, iterations, errors.  The
+    refinement stops at the target's FP64 backward error: componentwise
+    backward error <= 1e-15 * n."""
     At = haar_conditioned(torch, n, kappa, seed)
     g = torch.Generator(device="cuda").manual_seed(seed + 1)
     xt = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
@@ -78,9 +82,11 @@ def mixed_extra(ox, torch, n, kappa, t_fp64, seed=0x0B1A5):
     A = ox.DMatrix(At.T.contiguous().reshape(-1), n, n)   # column-major A
     bd = ox.DMatrix(b.clone(), n, 1)
     out = {}
-    runs = (("float32", "ir", "partial"), ("float32", "gmres", "partial"), ("float32", "gmres", "rbt"))
+    rtol = 1e-15 * n
+    runs = (("float32", "ir", "partial"), ("float32", "gmres", "partial"), ("bfloat16", "gmres", "partial"),
+            ("float16", "gmres", "partial"), ("float8_e4m3fn", "gmres", "partial"))
     for low, solver, pivot in runs:
-        kw = dict(lowtype=low, solver=solver, maxiter=20, gmres_restart=30, pivot=pivot)
+        kw = dict(lowtype=low, solver=solver, maxiter=20, rtol=rtol, gmres_restart=30, pivot=pivot)
         r = ox.mixed_solve(A, bd, **kw)   # warm
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -102,18 +108,23 @@ def mixed_extra(ox, torch, n, kappa, t_fp64, seed=0x0B1A5):
     return out
 
 
-def cpu_baseline(n=4096, reps=3):
+def cpu_baseline(n=8192, reps=3, n_openblas=16384):
     """The oracle's restatement of lu_factor + solve on all host cores, and
     scipy-OpenBLAS beside it (both labelled proxies for the Julia reference,
-    which cannot run in this environment)."""
+    which cannot run in this environment).  The oracle runs a bounded sample
+    (n = 8192, about 13 s a run on 16 cores: the headline n = 16384 takes
+    about 100 s a run), after a warm-up at n = 2048; OpenBLAS runs at the
+    headline size."""
     import numpy as np
     import oracle as O
     import scipy.linalg as sl
     cores = len(os.sched_getaffinity(0))
     rng = np.random.default_rng(0x0B1A5)
-    A = rng.standard_normal((n, n)); b = rng.standard_normal(n)
-    flops = 2.0 / 3.0 * n ** 3 + 2.0 * n ** 2
+    fl = lambda m: 2.0 / 3.0 * m ** 3 + 2.0 * m ** 2   # noqa: E731
     O.lib()
+    W = rng.standard_normal((2048, 2048))
+    F, ipiv, info = O.dgetrf(W, 128, threads=cores)          # warm-up (threads, pages)
+    A = rng.standard_normal((n, n)); b = rng.standard_normal(n)
     times = []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -121,7 +132,10 @@ def cpu_baseline(n=4096, reps=3):
         O.dgetrs(F, ipiv, b, threads=cores)
         times.append(time.perf_counter() - t0)
     t = sorted(times)[len(times) // 2]
+    del F, A
     os.environ.setdefault("OPENBLAS_NUM_THREADS", str(cores))
+    A = rng.standard_normal((n_openblas, n_openblas)); b = rng.standard_normal(n_openblas)
+    sl.lu_factor(W, check_finite=False)                       # warm-up
     ob = []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -129,10 +143,12 @@ def cpu_baseline(n=4096, reps=3):
         sl.lu_solve((lu, piv), b, check_finite=False)
         ob.append(time.perf_counter() - t0)
     to = sorted(ob)[len(ob) // 2]
-    return ({"value": round(flops / t / 1e9, 2), "unit": "GFLOP/s", "cores": cores, "kind": "port",
-             "sample": f"oracle lu_factor(blocksize=128)+solve, n={n}, 1 rhs, median of {reps}, {cores} threads"},
-            {"value": round(flops / to / 1e9, 2), "unit": "GFLOP/s", "threads": cores,
-             "sample": f"scipy-OpenBLAS dgetrf+dgetrs n={n}"})
+    del A, lu
+    return ({"value": round(fl(n) / t / 1e9, 2), "unit": "GFLOP/s", "cores": cores, "kind": "port",
+             "sample": f"oracle lu_factor(blocksize=128)+solve, n={n}, 1 rhs, median of {reps} warm runs, "
+                       f"{cores} threads (n=16384 takes ~100 s a run)"},
+            {"value": round(fl(n_openblas) / to / 1e9, 2), "unit": "GFLOP/s", "threads": cores,
+             "sample": f"scipy-OpenBLAS dgetrf+dgetrs n={n_openblas}, median of {reps} warm runs"})
 
 
 def trailing_shapes(n=16384):
@@ -288,7 +304,44 @@ def main():
         ts = time.perf_counter() - t1
         extra["getrs_64rhs_ms"] = round(ts * 1e3, 3)
         extra["getrs_64rhs_gflops"] = round(2.0 * n * n * 64 / ts / 1e9, 1)
-        del F, B64
+        log(f"getrs 64 rhs n={n}: {ts * 1e3:.3f} ms")
+        # a second solve with the same factors reuses their solve plan
+        b1 = ox.DMatrix(b0.clone(), n, 1)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        F.ldiv_(b1)
+        torch.cuda.synchronize()
+        extra["getrs_1rhs_repeat_ms"] = round((time.perf_counter() - t1) * 1e3, 3)
+        del F, B64, b1
+        # rocSOLVER's getrf through torch on the same box, for comparison (labelled)
+        Ac = A0.buf.view(n, n).clone()
+        torch.linalg.lu_factor(Ac)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(3):
+            torch.linalg.lu_factor(Ac)
+        torch.cuda.synchronize()
+        tr = (time.perf_counter() - t1) / 3
+        extra["rocsolver_getrf_ms"] = round(tr * 1e3, 2)
+        log(f"rocSOLVER getrf (torch.linalg.lu_factor) n={n}: {tr * 1e3:.2f} ms")
+        del Ac
+        # Cholesky of an SPD matrix of the same size (potrf2 recursion)
+        S = ox.DMatrix(torch.randn(n * n, dtype=torch.float64, device=dev, generator=g), n, n)
+        St = S.buf.view(n, n)
+        St.copy_(St @ St.T)
+        St.diagonal().add_(n)
+        C0 = S.copy()
+        ox.cholesky_(C0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        C1 = S.copy()
+        ox.cholesky_(C1)
+        torch.cuda.synchronize()
+        tc = time.perf_counter() - t1
+        extra["cholesky_ms"] = round(tc * 1e3, 2)
+        extra["cholesky_tflops"] = round(n ** 3 / 3.0 / tc / 1e12, 2)
+        log(f"cholesky n={n}: {tc * 1e3:.2f} ms")
+        del S, St, C0, C1
 
     mixed = None
     if not args.no_mixed:
@@ -335,7 +388,8 @@ def main():
         out["roofline"]["traffic_launches"] = f"{t['launches_measured']} dgemm_m16_kernel launches"
     if mixed:
         best = max(mixed.values(), key=lambda v: v["speedup_vs_fp64_lu_solve"] if v["converged"] else 0)
-        out["mixed"] = {"kappa": args.kappa, "n": n, "best_speedup_vs_fp64_lu_solve":
+        out["mixed"] = {"kappa": args.kappa, "n": n, "target_backward_error": 1e-15 * n,
+                        "best_speedup_vs_fp64_lu_solve":
                         best["speedup_vs_fp64_lu_solve"] if best["converged"] else None, "runs": mixed}
     out.update(extra)
     print(json.dumps(out), flush=True)

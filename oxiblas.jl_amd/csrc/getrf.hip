@@ -491,7 +491,10 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     const bool square = (m == n);
     const i64 ldw = square ? lda : n;
     const size_t wbytes = square ? 0 : size_t(n) * size_t(m) * sizeof(T);
-    const size_t pub_doubles = size_t(OXI_LEAF_W) * (size_t(256) + 1) * OXI_PUB_STRIDE + size_t(OXI_LEAF_W) * OXI_PUB_STRIDE;
+    // leaf records: column jn < w of workgroup p < P at jn * P + p, the
+    // diagonal rows at w * P + jn (panel_kernel.hpp), P <= OXI_LEAF_MAX_P
+    const size_t pub_doubles = size_t(OXI_LEAF_W) * (size_t(OXI_LEAF_MAX_P) + 1) * OXI_PUB_STRIDE +
+                               size_t(OXI_LEAF_W) * OXI_PUB_STRIDE;
     const size_t pairs_bytes = (size_t(4 * PERM_MAX_STEPS + 4) * 4 + 255) & ~size_t(255);
     const size_t off_pub = (wbytes + 255) & ~size_t(255);
     const size_t off_pairs = off_pub + ((pub_doubles * 8 + 255) & ~size_t(255));
