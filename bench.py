@@ -54,57 +54,64 @@ def gemm_standalone(ox, torch, m, n, k, reps=5):
     return 2.0 * m * n * k / (ms * 1e-3) / 1e12, ms
 
 
-def haar_conditioned(torch, n, kappa, seed):
-    """A = U diag(sigma) V^T, sigma_i = kappa^(-(i-1)/(n-1)); U, V the Q factors
-    of Gaussian matrices with the sign of diag(R) fixed (test data only)."""
+def haar_pair(torch, n, seed):
+    """U, V: the Q factors of Gaussian matrices with the sign of diag(R) fixed
+    (Haar-distributed orthogonal matrices; test data only)."""
     g = torch.Generator(device="cuda").manual_seed(seed)
 
     def q():
         Q, R = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g))
         return Q * torch.sign(torch.diagonal(R))
 
-    s = kappa ** (-torch.arange(n, dtype=torch.float64, device="cuda") / max(n - 1, 1))
-    U = q()
-    A = (U * s) @ q().T
-    del U
-    return A
+    return q(), q()
 
 
 def mixed_extra(ox, torch, n, kappa, t_fp64, seed=0x0B1A5):
-    """Mixed LU + refinement on a kappa-conditioned system, per low type and
-    solver: time, speedup over the FP64 LU-solve, iterations, errors.  The
-    refinement stops at the target's FP64 backward error: componentwise
-    backward error <= 1e-15 * n."""
-    At = haar_conditioned(torch, n, kappa, seed)
+    """Mixed LU + refinement on A = U diag(sigma) V^T, sigma_i =
+    kappa^(-(i-1)/(n-1)), per low type: time, speedup over the FP64 LU-solve,
+    iterations, errors.  The refinement stops at the target's FP64 backward
+    error: componentwise backward error <= 1e-15 n.  At the headline kappa
+    every low type runs; at kappa = 1e2 the low-precision trailing products,
+    the only condition at which their factors still precondition GMRES-IR at
+    this size.  Runs that cannot converge are bounded (maxiter 4)."""
+    U, V = haar_pair(torch, n, seed)
     g = torch.Generator(device="cuda").manual_seed(seed + 1)
     xt = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
-    b = At @ xt
-    A = ox.DMatrix(At.T.contiguous().reshape(-1), n, n)   # column-major A
-    bd = ox.DMatrix(b.clone(), n, 1)
-    out = {}
     rtol = 1e-15 * n
-    runs = (("float32", "ir", "partial"), ("float32", "gmres", "partial"), ("bfloat16", "gmres", "partial"),
-            ("float16", "gmres", "partial"), ("float8_e4m3fn", "gmres", "partial"))
-    for low, solver, pivot in runs:
-        kw = dict(lowtype=low, solver=solver, maxiter=20, rtol=rtol, gmres_restart=30, pivot=pivot)
-        r = ox.mixed_solve(A, bd, **kw)   # warm
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        r = ox.mixed_solve(A, bd, **kw)
-        torch.cuda.synchronize()
-        t = time.perf_counter() - t0
-        x = r.x.buf[:n]
-        fwd = (torch.linalg.norm(x - xt) / torch.linalg.norm(xt)).item()
-        key = f"mixed_{low}_{solver}" + ("_rbt" if pivot == "rbt" else "")
-        out[key] = {"ms": round(t * 1e3, 2), "speedup_vs_fp64_lu_solve": round(t_fp64 / t, 3),
-                    "converged": r.converged, "backward_error": r.backward_error,
-                    "backward_error_over_n": r.backward_error / n, "forward_error": fwd,
-                    "iterations": r.iterations, "gmres_iterations": r.inner_iterations, "fallback": r.fallback,
-                    "factor_ms": round(r.factor_ms, 2), "refine_ms": round(r.refine_ms, 2)}
-        log(f"mixed {low}/{solver}/{pivot} n={n} kappa={kappa:g}: {t * 1e3:.1f} ms ({t_fp64 / t:.2f}x), "
-            f"berr {r.backward_error:.2e}, fwd {fwd:.2e}, it {r.iterations}/{r.inner_iterations}, "
-            f"factor {r.factor_ms:.1f} ms refine {r.refine_ms:.1f} ms")
-    del A, bd, At, b
+    out = {}
+    plan = ((kappa, (("float32", "ir"), ("float32", "gmres"), ("bfloat16", "gmres"), ("float16", "gmres"),
+                     ("float8_e4m3fn", "gmres"))),
+            (1e2, (("bfloat16", "gmres"), ("float16", "gmres"), ("float8_e4m3fn", "gmres"))))
+    for kap, runs in plan:
+        s = kap ** (-torch.arange(n, dtype=torch.float64, device="cuda") / max(n - 1, 1))
+        At = (U * s) @ V.T
+        b = At @ xt
+        A = ox.DMatrix(At.T.contiguous().reshape(-1), n, n)   # column-major A
+        bd = ox.DMatrix(b.clone(), n, 1)
+        del At
+        for low, solver in runs:
+            bounded = low != "float32" and kap > 1e2
+            kw = dict(lowtype=low, solver=solver, maxiter=4 if bounded else 20, rtol=rtol, gmres_restart=30)
+            r = ox.mixed_solve(A, bd, **kw)   # warm
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = ox.mixed_solve(A, bd, **kw)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            x = r.x.buf[:n]
+            fwd = (torch.linalg.norm(x - xt) / torch.linalg.norm(xt)).item()
+            key = f"mixed_{low}_{solver}_kappa{kap:g}"
+            out[key] = {"kappa": kap, "ms": round(t * 1e3, 2), "speedup_vs_fp64_lu_solve": round(t_fp64 / t, 3),
+                        "converged": r.converged, "backward_error": r.backward_error,
+                        "backward_error_over_n": r.backward_error / n, "forward_error": fwd,
+                        "iterations": r.iterations, "gmres_iterations": r.inner_iterations, "fallback": r.fallback,
+                        "factor_ms": round(r.factor_ms, 2), "refine_ms": round(r.refine_ms, 2),
+                        "maxiter": kw["maxiter"]}
+            log(f"mixed {low}/{solver} n={n} kappa={kap:g}: {t * 1e3:.1f} ms ({t_fp64 / t:.2f}x), "
+                f"converged={r.converged} berr {r.backward_error:.2e}, fwd {fwd:.2e}, "
+                f"it {r.iterations}/{r.inner_iterations}, factor {r.factor_ms:.1f} ms refine {r.refine_ms:.1f} ms")
+        del A, bd, b
+    del U, V
     return out
 
 
@@ -205,6 +212,44 @@ def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, out_js
     print(json.dumps(out))
 
 
+def run_part(ox, torch, part, n, reps=3):
+    """One extra workload, reps times, synchronised, with its time per run on
+    stderr: the command the kernel summaries under profiles/ come from."""
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(0x0B1A5)
+    if part == "cholesky":
+        S = ox.DMatrix(torch.randn(n * n, dtype=torch.float64, device=dev, generator=g), n, n)
+        St = S.buf.view(n, n)
+        St.copy_(St @ St.T)
+        St.diagonal().add_(n)
+        work = lambda: ox.cholesky_(S.copy())                     # noqa: E731
+    elif part in ("getrs64", "trsv"):
+        A = ox.DMatrix(torch.randn(n * n, dtype=torch.float64, device=dev, generator=g), n, n)
+        F = ox.lu(A)
+        k = 64 if part == "getrs64" else 1
+        B = ox.DMatrix(torch.randn(n * k, dtype=torch.float64, device=dev, generator=g), n, k)
+        work = lambda: F.ldiv_(B.copy())                          # noqa: E731
+    elif part in ("mixed32", "lowp"):
+        U, V = haar_pair(torch, n, 0x0B1A5)
+        kap = 1e6 if part == "mixed32" else 1e2
+        s = kap ** (-torch.arange(n, dtype=torch.float64, device=dev) / max(n - 1, 1))
+        At = (U * s) @ V.T
+        del U, V
+        b = At @ torch.randn(n, dtype=torch.float64, device=dev, generator=g)
+        A = ox.DMatrix(At.T.contiguous().reshape(-1), n, n)
+        del At
+        bd = ox.DMatrix(b, n, 1)
+        low = "float32" if part == "mixed32" else "bfloat16"
+        work = lambda: ox.mixed_solve(A, bd, lowtype=low, solver="gmres", rtol=1e-15 * n, maxiter=20)  # noqa: E731
+    work()
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        work()
+        torch.cuda.synchronize()
+        log(f"{part} n={n}: {(time.perf_counter() - t0) * 1e3:.3f} ms")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +264,8 @@ def main():
     ap.add_argument("--traffic-summary", nargs=5, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV",
                                                            "COPY_WRITE_CSV", "OUT_JSON"),
                     help="write the HBM traffic per dgemm_m16_kernel launch from counter runs and exit")
+    ap.add_argument("--part", choices=("cholesky", "getrs64", "trsv", "mixed32", "lowp"), default=None,
+                    help="only run one extra part a few times (for rocprofv3 summaries of its kernels) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 times): the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -244,6 +291,9 @@ def main():
     import oxiblas_mi355x as ox
     ox.load()
     n = args.n
+    if args.part:
+        run_part(ox, torch, args.part, n)
+        return
     dev = torch.device("cuda:0")
     g = torch.Generator(device=dev).manual_seed(0x0B1A5)
     A0 = ox.DMatrix(torch.randn(n * n, dtype=torch.float64, device=dev, generator=g), n, n)
@@ -387,10 +437,11 @@ def main():
         out["roofline"]["traffic_algorithmic"] = t["algorithmic_bytes_per_launch"]
         out["roofline"]["traffic_launches"] = f"{t['launches_measured']} dgemm_m16_kernel launches"
     if mixed:
-        best = max(mixed.values(), key=lambda v: v["speedup_vs_fp64_lu_solve"] if v["converged"] else 0)
+        at = [v for v in mixed.values() if v["kappa"] == args.kappa and v["converged"]]
+        best = max(at, key=lambda v: v["speedup_vs_fp64_lu_solve"]) if at else None
         out["mixed"] = {"kappa": args.kappa, "n": n, "target_backward_error": 1e-15 * n,
-                        "best_speedup_vs_fp64_lu_solve":
-                        best["speedup_vs_fp64_lu_solve"] if best["converged"] else None, "runs": mixed}
+                        "best_speedup_vs_fp64_lu_solve": best["speedup_vs_fp64_lu_solve"] if best else None,
+                        "runs": mixed}
     out.update(extra)
     print(json.dumps(out), flush=True)
 
