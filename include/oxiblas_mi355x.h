@@ -117,6 +117,19 @@ int oxi_dtrsm(oxi_handle_t handle, char side, char uplo, char trans, char diag, 
 int oxi_dlaswp(oxi_handle_t handle, int64_t ncols, double* A, int64_t lda, int64_t k1, int64_t k2,
                const int64_t* ipiv, int inverse);
 
+/* oxi_dlaswp without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results and errors as oxi_dlaswp; the pair list of each block of at most
+ * 512 interchanges and a status word live in a fixed 8.5 KB device array of
+ * the library (one per GPU, held until the process ends) instead of the
+ * handle workspace, so the call allocates nothing (rows and columns below
+ * 2^31).  Calls that overlap on one GPU (different handles or streams) share
+ * that array and race.  Synchronises the handle's stream once, for the
+ * pivot range check. */
+int oxi_dlaswp_noalloc(oxi_handle_t handle, int64_t ncols, double* A, int64_t lda, int64_t k1, int64_t k2,
+                       const int64_t* ipiv, int inverse);
+
 /* ---- LU with partial pivoting ------------------------------------------- */
 /* lu!(A) / _lu_factor!(F; blocksize, check)  (src/lapack/factorizations.jl:30-77,
  * src/api/linalg_api.jl:218): A (m x n, device) is overwritten by the unit

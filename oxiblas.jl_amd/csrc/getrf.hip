@@ -697,6 +697,91 @@ extern "C" int oxi_sgetrf_lowp(oxi_handle_t h, int prec, int64_t m, int64_t n, f
     return getrf_t<float>(h, m, n, A, lda, ipiv, nb, info, prec == OXI_PREC_FP32 ? -1 : prec);
 }
 
+// ---------------------------------------------------------------------------
+// laswp without allocation: the pair list of one block of interchanges and a
+// status word live in `scratch`, a file-scope device array (each GPU holds its
+// own copy until the process ends).  Host code never passes `scratch` itself
+// to a kernel: its device address comes from hipGetSymbolAddress, once per
+// device.  Calls that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+constexpr int LASWP_SCRATCH_INTS = 4 * PERM_MAX_STEPS + 4 + 64;   // pairs of one block + status words
+static __device__ __align__(256) int scratch[LASWP_SCRATCH_INTS];
+constexpr int LASWP_STATUS = 4 * PERM_MAX_STEPS + 4;              // index of the status word
+
+namespace {
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[64] = {};    // per device: the device address of `scratch`
+
+int scratch_ptr(oxi_handle_t h, int** out) {
+    if (h->device < 0 || h->device >= 64) return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "device index %d", h->device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[h->device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return oxi_set_error(h, OXI_ERR_HIP, "laswp scratch address: %s", hipGetErrorString(e));
+        g_scratch_ptr[h->device] = (int*)p;
+    }
+    *out = g_scratch_ptr[h->device];
+    return OXI_OK;
+}
+
+__global__ void clear_status_kernel(int* status) {
+    if (threadIdx.x == 0) *status = 0;
+}
+// status = 1 when a pivot of steps k1..k2 (1-based) lies outside 1..lda
+__global__ void check_pivots_kernel(const i64* __restrict__ ipiv, i64 k1, i64 k2, i64 lda, int* status) {
+    for (i64 j = k1 - 1 + i64(blockIdx.x) * blockDim.x + threadIdx.x; j < k2; j += i64(gridDim.x) * blockDim.x) {
+        const i64 p = ipiv[j];
+        if (p < 1 || p > lda) atomicOr(status, 1);
+    }
+}
+}  // namespace
+
+extern "C" int oxi_dlaswp_noalloc(oxi_handle_t h, int64_t ncols, double* A, int64_t lda, int64_t k1, int64_t k2,
+                                  const int64_t* ipiv, int inverse) {
+    if (!h) return -1;
+    if (ncols < 0) return -2;
+    if (lda < 1) return -4;
+    if (k1 < 1) return -5;
+    if (k2 < k1 - 1) return -6;
+    std::lock_guard<std::recursive_mutex> lk(h->guard);
+    if (ncols == 0 || k2 < k1) return OXI_OK;
+    if (!A) return -3;
+    if (!ipiv) return -7;
+    if (k2 > lda) return -6;
+    if (lda > i64(0x7fffffff) || ncols > i64(0x7fffffff))
+        return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "laswp_noalloc: more than 2^31 - 1 rows or columns");
+    (void)hipSetDevice(h->device);
+    hipStream_t s = h->stream;
+    int* sp = nullptr;
+    OXI_TRY(scratch_ptr(h, &sp));
+    int* status = sp + LASWP_STATUS;
+    // every row a step names lies inside the leading dimension (checked on the
+    // device; one word comes back)
+    hipLaunchKernelGGL(clear_status_kernel, dim3(1), dim3(64), 0, s, status);
+    hipLaunchKernelGGL(check_pivots_kernel, dim3(unsigned(std::min<i64>(cdiv(k2 - k1 + 1, 256), 1024))), dim3(256), 0,
+                       s, ipiv, k1, k2, lda, status);
+    OXI_TRY(launch_check(h, "laswp pivot check"));
+    int bad = 0;
+    OXI_HIP(h, hipMemcpyAsync(&bad, status, sizeof(int), hipMemcpyDeviceToHost, s));
+    OXI_HIP(h, hipStreamSynchronize(s));
+    if (bad) return oxi_set_error(h, -7, "laswp: a pivot of steps %lld..%lld lies outside 1..lda", (long long)k1,
+                                  (long long)k2);
+    const i64 total = k2 - k1 + 1;
+    const i64 nblocks = cdiv(total, PERM_MAX_STEPS);
+    for (i64 bi = 0; bi < nblocks; ++bi) {
+        const i64 b = inverse ? nblocks - 1 - bi : bi;
+        const i64 s1 = (k1 - 1) + b * PERM_MAX_STEPS;
+        const int cnt = int(std::min<i64>(PERM_MAX_STEPS, (k2 - 1) - s1 + 1));
+        hipLaunchKernelGGL(perm_pairs_kernel, dim3(1), dim3(256), 0, s, ipiv, i64(0), s1, cnt, inverse, sp);
+        const size_t shm = size_t(2 * cnt) * sizeof(double);
+        hipLaunchKernelGGL(apply_pairs_cm_kernel, dim3(unsigned(ncols)), dim3(256), shm, s, A, lda, ncols, sp, i64(0));
+        OXI_TRY(launch_check(h, "laswp"));
+    }
+    return OXI_OK;
+}
+
 extern "C" int oxi_dlaswp(oxi_handle_t h, int64_t ncols, double* A, int64_t lda, int64_t k1, int64_t k2,
                           const int64_t* ipiv, int inverse) {
     if (!h) return -1;

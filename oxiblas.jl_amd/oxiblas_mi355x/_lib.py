@@ -92,6 +92,7 @@ SIGNATURES = {
     "oxi_dpotrs": (C.c_int, [_vp, _ch, _i64, _i64, _vp, _i64, _vp, _i64]),
     "oxi_dtrsm": (C.c_int, [_vp, _ch, _ch, _ch, _ch, _i64, _i64, _dbl, _vp, _i64, _vp, _i64, C.POINTER(_i64)]),
     "oxi_dlaswp": (C.c_int, [_vp, _i64, _vp, _i64, _i64, _i64, _vp, C.c_int]),
+    "oxi_dlaswp_noalloc": (C.c_int, [_vp, _i64, _vp, _i64, _i64, _i64, _vp, C.c_int]),
     "oxi_dgetrf": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, C.POINTER(_i64)]),
     "oxi_dgetrs": (C.c_int, [_vp, _ch, _i64, _i64, _vp, _i64, _vp, _vp, _i64]),
     "oxi_dgetrs_plan_create": (C.c_int, [_vp, _i64, _vp, _i64, _vp, C.POINTER(_vp)]),
