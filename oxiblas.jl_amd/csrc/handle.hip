@@ -17,44 +17,30 @@ int oxi_set_error(oxi_handle_t h, int code, const char* fmt, ...) {
     return code;
 }
 
-int oxi_workspace(oxi_handle_t h, size_t bytes, void** out) {
-    if (bytes <= h->ws_bytes) { *out = h->ws; return OXI_OK; }
-    // round up to 64 MiB granules so growth is rare
+// Grow-only device buffer: the old block is freed after the handle's streams
+// are idle, the new one rounded up to 64 MiB granules so growth is rare.
+static int grow(oxi_handle_t h, void*& buf, size_t& have, size_t bytes, void** out) {
+    if (bytes <= have) { *out = buf; return OXI_OK; }
     size_t want = ((bytes + (size_t(64) << 20) - 1) >> 26) << 26;
-    if (h->ws) {
+    if (buf) {
         OXI_HIP(h, hipStreamSynchronize(h->stream));
         OXI_HIP(h, hipStreamSynchronize(h->side));
         if (h->bulk) OXI_HIP(h, hipStreamSynchronize(h->bulk));
-        OXI_HIP(h, hipFree(h->ws));
-        h->ws = nullptr; h->ws_bytes = 0;
+        OXI_HIP(h, hipFree(buf));
+        buf = nullptr; have = 0;
     }
-    if (hipMalloc(&h->ws, want) != hipSuccess) {
-        h->ws = nullptr;
+    if (hipMalloc(&buf, want) != hipSuccess) {
+        buf = nullptr;
         return oxi_set_error(h, OXI_ERR_ALLOC, "workspace allocation of %zu bytes failed", want);
     }
-    h->ws_bytes = want;
-    *out = h->ws;
+    have = want;
+    *out = buf;
     return OXI_OK;
 }
 
-int oxi_workspace2(oxi_handle_t h, size_t bytes, void** out) {
-    if (bytes <= h->ws2_bytes) { *out = h->ws2; return OXI_OK; }
-    size_t want = ((bytes + (size_t(64) << 20) - 1) >> 26) << 26;
-    if (h->ws2) {
-        OXI_HIP(h, hipStreamSynchronize(h->stream));
-        OXI_HIP(h, hipStreamSynchronize(h->side));
-        if (h->bulk) OXI_HIP(h, hipStreamSynchronize(h->bulk));
-        OXI_HIP(h, hipFree(h->ws2));
-        h->ws2 = nullptr; h->ws2_bytes = 0;
-    }
-    if (hipMalloc(&h->ws2, want) != hipSuccess) {
-        h->ws2 = nullptr;
-        return oxi_set_error(h, OXI_ERR_ALLOC, "workspace allocation of %zu bytes failed", want);
-    }
-    h->ws2_bytes = want;
-    *out = h->ws2;
-    return OXI_OK;
-}
+int oxi_workspace(oxi_handle_t h, size_t bytes, void** out) { return grow(h, h->ws, h->ws_bytes, bytes, out); }
+int oxi_workspace2(oxi_handle_t h, size_t bytes, void** out) { return grow(h, h->ws2, h->ws2_bytes, bytes, out); }
+int oxi_workspace3(oxi_handle_t h, size_t bytes, void** out) { return grow(h, h->ws3, h->ws3_bytes, bytes, out); }
 
 extern "C" {
 
@@ -114,6 +100,7 @@ int oxi_destroy(oxi_handle_t h) {
     (void)hipDeviceSynchronize();
     if (h->ws) (void)hipFree(h->ws);
     if (h->ws2) (void)hipFree(h->ws2);
+    if (h->ws3) (void)hipFree(h->ws3);
     if (h->dev_flags) (void)hipFree(h->dev_flags);
     if (h->dev_info) (void)hipFree(h->dev_info);
     if (h->trsm_inv) (void)hipFree(h->trsm_inv);

@@ -35,6 +35,8 @@ struct oxi_handle_s {
     size_t ws_bytes = 0;
     void* ws2 = nullptr;                // second workspace for drivers that call the others
     size_t ws2_bytes = 0;
+    void* ws3 = nullptr;                // workspace of the blocked left triangular solve
+    size_t ws3_bytes = 0;
     int* dev_flags = nullptr;           // [0]: barrier timeout flag, [1..]: reserved
     i64* dev_info = nullptr;            // device-side info (first zero pivot)
     std::string last_error;
@@ -72,6 +74,9 @@ int oxi_workspace(oxi_handle_t h, size_t bytes, void** out);
 // Second grow-only workspace, for drivers (mixed solve) whose buffers must
 // survive the calls they make into the routines that use the first one.
 int oxi_workspace2(oxi_handle_t h, size_t bytes, void** out);
+// Third one, used only by the blocked left triangular solve (trsm.hip), which
+// runs inside routines that hold the other two.
+int oxi_workspace3(oxi_handle_t h, size_t bytes, void** out);
 
 // ---------------------------------------------------------------------------
 // Internal launchers (stream-explicit, no argument validation).

@@ -128,7 +128,9 @@ void launch_right_leaf(hipStream_t s, int unit, int trans, i64 rows, int n, cons
 
 // Inverses of the 64 x 64 diagonal blocks of an upper op(T): block b's
 // V = U_bb^-1 goes to inv + 4096 b, column-major (ld 64, zeros below the
-// diagonal).  Thread j computes column j right-looking, with the column's
+// diagonal); with strip > 0, to its place in a column-major strip of ld
+// `ldv` whose rows z*strip .. (z+1)*strip - 1 hold the strip x strip block z
+// (inv + 64 b + ((64 b) % strip) * ldv).  Thread j computes column j right-looking, with the column's
 // 64 partial sums in registers: for k = j down to 0, V[k][j] = acc_k /
 // U[k][k], then acc_i -= U[i][k] V[k][j] for every i < k.  The k steps form
 // the only dependent chain (64 divisions); the i updates of a step are
@@ -137,7 +139,7 @@ void launch_right_leaf(hipStream_t s, int unit, int trans, i64 rows, int n, cons
 // U[i][i] does not have), so only the order of each sum differs from it.
 template <bool UNIT, bool TRANS>
 __global__ __launch_bounds__(64) void trsm_inv64_kernel(const double* __restrict__ T, i64 ldt,
-                                                        double* __restrict__ inv) {
+                                                        double* __restrict__ inv, i64 ldv = 64, int strip = 0) {
     __shared__ double Ut[64][65];   // Ut[c][r] = op(T)[r][c]: columns contiguous
     __shared__ double V[64][65];
     const int tj = threadIdx.x;
@@ -169,9 +171,11 @@ __global__ __launch_bounds__(64) void trsm_inv64_kernel(const double* __restrict
     }
     __syncthreads();
     // column-major store, the thread index along rows
-    double* out = inv + 4096 * i64(blockIdx.x);
+    const i64 d = 64 * i64(blockIdx.x);
+    double* out = strip > 0 ? inv + d + (d % strip) * ldv : inv + 64 * d;
+    const i64 ld = strip > 0 ? ldv : 64;
 #pragma unroll 8
-    for (int c = 0; c < 64; ++c) out[tj + 64 * c] = V[tj][c];
+    for (int c = 0; c < 64; ++c) out[tj + ld * c] = V[tj][c];
 }
 
 // Left side: X (n x nrhs) <- op(T)^-1 X, one wave per right-hand side, lane =
@@ -316,6 +320,87 @@ static const double* left_inverses(oxi_handle_t h, hipStream_t s, int upper, int
     return hipGetLastError() == hipSuccess ? h->trsm_inv : nullptr;
 }
 
+// Left solves with a moderate number of right-hand sides on large triangles
+// (FP64, m a multiple of TRB and at least 2 TRB, 16 .. TRB_MAX_RHS
+// right-hand sides).  The recursion's updates are skinny (m/2 x nrhs x m/2 at
+// the top, and every level costs about as much, one product after the
+// other, each on a few workgroups); here op(A) is swept in blocks of TRB
+// rows: X_b = op(A)_bb^-1 B_b, one product with the block's explicit
+// inverse, then B_rest -= op(A)[rest, b] X_b, whose rows spread over the
+// chip.  The TRB x TRB inverses come from the 64 x 64 ones by doubling: for
+// U = [U11 U12; 0 U22], U^-1 = [V11, -V11 U12 V22; 0, V22], two batched
+// products per doubling and position.  As in the 64 x 64 leaves, each
+// block's solve is a product with its inverse rather than a substitution.
+constexpr int TRB = 256, TRB_MAX_RHS = 512;
+
+static bool left_blocked_applies(i64 m, i64 nrhs) {
+    return m >= 2 * TRB && m % TRB == 0 && nrhs >= 16 && nrhs <= TRB_MAX_RHS;
+}
+
+static int trsm_left_blocked(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 nrhs,
+                             const double* A, i64 lda, double* B, i64 ldb) {
+    const bool lower_op = (!upper) != (trans != 0);   // op(A) lower?
+    // the upper matrix U' whose inverse the kernels build: U' = op(A) for an
+    // upper op(A), op(A)^T for a lower one; U'(i, j) = ktrans ? A(j, i) : A(i, j)
+    const bool ktrans = lower_op ? !trans : (trans != 0);
+    // S: m x TRB strip (ld m), rows z*TRB.. hold block z's inverse; T: the
+    // doubling's temporaries; X: the solution until the final copy
+    const size_t sS = size_t(m) * TRB, sT = size_t(m) * TRB / 2, sX = size_t(m) * size_t(nrhs);
+    void* ws;
+    OXI_TRY(oxi_workspace3(h, (sS + sT + sX) * sizeof(double), &ws));
+    double* S = (double*)ws;
+    double* T = S + sS;
+    double* X = T + sT;
+    if (oxi_launch_dscale(s, m, TRB, 0.0, S, m) != OXI_OK) return oxi_set_error(h, OXI_ERR_HIP, "trsm strip clear");
+    const dim3 grid(unsigned(m / TRL)), block(64);
+    if (unit) {
+        if (ktrans) hipLaunchKernelGGL((trsm_inv64_kernel<true, true>), grid, block, 0, s, A, lda, S, m, TRB);
+        else hipLaunchKernelGGL((trsm_inv64_kernel<true, false>), grid, block, 0, s, A, lda, S, m, TRB);
+    } else {
+        if (ktrans) hipLaunchKernelGGL((trsm_inv64_kernel<false, true>), grid, block, 0, s, A, lda, S, m, TRB);
+        else hipLaunchKernelGGL((trsm_inv64_kernel<false, false>), grid, block, 0, s, A, lda, S, m, TRB);
+    }
+    OXI_TRY(launch_check(h, "trsm block inverses"));
+    const i64 nz = m / TRB;
+    for (i64 w = TRL; w < TRB; w *= 2) {
+        for (i64 l = 0; l < TRB; l += 2 * w) {
+            // in every block z: p = local rows/columns [l, l+w), q = [l+w, l+2w)
+            const double* Upq = ktrans ? A + (l + w) + l * lda : A + l + (l + w) * lda;
+            const double* Vp = S + l + l * m;
+            const double* Vq = S + (l + w) + (l + w) * m;
+            double* Xpq = S + l + (l + w) * m;
+            OXI_TRY(oxi_launch_dgemm_strided(h, s, ktrans ? 1 : 0, 0, w, w, w, 1.0, Upq, lda, TRB * (lda + 1), Vq, m,
+                                             TRB, 0.0, T, w, w * w, nz));
+            OXI_TRY(oxi_launch_dgemm_strided(h, s, 0, 0, w, w, w, -1.0, Vp, m, TRB, T, w, w * w, 0.0, Xpq, m, TRB,
+                                             nz));
+        }
+    }
+    if (lower_op) {
+        // op(A)_bb^-1 = (U'_bb^-1)^T
+        for (i64 b0 = 0; b0 < m; b0 += TRB) {
+            OXI_TRY(oxi_launch_dgemm(h, s, 1, 0, TRB, nrhs, TRB, 1.0, S + b0, m, B + b0, ldb, 0.0, X + b0, m));
+            const i64 r0 = b0 + TRB;
+            if (r0 < m) {
+                const double* Ar = trans ? A + b0 + r0 * lda : A + r0 + b0 * lda;   // op(A)[r0][b0]
+                OXI_TRY(oxi_launch_dgemm(h, s, trans ? 1 : 0, 0, m - r0, nrhs, TRB, -1.0, Ar, lda, X + b0, m, 1.0,
+                                         B + r0, ldb));
+            }
+        }
+    } else {
+        for (i64 b0 = m - TRB; b0 >= 0; b0 -= TRB) {
+            OXI_TRY(oxi_launch_dgemm(h, s, 0, 0, TRB, nrhs, TRB, 1.0, S + b0, m, B + b0, ldb, 0.0, X + b0, m));
+            if (b0 > 0) {
+                const double* Ar = trans ? A + b0 : A + b0 * lda;                   // op(A)[0][b0]
+                OXI_TRY(oxi_launch_dgemm(h, s, trans ? 1 : 0, 0, b0, nrhs, TRB, -1.0, Ar, lda, X + b0, m, 1.0, B,
+                                         ldb));
+            }
+        }
+    }
+    OXI_HIP(h, hipMemcpy2DAsync(B, size_t(ldb) * 8, X, size_t(m) * 8, size_t(m) * 8, size_t(nrhs),
+                                hipMemcpyDeviceToDevice, s));
+    return OXI_OK;
+}
+
 template <typename TY>
 static int trsm_left_t(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 nrhs, const TY* A,
                        i64 lda, TY* B, i64 ldb, const double* inv = nullptr) {
@@ -392,6 +477,9 @@ static int trsm_right_t(oxi_handle_t h, hipStream_t s, int upper, int trans, int
 
 int oxi_dtrsm_left(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64 m, i64 nrhs, const double* A,
                    i64 lda, double* B, i64 ldb) {
+    static const bool blocked = !getenv("OXIBLAS_TRSM_RECURSIVE");   // tuning: the recursion throughout
+    if (blocked && left_blocked_applies(m, nrhs))
+        return trsm_left_blocked(h, s, upper, trans, unit, m, nrhs, A, lda, B, ldb);
     const double* inv = left_inverses(h, s, upper, trans, unit, m, nrhs, A, lda);
     return trsm_left_t<double>(h, s, upper, trans, unit, m, nrhs, A, lda, B, ldb, inv);
 }
