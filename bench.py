@@ -79,9 +79,13 @@ def mixed_extra(ox, torch, n, kappa, t_fp64, seed=0x0B1A5):
     xt = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
     rtol = 1e-15 * n
     out = {}
-    plan = ((kappa, (("float32", "ir"), ("float32", "gmres"), ("bfloat16", "gmres"), ("float16", "gmres"),
-                     ("float8_e4m3fn", "gmres"))),
-            (1e2, (("bfloat16", "gmres"), ("float16", "gmres"), ("float8_e4m3fn", "gmres"))))
+    # (low type, solver, pivot); "tournament": the FP32 factor's leaves pick
+    # their pivots by a tournament (one exchange per 32 columns)
+    plan = ((kappa, (("float32", "ir", "partial"), ("float32", "gmres", "partial"), ("float32", "ir", "tournament"),
+                     ("float32", "gmres", "tournament"), ("bfloat16", "gmres", "partial"),
+                     ("float16", "gmres", "partial"), ("float8_e4m3fn", "gmres", "partial"))),
+            (1e2, (("bfloat16", "gmres", "partial"), ("float16", "gmres", "partial"),
+                   ("float8_e4m3fn", "gmres", "partial"))))
     for kap, runs in plan:
         s = kap ** (-torch.arange(n, dtype=torch.float64, device="cuda") / max(n - 1, 1))
         At = (U * s) @ V.T
@@ -89,9 +93,11 @@ def mixed_extra(ox, torch, n, kappa, t_fp64, seed=0x0B1A5):
         A = ox.DMatrix(At.T.contiguous().reshape(-1), n, n)   # column-major A
         bd = ox.DMatrix(b.clone(), n, 1)
         del At
-        for low, solver in runs:
-            bounded = low != "float32" and kap > 1e2
-            kw = dict(lowtype=low, solver=solver, maxiter=4 if bounded else 20, rtol=rtol, gmres_restart=30)
+        for low, solver, pivot in runs:
+            # runs that measurements show cannot converge here are bounded
+            expected = low == "float32" or (low == "float16" and kap <= 1e2)
+            kw = dict(lowtype=low, solver=solver, maxiter=20 if expected else 4, rtol=rtol, gmres_restart=30,
+                      pivot=pivot)
             r = ox.mixed_solve(A, bd, **kw)   # warm
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -100,14 +106,14 @@ def mixed_extra(ox, torch, n, kappa, t_fp64, seed=0x0B1A5):
             t = time.perf_counter() - t0
             x = r.x.buf[:n]
             fwd = (torch.linalg.norm(x - xt) / torch.linalg.norm(xt)).item()
-            key = f"mixed_{low}_{solver}_kappa{kap:g}"
+            key = f"mixed_{low}_{solver}{'_tournament' if pivot != 'partial' else ''}_kappa{kap:g}"
             out[key] = {"kappa": kap, "ms": round(t * 1e3, 2), "speedup_vs_fp64_lu_solve": round(t_fp64 / t, 3),
                         "converged": r.converged, "backward_error": r.backward_error,
                         "backward_error_over_n": r.backward_error / n, "forward_error": fwd,
                         "iterations": r.iterations, "gmres_iterations": r.inner_iterations, "fallback": r.fallback,
                         "factor_ms": round(r.factor_ms, 2), "refine_ms": round(r.refine_ms, 2),
-                        "maxiter": kw["maxiter"]}
-            log(f"mixed {low}/{solver} n={n} kappa={kap:g}: {t * 1e3:.1f} ms ({t_fp64 / t:.2f}x), "
+                        "maxiter": kw["maxiter"], "pivot": pivot}
+            log(f"mixed {low}/{solver}/{pivot} n={n} kappa={kap:g}: {t * 1e3:.1f} ms ({t_fp64 / t:.2f}x), "
                 f"converged={r.converged} berr {r.backward_error:.2e}, fwd {fwd:.2e}, "
                 f"it {r.iterations}/{r.inner_iterations}, factor {r.factor_ms:.1f} ms refine {r.refine_ms:.1f} ms")
         del A, bd, b
@@ -229,9 +235,9 @@ def run_part(ox, torch, part, n, reps=3):
         k = 64 if part == "getrs64" else 1
         B = ox.DMatrix(torch.randn(n * k, dtype=torch.float64, device=dev, generator=g), n, k)
         work = lambda: F.ldiv_(B.copy())                          # noqa: E731
-    elif part in ("mixed32", "lowp"):
+    elif part in ("mixed32", "mixed32tp", "lowp"):
         U, V = haar_pair(torch, n, 0x0B1A5)
-        kap = 1e6 if part == "mixed32" else 1e2
+        kap = 1e2 if part == "lowp" else 1e6
         s = kap ** (-torch.arange(n, dtype=torch.float64, device=dev) / max(n - 1, 1))
         At = (U * s) @ V.T
         del U, V
@@ -239,8 +245,10 @@ def run_part(ox, torch, part, n, reps=3):
         A = ox.DMatrix(At.T.contiguous().reshape(-1), n, n)
         del At
         bd = ox.DMatrix(b, n, 1)
-        low = "float32" if part == "mixed32" else "bfloat16"
-        work = lambda: ox.mixed_solve(A, bd, lowtype=low, solver="gmres", rtol=1e-15 * n, maxiter=20)  # noqa: E731
+        low = "bfloat16" if part == "lowp" else "float32"
+        pivot = "tournament" if part == "mixed32tp" else "partial"
+        work = lambda: ox.mixed_solve(A, bd, lowtype=low, solver="gmres", rtol=1e-15 * n, maxiter=20,  # noqa: E731
+                                      pivot=pivot)
     work()
     torch.cuda.synchronize()
     for _ in range(reps):
@@ -264,7 +272,7 @@ def main():
     ap.add_argument("--traffic-summary", nargs=5, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV",
                                                            "COPY_WRITE_CSV", "OUT_JSON"),
                     help="write the HBM traffic per dgemm_m16_kernel launch from counter runs and exit")
-    ap.add_argument("--part", choices=("cholesky", "getrs64", "trsv", "mixed32", "lowp"), default=None,
+    ap.add_argument("--part", choices=("cholesky", "getrs64", "trsv", "mixed32", "mixed32tp", "lowp"), default=None,
                     help="only run one extra part a few times (for rocprofv3 summaries of its kernels) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 times): the known-size kernel that scales "

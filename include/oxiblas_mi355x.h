@@ -99,7 +99,10 @@ int oxi_dgemm_batched(oxi_handle_t handle, char transA, char transB, int64_t m, 
  * the triangle `uplo` ('U'/'L') only, the other triangle untouched; trans 'N':
  * A is n x k, 'T'/'C': A is k x n.  alpha == 0 or k == 0 scales the triangle;
  * beta == 0 never reads C (_triangle_scale!, :546-553).  FP64 MFMA; tiles
- * outside the triangle do no work. */
+ * outside the triangle do no work.  The other triangle keeps its values;
+ * with beta != 0 its elements inside the 128 x 128 tiles on the diagonal
+ * (n >= about 2700) are read and stored back unchanged, so it must not be
+ * written concurrently by other work. */
 int oxi_dsyrk(oxi_handle_t handle, char uplo, char trans, int64_t n, int64_t k, double alpha, const double* A,
               int64_t lda, double beta, double* C, int64_t ldc);
 
@@ -291,6 +294,17 @@ int oxi_hgemm(oxi_handle_t handle, char transA, char transB, int64_t m, int64_t 
 int oxi_sgetrf_lowp(oxi_handle_t handle, int prec, int64_t m, int64_t n, float* A, int64_t lda, int64_t* ipiv,
                     int64_t nb, int64_t* info);
 
+/* oxi_sgetrf_lowp with tournament pivoting (no reference counterpart; the
+ * factor of oxi_dgesv_mixed with pivot 2): each 32-column leaf's pivots are
+ * the partial-pivoting pivots of the candidate rows that partial pivoting
+ * picks inside each block of 256 rows (512 beyond 8192 rows), so one exchange
+ * per leaf replaces one per column; every row is then eliminated with those
+ * pivot rows by the operations of partial pivoting.  Ties go to the row of
+ * lowest position, NaN never wins.  Panels of more than 16384 rows pivot
+ * partially.  Same arguments, info and errors as oxi_sgetrf_lowp. */
+int oxi_sgetrf_tournament(oxi_handle_t handle, int prec, int64_t m, int64_t n, float* A, int64_t lda, int64_t* ipiv,
+                          int64_t nb, int64_t* info);
+
 /* FP32 solve with the factors of oxi_sgetrf_lowp (as oxi_dgetrs). */
 int oxi_sgetrs(oxi_handle_t handle, char trans, int64_t n, int64_t nrhs, const float* LU, int64_t lda,
                const int64_t* ipiv, float* B, int64_t ldb);
@@ -330,7 +344,11 @@ typedef struct {
  * V after the triangular solves.  It applies to prec = OXI_PREC_FP32 (factors
  * rounded further lose too much to multipliers that pivoting no longer bounds);
  * n must be a multiple of 4 and the factor free of zero pivots.  Otherwise the
- * partially pivoted factor is used. */
+ * partially pivoted factor is used.  pivot 2 (no reference counterpart): the
+ * low-precision LU chooses each 32-column leaf's pivots by a tournament over
+ * its row blocks (communication-avoiding LU, Grigori, Demmel and Xiang 2011):
+ * one exchange between the workgroups of a leaf instead of one per column;
+ * panels of more than 16384 rows pivot partially. */
 int oxi_dgesv_mixed(oxi_handle_t handle, int prec, int solver, int64_t n, int64_t nrhs, const double* A,
                     int64_t lda, const double* B, int64_t ldb, double* X, int64_t ldx, int32_t maxiter, double rtol,
                     int32_t gmres_restart, int32_t pivot, oxi_ir_info* info);

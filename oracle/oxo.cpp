@@ -491,13 +491,116 @@ i64 lu_pivot(const T* P, i64 ldp, i64 mp, i64 j) {
     return p;
 }
 
-struct LuState { i64* pivots; i64 info; bool check; bool stop; int threads; };
+struct LuState { i64* pivots; i64 info; bool check; bool stop; int threads; bool tournament = false; };
+
+// Tournament pivoting for a leaf (oxi_sgetrf_tournament, the FP32 factor of
+// the mixed solve with pivot 2; no reference counterpart).  tp_gepp: partial
+// pivoting over a set of rows (row-major, 32 values each, positions id) for
+// columns 0..w-1, with ties to the lowest position and NaN never winning;
+// returns the pivot positions in order and, with prow, each pivot row as
+// partial pivoting leaves it (multipliers left of its column, U from it on).
+// A zero pivot leaves its column and updates nothing.
+template <class T>
+static T tp_key(T v) {
+    T a = std::fabs(v);
+    return a != a ? T(-1) : a;
+}
+template <class T>
+static void tp_gepp(std::vector<T>& V, const std::vector<i64>& id, int w, std::vector<i64>& piv,
+                    std::vector<T>* prow) {
+    const size_t nr = id.size();
+    std::vector<char> live(nr, 1);
+    piv.clear();
+    for (int c = 0; c < w; ++c) {
+        i64 best = -1, bid = std::numeric_limits<i64>::max();
+        T bk = T(-2);
+        for (size_t r = 0; r < nr; ++r)
+            if (live[r]) {
+                const T k = tp_key(V[r * 32 + c]);
+                if (k > bk || (k == bk && id[r] < bid)) { bk = k; bid = id[r]; best = i64(r); }
+            }
+        if (best < 0) break;
+        piv.push_back(id[best]);
+        live[best] = 0;
+        const T* u = &V[best * 32];
+        if (prow) prow->insert(prow->end(), u, u + 32);
+        const T d = u[c];
+        if (d == T(0)) continue;
+        for (size_t r = 0; r < nr; ++r)
+            if (live[r]) {
+                T* v = &V[r * 32];
+                const T l = v[c] / d;
+                v[c] = l;
+                for (int k = c + 1; k < 32; ++k) v[k] = std::fma(-l, u[k], v[k]);
+            }
+    }
+}
+
+// The tournament leaf on the mp x np block P (np <= 32): candidates by
+// partial pivoting inside each block of R rows (R = 256, 512 beyond 8192
+// rows), the leaf's pivots by partial pivoting over the candidates' original
+// rows, then every other row eliminated with the pivot rows; rows are stored
+// in the order the interchanges give, pivots as for the partial-pivoting leaf.
+template <class T>
+void tp_leaf(T* P, i64 ldp, i64 mp, i64 np, i64 offset, LuState& st) {
+    const int w = int(np);
+    const i64 R = mp > 8192 ? 512 : 256;
+    auto load = [&](i64 r, T* dst) {
+        for (int c = 0; c < 32; ++c) dst[c] = c < w ? P[r + c * ldp] : T(0);
+    };
+    std::vector<i64> cand, pv;
+    for (i64 g0 = 0; g0 < mp; g0 += R) {
+        const i64 g1 = std::min(mp, g0 + R);
+        std::vector<T> V(size_t(g1 - g0) * 32);
+        std::vector<i64> id;
+        for (i64 r = g0; r < g1; ++r) { id.push_back(r); load(r, &V[size_t(r - g0) * 32]); }
+        tp_gepp(V, id, w, pv, (std::vector<T>*)nullptr);
+        cand.insert(cand.end(), pv.begin(), pv.end());
+    }
+    std::vector<T> V(cand.size() * 32), prow;
+    for (size_t k = 0; k < cand.size(); ++k) load(cand[k], &V[k * 32]);
+    std::vector<i64> pid;
+    tp_gepp(V, cand, w, pid, &prow);
+    for (int c = 0; c < w; ++c)
+        if (prow[size_t(c) * 32 + c] == T(0)) {
+            if (st.info == 0) st.info = offset + c + 1;
+            if (st.check) { st.stop = true; return; }
+            break;
+        }
+    // the interchanges: position of every row as the steps move it
+    std::vector<i64> perm(mp), where(mp);
+    for (i64 i = 0; i < mp; ++i) perm[i] = where[i] = i;
+    for (int c = 0; c < w; ++c) {
+        const i64 q = where[pid[c]];
+        st.pivots[offset + c] = offset + q + 1;
+        const i64 a = perm[c], b = perm[q];
+        std::swap(perm[c], perm[q]);
+        where[a] = q; where[b] = c;
+    }
+    std::vector<T> out(size_t(mp) * 32);
+    for (i64 i = 0; i < mp; ++i) {
+        T* o = &out[size_t(i) * 32];
+        if (i < w) { std::copy(&prow[size_t(i) * 32], &prow[size_t(i) * 32] + 32, o); continue; }
+        load(perm[i], o);
+        for (int c = 0; c < w; ++c) {
+            const T* u = &prow[size_t(c) * 32];
+            const T d = u[c];
+            if (d == T(0)) continue;
+            const T l = o[c] / d;
+            o[c] = l;
+            for (int k = c + 1; k < 32; ++k) o[k] = std::fma(-l, u[k], o[k]);
+        }
+    }
+    for (i64 i = 0; i < mp; ++i)
+        for (int c = 0; c < w; ++c) P[i + c * ldp] = out[size_t(i) * 32 + c];
+}
 
 // _lu_panel! (src/lapack/factorizations.jl:108-152): recursive getrf2 with a
 // column-by-column leaf of up to 32 columns (division by the pivot).
 template <class T>
 void lu_panel(T* P, i64 ldp, i64 mp, i64 np, i64 offset, LuState& st) {
     if (st.stop) return;
+    if (np <= 32 && st.tournament && mp >= np) { tp_leaf(P, ldp, mp, np, offset, st); return; }
     if (np <= 32) {
         for (i64 j = 0; j < std::min(mp, np); ++j) {
             i64 p = lu_pivot(P, ldp, mp, j);
@@ -536,13 +639,14 @@ void lu_panel(T* P, i64 ldp, i64 mp, i64 np, i64 offset, LuState& st) {
 // trailing GEMM update.  Look-ahead only reorders independent work, so the
 // sequential order below gives the same factors (lapack_dense.jl:33-38).
 template <class T>
-int getrf(i64 m, i64 n, T* F, i64 ldf, i64* pivots, i64 bs, int check, i64* info, int threads) {
+int getrf(i64 m, i64 n, T* F, i64 ldf, i64* pivots, i64 bs, int check, i64* info, int threads,
+          bool tournament = false) {
     if (m < 0) return -1;
     if (n < 0) return -2;
     if (ldf < std::max<i64>(1, m)) return -4;
     if (bs <= 0) return -6;
     i64 q = std::min(m, n);
-    LuState st{pivots, 0, check != 0, false, threads};
+    LuState st{pivots, 0, check != 0, false, threads, tournament};
     for (i64 first = 0; first < q; first += bs) {
         i64 last = std::min(q, first + bs) - 1;  // inclusive
         lu_panel(F + first + first * ldf, ldf, m - first, last - first + 1, first, st);
@@ -947,6 +1051,10 @@ int oxo_dgetrf(i64 m, i64 n, double* A, i64 lda, i64* ipiv, i64 nb, int check, i
 }
 int oxo_sgetrf(i64 m, i64 n, float* A, i64 lda, i64* ipiv, i64 nb, int check, i64* info, int threads) {
     return getrf<float>(m, n, A, lda, ipiv, nb, check, info, threads);
+}
+// the same with tournament-pivoting leaves (tp_leaf)
+int oxo_sgetrf_tp(i64 m, i64 n, float* A, i64 lda, i64* ipiv, i64 nb, int check, i64* info, int threads) {
+    return getrf<float>(m, n, A, lda, ipiv, nb, check, info, threads, true);
 }
 // _lu_swap!(F, pivots, k1, k2, cols) (src/lapack/factorizations.jl:80-88) on
 // columns 0..ncols-1; inverse != 0 applies the steps in reverse order as

@@ -42,6 +42,7 @@ def lib():
         L.oxo_dtrsv.argtypes = [ch, ch, ch, i64, dp, i64, dp]
         L.oxo_dgetrf.argtypes = [i64, i64, dp, i64, ip, i64, C.c_int, ip, C.c_int]
         L.oxo_sgetrf.argtypes = [i64, i64, fp, i64, ip, i64, C.c_int, ip, C.c_int]
+        L.oxo_sgetrf_tp.argtypes = [i64, i64, fp, i64, ip, i64, C.c_int, ip, C.c_int]
         L.oxo_dlaswp.argtypes = [i64, dp, i64, i64, i64, ip, C.c_int]
         L.oxo_dgetri.argtypes = [i64, dp, i64, ip, dp, i64, C.c_int]
         L.oxo_dlogabsdet.argtypes = [i64, dp, i64, ip, dp, dp]
@@ -67,7 +68,7 @@ def lib():
                                       C.POINTER(C.c_int)]
         L.oxo_mixed_solve_chol.argtypes = L.oxo_mixed_solve.argtypes
         L.oxo_mixed_solve_chol.restype = C.c_int
-        for f in ("oxo_dgemm", "oxo_sgemm", "oxo_dtrsm", "oxo_dtrsv", "oxo_dgetrf", "oxo_sgetrf",
+        for f in ("oxo_dgemm", "oxo_sgemm", "oxo_dtrsm", "oxo_dtrsv", "oxo_dgetrf", "oxo_sgetrf", "oxo_sgetrf_tp",
                   "oxo_dgetrs", "oxo_sgetrs", "oxo_dlaswp", "oxo_dsyrk", "oxo_dpotrf", "oxo_dpotrs", "oxo_dgetri",
                   "oxo_drefine_solve", "oxo_dexpert_solve", "oxo_mixed_solve", "oxo_max_threads"):
             getattr(L, f).restype = C.c_int
@@ -158,13 +159,14 @@ def dgetrf(A, blocksize=128, check=False, threads=1):
     return F, ipiv[:min(m, n)], info.value
 
 
-def sgetrf(A, blocksize=128, check=False, threads=1):
+def sgetrf(A, blocksize=128, check=False, threads=1, tournament=False):
+    """FP32 lu_factor; tournament=True: the leaves of oxi_sgetrf_tournament."""
     F = np.array(A, dtype=np.float32, order="F", copy=True)
     m, n = F.shape
     ipiv = np.zeros(max(min(m, n), 1), dtype=np.int64)
     info = C.c_int64(0)
-    rc = lib().oxo_sgetrf(m, n, _ptr(F, fp), max(1, m), _ptr(ipiv, ip), blocksize, 1 if check else 0,
-                          C.byref(info), threads)
+    fn = lib().oxo_sgetrf_tp if tournament else lib().oxo_sgetrf
+    rc = fn(m, n, _ptr(F, fp), max(1, m), _ptr(ipiv, ip), blocksize, 1 if check else 0, C.byref(info), threads)
     _check(rc, "sgetrf")
     return F, ipiv[:min(m, n)], info.value
 

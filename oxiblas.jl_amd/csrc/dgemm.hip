@@ -110,6 +110,28 @@ __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int& 
     tn = in / gm;
 }
 
+// The same for a grid over the tiles of one triangle of a square tile grid
+// (T tiles a side, T (T + 1) / 2 workgroups): each XCD gets a contiguous
+// range of the lower triangle's tiles in row order (tile row tm holds tiles
+// 0 .. tm), so every XCD has the same number of tiles; an upper triangle
+// takes the transposed pairs.  (The rectangular grid with the tiles outside
+// the triangle leaving at once left some XCDs with almost no tiles.)
+__device__ __forceinline__ void tri_tile_of(int bid, int T, bool upper, int& tm, int& tn) {
+    const int total = T * (T + 1) / 2;
+    const int nx = 8;
+    int logical = bid;
+    if (total >= nx) {
+        int xcd = bid % nx, local = bid / nx, per = total / nx, rem = total % nx;
+        logical = xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+    }
+    int r = int((__builtin_sqrt(8.0 * double(logical) + 1.0) - 1.0) * 0.5);
+    while (r * (r + 1) / 2 > logical) --r;
+    while ((r + 1) * (r + 2) / 2 <= logical) ++r;
+    const int c = logical - r * (r + 1) / 2;
+    tm = upper ? c : r;
+    tn = upper ? r : c;
+}
+
 // Launch modes of the tile kernels.  PLAIN: one product.  STRIDED: product
 // blockIdx.y of a batch whose operands sit at fixed strides.  PTRS: product
 // blockIdx.y of a batch given by device arrays of operand pointers.  LOWER /
@@ -147,6 +169,19 @@ __device__ __forceinline__ bool keep(i64 row, i64 col) {
     if (MODE == UPPER) return row <= col;
     return true;
 }
+// The same for element (m0 + r, n0 + c) of a tile, from its offsets r, c in
+// the tile and dn = n0 - m0 clamped to +-2^30 (32-bit compares; a clamped
+// tile lies wholly on one side of the diagonal, and so do its elements)
+__device__ __forceinline__ int diag_offset(i64 m0, i64 n0) {
+    const i64 d = n0 - m0;
+    return d > (i64(1) << 30) ? (1 << 30) : d < -(i64(1) << 30) ? -(1 << 30) : int(d);
+}
+template <int MODE>
+__device__ __forceinline__ bool keep_local(int r, int c, int dn) {
+    if (MODE == LOWER) return r - c >= dn;
+    if (MODE == UPPER) return r - c <= dn;
+    return true;
+}
 
 template <int BM, int BN, bool TA, bool TB, bool BETA0, int MODE = PLAIN>
 __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, double alpha,
@@ -163,7 +198,8 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(i64 M, i64 N, i64 K, dou
     const int lane = t & 63, wave = t >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     int tm, tn;
-    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    if ((MODE == LOWER || MODE == UPPER) && BM == BN) tri_tile_of(blockIdx.x, tiles_m, MODE == UPPER, tm, tn);
+    else tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const i64 m0 = i64(tm) * BM, n0 = i64(tn) * BN;
     if (!tile_in_triangle<MODE>(m0, n0, BM, BN)) return;
     batch_operands<MODE>(ba, A, B, C);
@@ -370,6 +406,7 @@ __device__ __forceinline__ void m16_tile(double (*As)[BK16 * 129], double (*Bs)[
         // round trip per half.
         constexpr int LDE = 129;
         const int er = t & 127, ec0 = t >> 7;     // row in the tile, first column (then every 2nd)
+        const int dn = diag_offset(m0, n0);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             if (wn == h) {
@@ -396,8 +433,13 @@ __device__ __forceinline__ void m16_tile(double (*As)[BK16 * 129], double (*Bs)[
                 for (int k = 0; k < 16; ++k) {
                     const int c = 16 * b + k;
                     const double v = alpha * E[(ec0 + 2 * c) * LDE + er];
-                    if (keep<MODE>(m0 + er, n0 + h * 64 + ec0 + 2 * c))
-                        cc[i64(2 * c) * ldc] = BETA0 ? v : __builtin_fma(beta, cv[b][k], v);
+                    const bool in = keep_local<MODE>(er, h * 64 + ec0 + 2 * c, dn);
+                    if (BETA0) {
+                        if (in) cc[i64(2 * c) * ldc] = v;
+                    } else {
+                        // outside the triangle the element gets back the value just read
+                        cc[i64(2 * c) * ldc] = in ? __builtin_fma(beta, cv[b][k], v) : cv[b][k];
+                    }
                 }
             if (h == 0) __syncthreads();
         }
@@ -406,6 +448,7 @@ __device__ __forceinline__ void m16_tile(double (*As)[BK16 * 129], double (*Bs)[
     // Edge tile: one block column at a time, its C loads (issued without
     // branches; an element outside C reads the nearest one inside and is
     // not stored) before its stores.
+    const int dn = diag_offset(m0, n0);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const i64 col = n0 + wn * 64 + j * 16 + li;
@@ -426,7 +469,8 @@ __device__ __forceinline__ void m16_tile(double (*As)[BK16 * 129], double (*Bs)[
             for (int r = 0; r < 4; ++r) {
                 const i64 row = m0 + wm * 64 + i * 16 + lk + 4 * r;
                 const double v = alpha * acc[i][j][r];
-                if ((!EDGE || (row < M && col < N)) && keep<MODE>(row, col))
+                if ((!EDGE || (row < M && col < N)) &&
+                    keep_local<MODE>(wm * 64 + i * 16 + lk + 4 * r, wn * 64 + j * 16 + li, dn))
                     C[col * ldc + row] = BETA0 ? v : __builtin_fma(beta, cv[i][r], v);
             }
     }
@@ -443,7 +487,8 @@ __global__ __launch_bounds__(NT, 2) void dgemm_m16_kernel(i64 M, i64 N, i64 K, d
     double (*As)[BK16 * 129] = (double (*)[BK16 * 129])lds;
     double (*Bs)[BK16 * 129] = (double (*)[BK16 * 129])(lds + 2 * BK16 * 129);
     int tm, tn;
-    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    if (MODE == LOWER || MODE == UPPER) tri_tile_of(blockIdx.x, tiles_m, MODE == UPPER, tm, tn);
+    else tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const i64 m0 = i64(tm) * 128, n0 = i64(tn) * 128;
     if (!tile_in_triangle<MODE>(m0, n0, 128, 128)) return;
     batch_operands<MODE>(ba, A, B, C);
@@ -477,7 +522,9 @@ template <int BM, int BN, bool TA, bool TB, int MODE = PLAIN>
 void launch_tiles(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda, const double* B,
                   i64 ldb, double beta, double* C, i64 ldc, unsigned batch = 1, BatchArg ba = BatchArg{}) {
     const int tiles_m = int(cdiv(m, BM)), tiles_n = int(cdiv(n, BN));
-    dim3 grid(tiles_m * tiles_n, batch), block(NT);
+    // one triangle of a square grid of square tiles: only its tiles launch
+    const bool tri = (MODE == LOWER || MODE == UPPER) && BM == BN;
+    dim3 grid(tri ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n, batch), block(NT);
     if (beta == 0.0)
         hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, TA, TB, true, MODE>), grid, block, 0, s, m, n, k, alpha, A, lda,
                            B, ldb, beta, C, ldc, tiles_m, tiles_n, ba);
@@ -490,7 +537,8 @@ template <bool TA, bool TB, int MODE = PLAIN>
 void launch_m16(hipStream_t s, i64 m, i64 n, i64 k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
                 double beta, double* C, i64 ldc, unsigned batch = 1, BatchArg ba = BatchArg{}) {
     const int tiles_m = int(cdiv(m, 128)), tiles_n = int(cdiv(n, 128));
-    dim3 grid(tiles_m * tiles_n, batch), block(NT);
+    const bool tri = MODE == LOWER || MODE == UPPER;   // one triangle of a square grid: only its tiles launch
+    dim3 grid(tri ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n, batch), block(NT);
     if (beta == 0.0)
         hipLaunchKernelGGL((dgemm_m16_kernel<TA, TB, true, MODE>), grid, block, 0, s, m, n, k, alpha, A, lda, B, ldb,
                            beta, C, ldc, tiles_m, tiles_n, ba);

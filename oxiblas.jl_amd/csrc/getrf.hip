@@ -353,6 +353,7 @@ struct GetrfCtx {
     int* pairs;
     int* flags;
     int prec;          // FP32 only: -1 FP32 trailing update, 0 BF16, 1 FP16, 2 FP8 e4m3
+    bool tournament = false;   // FP32 only: tournament-pivoting leaves (panel_tp.hip)
     void* lowp_ws; size_t lowp_bytes;
     i64 pc0 = 0, pc1 = 0;   // the top-level panel being factored: columns [pc0, pc1)
 };
@@ -407,6 +408,7 @@ int leaf(GetrfCtx<T>& g, i64 c0, int w) {
         return oxi_set_error(g.h, OXI_ERR_UNSUPPORTED, "panel of %lld rows exceeds the leaf kernel capacity",
                              (long long)mrows);
     LeafArgs a{(void*)g.Wm, g.ldw, c0, w, mrows, g.ipiv, g.dinfo, g.tag0, g.pub, int(P), g.flags, g.pc0, g.pc1};
+    if (sizeof(T) == 4 && g.tournament && kind <= 1 && P <= OXI_TP_MAX_P) return oxi_launch_lu_leaf_tp(g.h, g.s, a, kind);
     if (sizeof(T) == 8) OXI_TRY(oxi_launch_lu_leaf(g.h, g.s, a, kind));
     else OXI_TRY(oxi_launch_lu_leaf32(g.h, g.s, a, kind));
     return OXI_OK;
@@ -472,7 +474,8 @@ int choose_nb(i64 m, i64 n) {
 }
 
 template <typename T>
-int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64* info, int prec) {
+int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64* info, int prec,
+            bool tournament = false) {
     (void)hipSetDevice(h->device);
     hipStream_t s = h->stream;
     const i64 q = std::min(m, n);
@@ -514,6 +517,7 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     g.tag0 = (++h->getrf_seq) << 32;
     g.flags = h->dev_flags;
     g.prec = prec;
+    g.tournament = tournament;
     g.lowp_ws = base + off_lowp; g.lowp_bytes = lowp_bytes;
 
     static bool attr_set = false;
@@ -660,8 +664,9 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
 
 }  // namespace
 
-int oxi_getrf32(oxi_handle_t h, i64 m, i64 n, float* A, i64 lda, i64* ipiv, i64 nb, i64* info, int prec) {
-    return getrf_t<float>(h, m, n, A, lda, ipiv, nb, info, prec);
+int oxi_getrf32(oxi_handle_t h, i64 m, i64 n, float* A, i64 lda, i64* ipiv, i64 nb, i64* info, int prec,
+                bool tournament) {
+    return getrf_t<float>(h, m, n, A, lda, ipiv, nb, info, prec, tournament);
 }
 
 extern "C" int oxi_dgetrf(oxi_handle_t h, int64_t m, int64_t n, double* A, int64_t lda, int64_t* ipiv, int64_t nb,
@@ -680,8 +685,8 @@ extern "C" int oxi_dgetrf(oxi_handle_t h, int64_t m, int64_t n, double* A, int64
     return getrf_t<double>(h, m, n, A, lda, ipiv, nb, info, -1);
 }
 
-extern "C" int oxi_sgetrf_lowp(oxi_handle_t h, int prec, int64_t m, int64_t n, float* A, int64_t lda, int64_t* ipiv,
-                               int64_t nb, int64_t* info) {
+static int sgetrf_entry(oxi_handle_t h, int prec, int64_t m, int64_t n, float* A, int64_t lda, int64_t* ipiv,
+                        int64_t nb, int64_t* info, bool tournament) {
     if (!h) return -1;
     if (prec < OXI_PREC_BF16 || prec > OXI_PREC_FP32) return -2;
     if (m < 0) return -3;
@@ -694,7 +699,17 @@ extern "C" int oxi_sgetrf_lowp(oxi_handle_t h, int prec, int64_t m, int64_t n, f
     if (m == 0 || n == 0) return OXI_OK;
     if (!A) return -5;
     if (!ipiv) return -7;
-    return getrf_t<float>(h, m, n, A, lda, ipiv, nb, info, prec == OXI_PREC_FP32 ? -1 : prec);
+    return getrf_t<float>(h, m, n, A, lda, ipiv, nb, info, prec == OXI_PREC_FP32 ? -1 : prec, tournament);
+}
+
+extern "C" int oxi_sgetrf_lowp(oxi_handle_t h, int prec, int64_t m, int64_t n, float* A, int64_t lda, int64_t* ipiv,
+                               int64_t nb, int64_t* info) {
+    return sgetrf_entry(h, prec, m, n, A, lda, ipiv, nb, info, false);
+}
+
+extern "C" int oxi_sgetrf_tournament(oxi_handle_t h, int prec, int64_t m, int64_t n, float* A, int64_t lda,
+                                     int64_t* ipiv, int64_t nb, int64_t* info) {
+    return sgetrf_entry(h, prec, m, n, A, lda, ipiv, nb, info, true);
 }
 
 // ---------------------------------------------------------------------------

@@ -738,7 +738,7 @@ extern "C" int oxi_dgesv_mixed(oxi_handle_t h, int prec, int solver, int64_t n, 
     if (maxiter < 0) return -12;
     if (!std::isfinite(rtol) || rtol < 0) return -13;
     if (gmres_restart < 0 || gmres_restart > 1000) return -14;
-    if (pivot < 0 || pivot > 1) return -15;
+    if (pivot < 0 || pivot > 2) return -15;
     if (!info) return -16;
     std::lock_guard<std::recursive_mutex> lk(h->guard);
     *info = oxi_ir_info{};
@@ -852,7 +852,7 @@ extern "C" int oxi_dgesv_mixed(oxi_handle_t h, int prec, int solver, int64_t n, 
                 OXI_TRY(launch_ok(h, "equilibrate"));
             }
         }
-        if (!fl[1] && !c.rbt) OXI_TRY(oxi_getrf32(h, n, n, c.A32, n, c.ipiv, 0, &finfo, lprec));
+        if (!fl[1] && !c.rbt) OXI_TRY(oxi_getrf32(h, n, n, c.A32, n, c.ipiv, 0, &finfo, lprec, pivot == 2));
         if (fl[1] || finfo != 0) {
             c.fallback = true;
             if (hipMalloc(&c.E64, nn * 8) != hipSuccess) {

@@ -143,7 +143,9 @@ size_t oxi_getrf_nopiv_ws(int prec, i64 n);
 int oxi_getrf_nopiv32(oxi_handle_t h, hipStream_t s, int prec, i64 n, float* A, i64 lda, i64* dinfo, void* ws,
                       size_t ws_bytes);
 // FP32 getrf with a trailing update in prec (-1 FP32, 0 BF16, 1 FP16, 2 FP8)
-int oxi_getrf32(oxi_handle_t h, i64 m, i64 n, float* A, i64 lda, i64* ipiv, i64 nb, i64* info, int prec);
+// tournament: tournament-pivoting leaves where the panel allows them
+int oxi_getrf32(oxi_handle_t h, i64 m, i64 n, float* A, i64 lda, i64* ipiv, i64 nb, i64* info, int prec,
+                bool tournament = false);
 
 // First zero on the diagonal of an n x n triangle (1-based, 0 if none);
 // synchronises the stream.
@@ -197,6 +199,10 @@ struct LeafArgs {
 
 // kind 0: 256 threads x 1 row per workgroup; kind 1: 256 threads x 2 rows.
 int oxi_launch_lu_leaf(oxi_handle_t h, hipStream_t s, const LeafArgs& a, int kind);
+// FP32 leaf with tournament pivoting (panel_tp.hip), kind 0 or 1, at most
+// OXI_TP_MAX_P workgroups.
+constexpr int OXI_TP_MAX_P = 32;
+int oxi_launch_lu_leaf_tp(oxi_handle_t h, hipStream_t s, const LeafArgs& a, int kind);
 // FP32 leaf; kind 2: 256 threads x 4 rows.
 int oxi_launch_lu_leaf32(oxi_handle_t h, hipStream_t s, const LeafArgs& a, int kind);
 

@@ -531,7 +531,7 @@ __device__ __forceinline__ void leaf_slots(i64 pi_all, int w, int k_lo, int lane
 // update reads them.
 template <typename T>
 __global__ __launch_bounds__(128) void leaf_swaps_kernel(T* Wm, i64 ldw, i64 c0, int w, const i64* ipiv,
-                                                         const int* flags, i64 x_lo, i64 x_hi) {
+                                                         const int* flags, i64 x_lo, i64 x_hi, int first_block = 0) {
     const int lane = threadIdx.x & 63;
     const int half = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     // the pivots are loaded with the flag, not after its test, and without a
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(128) void leaf_swaps_kernel(T* Wm, i64 ldw, i64 c0,
     const i64 pv = ipiv[c0 + (lane < w ? lane : 0)];
     if (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     const i64 pi_all = lane < w ? pv - 1 - c0 : -1;
-    const int b = blockIdx.x;
+    const int b = int(blockIdx.x) + first_block;   // first_block = w - 1: the leaf's own columns are final
     if (b < w - 1) {
         if (half != 0) return;
         const int c = b;

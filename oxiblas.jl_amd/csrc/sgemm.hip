@@ -11,6 +11,8 @@
 // 4*(l/16)+r, column l%16.
 #include "oxi_internal.hpp"
 
+#include <cstdlib>
+
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 namespace {
@@ -141,6 +143,156 @@ __global__ __launch_bounds__(NT) void sgemm_mfma_kernel(i64 M, i64 N, i64 K, flo
         }
 }
 
+// Large products: 128 x 128 tiles with BK = 32 slabs (67.6 KB of LDS, two
+// workgroups per CU, at most 128 registers a lane), each wave 64 x 64 as 4 x 4
+// blocks of 16 x 16.  Tiles inside C with K a multiple of 32 load their slabs
+// without bounds checks and stage C through LDS for coalesced column passes
+// (a wave covers 64 consecutive rows of a column); edge tiles check every
+// element.  (The 128 x 128 form of sgemm_mfma_kernel above needs 260
+// registers, one workgroup per CU, and reads C sixteen columns at a time.)
+constexpr int BKL = 32, LDL = 129;
+
+template <bool OUTER_CONTIG, int O, bool EDGE>
+__device__ __forceinline__ void load_slab_l(float (&r)[O * BKL / NT], const float* __restrict__ X, i64 ldx, i64 o0,
+                                            i64 k0, i64 outer, i64 K, int t) {
+#pragma unroll
+    for (int i = 0; i < O * BKL / NT; ++i) {
+        int o, kk;
+        if (OUTER_CONTIG) { o = t % O; kk = t / O + i * (NT / O); }
+        else { kk = t % BKL; o = t / BKL + i * (NT / BKL); }
+        const i64 go = o0 + o, gk = k0 + kk;
+        if (EDGE) r[i] = (go < outer && gk < K) ? (OUTER_CONTIG ? X[go + gk * ldx] : X[gk + go * ldx]) : 0.0f;
+        else r[i] = OUTER_CONTIG ? X[go + gk * ldx] : X[gk + go * ldx];
+    }
+}
+template <bool OUTER_CONTIG, int O>
+__device__ __forceinline__ void store_slab_l(float* __restrict__ S, const float (&r)[O * BKL / NT], int t) {
+#pragma unroll
+    for (int i = 0; i < O * BKL / NT; ++i) {
+        int o, kk;
+        if (OUTER_CONTIG) { o = t % O; kk = t / O + i * (NT / O); }
+        else { kk = t % BKL; o = t / BKL + i * (NT / BKL); }
+        S[kk * LDL + o] = r[i];
+    }
+}
+
+template <bool EDGE, bool TA, bool TB, bool BETA0>
+__device__ __forceinline__ void sgemm_l_tile(float* lds, i64 M, i64 N, i64 K, float alpha, const float* __restrict__ A,
+                                             i64 lda, const float* __restrict__ B, i64 ldb, float beta,
+                                             float* __restrict__ C, i64 ldc, i64 m0, i64 n0) {
+    constexpr bool A_OC = !TA, B_OC = TB;
+    float* As[2] = {lds, lds + BKL * LDL};
+    float* Bs[2] = {lds + 2 * BKL * LDL, lds + 3 * BKL * LDL};
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int lrow = lane >> 4, lcol = lane & 15;
+    f4v acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
+    float ra[128 * BKL / NT], rb[128 * BKL / NT];
+    const int nslab = int((K + BKL - 1) / BKL);
+    load_slab_l<A_OC, 128, EDGE>(ra, A, lda, m0, 0, M, K, t);
+    load_slab_l<B_OC, 128, EDGE>(rb, B, ldb, n0, 0, N, K, t);
+    store_slab_l<A_OC, 128>(As[0], ra, t);
+    store_slab_l<B_OC, 128>(Bs[0], rb, t);
+    __syncthreads();
+    for (int s = 0; s < nslab; ++s) {
+        const int cur = s & 1;
+        if (s + 1 < nslab) {
+            load_slab_l<A_OC, 128, EDGE>(ra, A, lda, m0, i64(s + 1) * BKL, M, K, t);
+            load_slab_l<B_OC, 128, EDGE>(rb, B, ldb, n0, i64(s + 1) * BKL, N, K, t);
+        }
+        const float* as = As[cur];
+        const float* bs = Bs[cur];
+#pragma unroll
+        for (int ks = 0; ks < BKL / 4; ++ks) {
+            const int kr = ks * 4 + lrow;
+            float a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = as[kr * LDL + wm * 64 + i * 16 + lcol];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = bs[kr * LDL + wn * 64 + j * 16 + lcol];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (s + 1 < nslab) {
+            store_slab_l<A_OC, 128>(As[cur ^ 1], ra, t);
+            store_slab_l<B_OC, 128>(Bs[cur ^ 1], rb, t);
+        }
+        __syncthreads();
+    }
+    if (!EDGE) {
+        // C in two halves of 64 columns staged through LDS (128 x 64, column
+        // stride 129) by the two waves that hold them; then a thread per row
+        // of the tile and every other column, all its loads before its stores
+        float* E = lds;
+        const int er = t & 127, ec0 = t >> 7;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (wn == h) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) E[(j * 16 + lcol) * LDL + wm * 64 + i * 16 + 4 * lrow + r] = acc[i][j][r];
+            }
+            __syncthreads();
+            float* cc = C + (n0 + h * 64 + ec0) * ldc + m0 + er;
+            float cv[32];
+            if (!BETA0) {
+#pragma unroll
+                for (int k = 0; k < 32; ++k) cv[k] = cc[i64(2 * k) * ldc];
+            }
+#pragma unroll
+            for (int k = 0; k < 32; ++k) {
+                const float v = alpha * E[(ec0 + 2 * k) * LDL + er];
+                cc[i64(2 * k) * ldc] = BETA0 ? v : __builtin_fmaf(beta, cv[k], v);
+            }
+            if (h == 0) __syncthreads();
+        }
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const i64 col = n0 + wn * 64 + j * 16 + lcol;
+        const i64 colc = min(col, N - 1);
+        float cv[4][4];
+        if (!BETA0) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) cv[i][r] = C[colc * ldc + min(m0 + wm * 64 + i * 16 + 4 * lrow + r, M - 1)];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const i64 row = m0 + wm * 64 + i * 16 + 4 * lrow + r;
+                const float v = alpha * acc[i][j][r];
+                if (row < M && col < N) C[col * ldc + row] = BETA0 ? v : __builtin_fmaf(beta, cv[i][r], v);
+            }
+    }
+}
+
+template <bool TA, bool TB, bool BETA0>
+__global__ __launch_bounds__(NT, 2) void sgemm_l_kernel(i64 M, i64 N, i64 K, float alpha, const float* __restrict__ A,
+                                                        i64 lda, const float* __restrict__ B, i64 ldb, float beta,
+                                                        float* __restrict__ C, i64 ldc, int tiles_m, int tiles_n) {
+    __shared__ __align__(16) float lds[4 * BKL * LDL];
+    int tm, tn;
+    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const i64 m0 = i64(tm) * 128, n0 = i64(tn) * 128;
+    if (m0 + 128 <= M && n0 + 128 <= N && K % BKL == 0)
+        sgemm_l_tile<false, TA, TB, BETA0>(lds, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
+    else
+        sgemm_l_tile<true, TA, TB, BETA0>(lds, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
+}
+
 __global__ void sscale_kernel(i64 m, i64 n, float beta, float* __restrict__ C, i64 ldc) {
     const i64 total = m * n;
     for (i64 idx = i64(blockIdx.x) * blockDim.x + threadIdx.x; idx < total; idx += i64(gridDim.x) * blockDim.x) {
@@ -171,7 +323,18 @@ template <bool TA, bool TB>
 void launch_variant(hipStream_t s, i64 m, i64 n, i64 k, float alpha, const float* A, i64 lda, const float* B, i64 ldb,
                     float beta, float* C, i64 ldc) {
     constexpr i64 NCU = 256;
-    if (cdiv(m, 128) * cdiv(n, 128) >= NCU * 7 / 4) return launch_tiles<128, 128, TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (cdiv(m, 128) * cdiv(n, 128) >= NCU * 7 / 4) {
+        static const bool old_tiles = getenv("OXIBLAS_SGEMM_OLD") != nullptr;   // tuning: the first 128 x 128 form
+        if (old_tiles) return launch_tiles<128, 128, TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+        const int tiles_m = int(cdiv(m, 128)), tiles_n = int(cdiv(n, 128));
+        if (beta == 0.0f)
+            hipLaunchKernelGGL((sgemm_l_kernel<TA, TB, true>), dim3(tiles_m * tiles_n), dim3(NT), 0, s, m, n, k, alpha,
+                               A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
+        else
+            hipLaunchKernelGGL((sgemm_l_kernel<TA, TB, false>), dim3(tiles_m * tiles_n), dim3(NT), 0, s, m, n, k,
+                               alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
+        return;
+    }
     if (cdiv(m, 64) * cdiv(n, 64) >= NCU * 3 / 2) return launch_tiles<64, 64, TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     return launch_tiles<32, 32, TA, TB>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }

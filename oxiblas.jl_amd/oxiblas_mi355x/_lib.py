@@ -103,6 +103,7 @@ SIGNATURES = {
     "oxi_hgemm": (C.c_int, [_vp, _ch, _ch, _i64, _i64, _i64, _flt, _vp, _i64, _vp, _i64, _flt, _vp, _i64]),
     "oxi_gemm_lowp_update": (C.c_int, [_vp, C.c_int, _i64, _i64, _i64, _flt, _vp, _i64, _vp, _i64, _vp, _i64]),
     "oxi_sgetrf_lowp": (C.c_int, [_vp, C.c_int, _i64, _i64, _vp, _i64, _vp, _i64, C.POINTER(_i64)]),
+    "oxi_sgetrf_tournament": (C.c_int, [_vp, C.c_int, _i64, _i64, _vp, _i64, _vp, _i64, C.POINTER(_i64)]),
     "oxi_sgetrs": (C.c_int, [_vp, _ch, _i64, _i64, _vp, _i64, _vp, _vp, _i64]),
     "oxi_dgesv_mixed": (C.c_int, [_vp, C.c_int, C.c_int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, C.c_int32,
                                   _dbl, C.c_int32, C.c_int32, C.POINTER(IRInfo)]),

@@ -40,7 +40,7 @@ _PREC_NAMES = {
 _PREC_LABEL = {_lib.OXI_PREC_FP32: "Float32", _lib.OXI_PREC_BF16: "BFloat16",
                _lib.OXI_PREC_FP16: "Float16", _lib.OXI_PREC_FP8: "Float8_E4M3FN"}
 _SOLVERS = {"ir": 0, "gmres": 1, "gmres_ir": 1, "auto": 2}
-_PIVOTS = {"partial": 0, "rbt": 1}
+_PIVOTS = {"partial": 0, "rbt": 1, "tournament": 2}
 _REASONS = ("converged", "iteration_limit", "nonfinite_iteration", "working_factor_fallback")
 
 MixedSolveResult = namedtuple(
@@ -88,7 +88,10 @@ def mixed_solve(A, b, maxiter=10, rtol=1e-13, lowtype=None, fallback=True, facto
     applies a random butterfly transform and factors without pivoting (no
     device-wide pivot search per column; refinement restores the accuracy).
     "rbt" applies to the Float32 factor; other low types, n not a multiple of
-    4 or a zero pivot use partial pivoting.  Returns the reference's NamedTuple fields (x, converged, iterations,
+    4 or a zero pivot use partial pivoting.  "tournament" chooses each
+    32-column leaf's pivots by a tournament over its row blocks (one exchange
+    per leaf instead of one per column; panels above 16384 rows pivot
+    partially).  Neither option has a reference counterpart.  Returns the reference's NamedTuple fields (x, converged, iterations,
     residual, backward_error, metric, low_type, fallback, factorization,
     finite, reason) plus the GPU's normwise error, GMRES iteration count and
     device timings."""
@@ -103,7 +106,7 @@ def mixed_solve(A, b, maxiter=10, rtol=1e-13, lowtype=None, fallback=True, facto
     if solver not in _SOLVERS:
         raise ValueError("solver must be ir, gmres or auto")
     if pivot not in _PIVOTS:
-        raise ValueError("pivot must be partial or rbt")
+        raise ValueError("pivot must be partial, rbt or tournament")
     prec = precision_code(lowtype)
     vec = not isinstance(b, DMatrix) and np.ndim(b) == 1
     Ad, Bd = as_dmatrix(A), as_dmatrix(b)
@@ -264,17 +267,21 @@ class LUFactor32:
         return out[:, 0].copy() if np.ndim(B) == 1 else out
 
 
-def lu_lowp(A, lowtype="float32", check=True, blocksize=0):
+def lu_lowp(A, lowtype="float32", check=True, blocksize=0, pivot="partial"):
     """FP32 blocked LU with partial pivoting whose trailing products run in
-    `lowtype`; A is copied (a float32 device matrix or any host array)."""
+    `lowtype`; A is copied (a float32 device matrix or any host array).
+    pivot "tournament" chooses each 32-column leaf's pivots by a tournament
+    over its row blocks (oxi_sgetrf_tournament; no reference counterpart)."""
+    if pivot not in ("partial", "tournament"):
+        raise ValueError("pivot must be partial or tournament")
     import torch
     Ad = A.copy() if isinstance(A, DMatrix) else DMatrix.from_numpy(A, dtype=np.float32)
     q = min(Ad.m, Ad.n)
     ipiv = torch.zeros(max(q, 1), dtype=torch.int64, device=Ad.buf.device)
     info = C.c_int64(0)
     h = _h()
-    rc = h._L.oxi_sgetrf_lowp(h.ptr, precision_code(lowtype), Ad.m, Ad.n, Ad.ptr, Ad.ld, ipiv.data_ptr(),
-                              int(blocksize), C.byref(info))
+    fn = h._L.oxi_sgetrf_lowp if pivot == "partial" else h._L.oxi_sgetrf_tournament
+    rc = fn(h.ptr, precision_code(lowtype), Ad.m, Ad.n, Ad.ptr, Ad.ld, ipiv.data_ptr(), int(blocksize), C.byref(info))
     h.check(rc, "lu")
     if check and info.value != 0:
         raise SingularException(info.value)

@@ -41,7 +41,9 @@ def test_potrf_reads_lower_triangle_only(gpu, rng):
 
 def test_potrf_info_and_exceptions(gpu, rng):
     ox = gpu
-    for n, j in ((10, 4), (100, 70), (300, 1), (300, 150), (700, 640)):
+    # the last three run the blocked form (256-column blocks): a failing column
+    # at the end and the start of a block, and inside a later one
+    for n, j in ((10, 4), (100, 70), (300, 1), (300, 150), (700, 640), (1024, 256), (1024, 257), (1500, 1300)):
         A = spd(rng, n)
         A[j - 1, j - 1] = -1e3 * n
         with pytest.raises(ox.PosDefException) as e:

@@ -104,6 +104,13 @@ def test_mixed_n16384(gpu):
         if low == "float32":
             assert r.converged
             assert fwd <= 100 * 1e6 * EPS * math.sqrt(n)
+    # the FP32 factor with tournament pivoting (pivot 2) meets the same bar
+    r = gpu.mixed_solve(A, bd, lowtype="float32", solver="gmres", pivot="tournament", maxiter=30,
+                        rtol=1e-15 * n, gmres_restart=50)
+    fwd = (torch.linalg.norm(r.x.buf[:n] - xt) / torch.linalg.norm(xt)).item()
+    print(f"n={n} kappa=1e6 float32/gmres/tournament: converged={r.converged} berr={r.backward_error:.2e} "
+          f"fwd={fwd:.2e} it={r.iterations}/{r.inner_iterations} factor {r.factor_ms:.1f} ms refine {r.refine_ms:.1f} ms")
+    assert r.converged and not r.fallback and fwd <= 100 * 1e6 * EPS * math.sqrt(n)
 
 
 def test_adjoint_ldiv_n8192(gpu):

@@ -299,7 +299,7 @@ def test_pointer_batched_mixed_shapes(gpu, rng):
 @pytest.mark.parametrize("uplo", "UL")
 @pytest.mark.parametrize("trans", "NT")
 def test_syrk_vs_oracle(gpu, rng, uplo, trans):
-    for (n, k) in ((1, 1), (17, 9), (64, 64), (130, 257), (300, 40), (1000, 128)):
+    for (n, k) in ((1, 1), (17, 9), (64, 64), (130, 257), (300, 40), (1000, 128), (3000, 48)):   # 3000: 128 x 128 tiles
         A = rng.standard_normal((n, k) if trans == "N" else (k, n))
         C0 = np.asfortranarray(rng.standard_normal((n, n)))
         got = gpu.syrk_(C0.copy(order="F"), A, 0.7, -0.3, uplo, trans)
@@ -308,6 +308,14 @@ def test_syrk_vs_oracle(gpu, rng, uplo, trans):
         np.testing.assert_allclose(got, ref, rtol=3e-12, atol=3e-12 * k)
         mask = np.tril(np.ones((n, n), bool), -1) if uplo == "U" else np.triu(np.ones((n, n), bool), 1)
         assert np.array_equal(got[mask], C0[mask])         # the other triangle is untouched
+    # NaN in the other triangle stays NaN on the 128 x 128 tiles as well
+    n = 3000
+    A = rng.standard_normal((n, 40) if trans == "N" else (40, n))
+    C0 = np.asfortranarray(rng.standard_normal((n, n)))
+    mask = np.tril(np.ones((n, n), bool), -1) if uplo == "U" else np.triu(np.ones((n, n), bool), 1)
+    C0[mask] = np.nan
+    got = gpu.syrk_(C0.copy(order="F"), A, -1.0, 1.0, uplo, trans)
+    assert np.all(np.isnan(got[mask])) and np.all(np.isfinite(got[~mask]))
     A = rng.standard_normal((40, 10))
     got = gpu.syrk_(np.full((40, 40), np.nan), A, 1.0, 0.0, uplo)   # beta 0 never reads C
     tri = np.triu_indices(40) if uplo == "U" else np.tril_indices(40)
@@ -319,6 +327,35 @@ def test_syrk_vs_oracle(gpu, rng, uplo, trans):
     assert np.array_equal(got, expect)
     with pytest.raises(ValueError):
         gpu.syrk_(np.zeros((4, 4)), np.zeros((4, 2)), trans="C")
+
+
+@pytest.mark.parametrize("ta", "NT")
+@pytest.mark.parametrize("tb", "NT")
+def test_sgemm_vs_oracle(gpu, rng, ta, tb):
+    # Float32 gemm! (src/blas/gemm.jl:625-703) on every tile path: 32 x 32 and
+    # 64 x 64 (small grids), 128 x 128 interior tiles (K a multiple of 32),
+    # edge tiles and K not a multiple of 32; normwise error <= k eps32 |A||B|
+    # against the float64 product, and the oracle's Float32 gemm! beside it
+    e32 = float(np.finfo(np.float32).eps)
+    for (m, n, k) in ((17, 9, 40), (300, 260, 200), (2816, 2816, 128), (2900, 2830, 96), (2816, 2944, 45)):
+        A = rng.standard_normal((m, k) if ta == "N" else (k, m)).astype(np.float32)
+        B = rng.standard_normal((k, n) if tb == "N" else (n, k)).astype(np.float32)
+        C0 = rng.standard_normal((m, n)).astype(np.float32)
+        Ad = gpu.DMatrix.from_numpy(A, dtype=np.float32)
+        Bd = gpu.DMatrix.from_numpy(B, dtype=np.float32)
+        for beta in (0.0, -0.5):
+            Cd = gpu.DMatrix.from_numpy(C0 if beta else np.full((m, n), np.nan, np.float32), dtype=np.float32)
+            gpu.sgemm_(Cd, Ad, Bd, 1.25, beta, ta, tb)
+            got = Cd.numpy().astype(np.float64)
+            opA = (A if ta == "N" else A.T).astype(np.float64)
+            opB = (B if tb == "N" else B.T).astype(np.float64)
+            exact = 1.25 * opA @ opB + beta * C0.astype(np.float64)
+            err = np.linalg.norm(got - exact)
+            assert err <= 2 * k * e32 * (1.25 * np.linalg.norm(opA) * np.linalg.norm(opB) + abs(beta) * np.linalg.norm(C0))
+            if m * n <= 300 * 300:
+                Co = np.asfortranarray(C0.copy())
+                O.sgemm(Co, A, B, 1.25, beta, ta, tb)
+                np.testing.assert_allclose(got, Co, rtol=0, atol=4 * k * e32 * np.abs(exact).max())
 
 
 @pytest.mark.parametrize("ta", "NT")

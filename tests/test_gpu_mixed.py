@@ -142,6 +142,70 @@ def _fp8_block_round(X, axis):
     return out, scales
 
 
+@pytest.mark.parametrize("n", [64, 300, 1000, 3000])
+def test_tournament_lu_matches_oracle(gpu, rng, n):
+    # oxi_sgetrf_tournament against the oracle's restatement (tp_leaf): the
+    # same pivots, factors to a few n*eps32, and the reconstruction bound
+    A = rng.standard_normal((n, n)).astype(np.float32)
+    F = gpu.lu_lowp(A, "float32", pivot="tournament")
+    Fo, ipo, info = O.sgetrf(A, tournament=True, threads=8)
+    assert F.info == 0 and info == 0
+    Fh = F.factors.numpy()
+    e32 = float(np.finfo(np.float32).eps)
+    # the first leaf sees A itself, the same operations in both: its pivots
+    # (a tournament over up to 12 blocks of 256 rows here) agree exactly; later
+    # leaves see trailing updates rounded in different orders, and in Float32
+    # the top two candidates of a column lie closer than that rounding beyond
+    # about 1000 rows, as for the partial-pivoting factor
+    assert np.array_equal(F.pivots[:32], ipo[:32])
+    if n <= 1000:
+        assert np.array_equal(F.pivots, ipo)
+        assert np.abs(Fh.astype(np.float64) - Fo).max() <= 4 * n * e32 * max(1.0, np.abs(Fo).max())
+    L = np.tril(Fh.astype(np.float64), -1) + np.eye(n); U = np.triu(Fh.astype(np.float64))
+    p = O.perm_from_ipiv(F.pivots, n)
+    A64 = A.astype(np.float64)
+    assert np.linalg.norm(L @ U - A64[p - 1]) <= 30 * n * e32 * np.linalg.norm(A64)
+    # a zero column is reported at its step, as by the partial-pivoting leaf
+    Z = A.copy(); Z[:, min(n - 1, 40)] = 0
+    Fz = gpu.lu_lowp(Z, "float32", pivot="tournament", check=False)
+    assert Fz.info == O.sgetrf(Z, tournament=True)[2] == min(n - 1, 40) + 1
+
+
+def test_tournament_lu_n8192_and_solve(gpu):
+    """Larger panels (512-row blocks beyond 8192 rows do not occur; 32 blocks
+    of 256 rows per leaf at the top): reconstruction on the GPU in FP64, and
+    the mixed solve with tournament pivoting to the same accuracy."""
+    import torch
+    n = 8192
+    g = torch.Generator(device="cuda").manual_seed(7)
+    A = torch.randn(n, n, dtype=torch.float32, device="cuda", generator=g)
+    Ad = gpu.DMatrix(A.T.contiguous().reshape(-1), n, n)     # column-major float32
+    F = gpu.lu_lowp(Ad, "float32", pivot="tournament")
+    Ft = F.factors.buf[: n * n].view(n, n).T.double()          # column-major -> (row, col)
+    L = torch.tril(Ft, -1) + torch.eye(n, dtype=torch.float64, device="cuda")
+    U = torch.triu(Ft)
+    p = torch.from_numpy(O.perm_from_ipiv(F.pivots, n) - 1).to("cuda")
+    e32 = float(np.finfo(np.float32).eps)
+    A64 = A.double()
+    err = torch.linalg.matrix_norm(L @ U - A64[p]).item() / (n * e32 * torch.linalg.matrix_norm(A64).item())
+    assert err <= 30
+    rng = np.random.default_rng(3)
+    B = rng.standard_normal((2048, 2048)) + 2048 * np.eye(2048)
+    b = rng.standard_normal(2048)
+    r = gpu.mixed_solve(B, b, lowtype="float32", pivot="tournament", maxiter=10, rtol=16 * EPS)
+    assert r.converged and r.backward_error <= 16 * EPS and not r.fallback
+
+
+@pytest.mark.parametrize("kappa", [1e2, 1e6])
+def test_mixed_solve_tournament_conditioned(gpu, rng, kappa):
+    n = 1024
+    A = conditioned(rng, n, kappa); xt = rng.standard_normal(n); b = A @ xt
+    for solver in ("ir", "gmres"):
+        r = gpu.mixed_solve(A, b, lowtype="float32", solver=solver, pivot="tournament", maxiter=30, rtol=16 * EPS)
+        assert r.converged and r.backward_error <= 16 * EPS and not r.fallback
+        assert np.linalg.norm(r.x - xt) / np.linalg.norm(xt) <= 50 * kappa * EPS
+
+
 @pytest.mark.parametrize("prec", ["bfloat16", "float16", "float8_e4m3fn"])
 @pytest.mark.parametrize("M,N,K", [(300, 260, 200), (512, 640, 256), (33, 17, 5)])
 def test_lowp_update_vs_emulation(gpu, rng, prec, M, N, K):

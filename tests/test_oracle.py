@@ -344,6 +344,51 @@ def test_sgetrf_matches_lapack_sgetrf(rng, m, n, bs):
     assert np.abs(F.astype(np.float64) - lapack_lu).max() <= 64 * max(m, n) * EPS32 * np.abs(lapack_lu).max()
 
 
+@pytest.mark.parametrize("n", [5, 64, 200, 256])
+def test_sgetrf_tournament_one_block_is_partial_pivoting(rng, n):
+    # the tournament leaf of oxi_sgetrf_tournament: with one block of rows
+    # (at most 256) the candidates are the partial-pivoting pivots themselves
+    # and the elimination is partial pivoting's, operation for operation, so
+    # the factors are sgetrf's (pinned to LAPACK above) bit for bit
+    A = rng.standard_normal((n, n)).astype(np.float32)
+    F1, p1, _ = O.sgetrf(A)
+    F2, p2, _ = O.sgetrf(A, tournament=True)
+    assert np.array_equal(p1, p2) and np.array_equal(F1, F2)
+    assert np.array_equal(p2, sl.lu_factor(A, check_finite=False)[1][:n] + 1)
+
+
+@pytest.mark.parametrize("n", [300, 1000])
+def test_sgetrf_tournament_first_leaf_and_reconstruction(rng, n):
+    # several blocks of 256 rows: the first leaf's pivot rows are LAPACK
+    # sgetrf's pivots over the union of the pivot rows LAPACK picks in each
+    # block (columns 0..31), and the whole factor reconstructs P*A to the
+    # Float32 bound of test/lapack/lapack_dense.jl:11
+    A = rng.standard_normal((n, n)).astype(np.float32)
+    F, ipiv, info = O.sgetrf(A, tournament=True, threads=8)
+    assert info == 0
+    p = O.perm_from_ipiv(ipiv, n) - 1
+    assert np.array_equal(np.sort(p), np.arange(n))
+    cand = []
+    for g0 in range(0, n, 256):
+        blk = A[g0:g0 + 256, :32]
+        _, bp = sl.lu_factor(blk, check_finite=False)
+        order = np.arange(blk.shape[0])
+        for j, q in enumerate(bp):
+            order[[j, q]] = order[[q, j]]
+        cand.extend(g0 + order[:min(32, blk.shape[0])])
+    cand = np.sort(np.array(cand))
+    _, cp = sl.lu_factor(A[cand, :32], check_finite=False)
+    order = np.arange(len(cand))
+    for j, q in enumerate(cp):
+        order[[j, q]] = order[[q, j]]
+    assert np.array_equal(p[:32], cand[order[:32]])
+    L = np.tril(F.astype(np.float64), -1) + np.eye(n)
+    U = np.triu(F.astype(np.float64))
+    A64 = A.astype(np.float64)
+    assert np.linalg.norm(L @ U - A64[p]) <= 30 * n * EPS32 * np.linalg.norm(A64)
+    assert not np.array_equal(p, O.perm_from_ipiv(O.sgetrf(A)[1], n) - 1)   # not partial pivoting
+
+
 def test_sgetrf_singular_column_info(rng):
     A = rng.standard_normal((40, 40)).astype(np.float32)
     A[:, 17] = 0
