@@ -20,9 +20,14 @@
 //      only grows: block b stores b+1 after its x is written),
 //   3. reduces the partial sums into r = b_blk - sum and forms x = inv * r,
 //   4. writes its x and then bumps `ready`.
-// The matrix is read once (HBM-bound: n^2/2 elements per triangle); the
-// dependence chain is one publish, one x fetch and one small product per
-// block.  Using the block inverse instead of substitution inside the block
+// The matrix is read once (HBM-bound: n^2/2 elements per triangle).
+//
+// The last chunk is taken out of the dependence chain: with G_b = inv_b *
+// T_{b,b-1} (prepared with the inverses), a block computes
+// y = inv_b (b_blk - sum over chunks < b-1) while block b-1 is still being
+// solved, and only x = y - G_b x_{b-1} waits for block b-1.  The chain per
+// block is then one publish, one x fetch and one 128 x 128 product from
+// registers.  Using the block inverse instead of substitution inside the block
 // changes the rounding (not the result beyond it); the solve stays
 // deterministic.
 //
@@ -32,6 +37,8 @@
 // 2 = U^T forward, 3 = L^T (unit) backward; backward solves use logical index
 // i' = n-1-i, so the same code runs both directions with coalesced access.
 #include "oxi_internal.hpp"
+
+#include <cstdlib>
 
 namespace {
 
@@ -148,6 +155,45 @@ __global__ __launch_bounds__(NT) void trsv_dinv_kernel(const TM* __restrict__ M,
     }
 }
 
+// G_b = inv_b * T_{b,b-1} for b >= 1, G[q * RB + i] over logical rows i of
+// block b and physical columns q of block b-1 (the order the chain kernel
+// holds x_{b-1} in), as the chain kernel's last chunk would use them.  The
+// block T_{b,b-1} goes to LDS in the working type; the inverse is read from
+// memory.  Block 0 has no G.
+template <typename TM, typename TV, bool TRANS, bool BWD>
+__global__ __launch_bounds__(NT) void trsv_g_kernel(const TM* __restrict__ M, i64 ld, i64 n, const TV* __restrict__ inv,
+                                                    TV* __restrict__ G) {
+    __shared__ TV Tp[RB * (RB + 1)];   // Tp[q * (RB + 1) + pr]: physical row pr of block b, column q of block b-1
+    const int t = threadIdx.x;
+    const int b = blockIdx.x + 1;
+    const i64 lo = i64(b) * RB;
+    const int rows = int(min(i64(RB), n - lo));
+    const i64 p0 = BWD ? n - lo - rows : lo;
+    const i64 q0 = BWD ? n - i64(b) * RB : i64(b - 1) * RB;
+    for (int e = t; e < RB * RB; e += NT) {
+        const int pr = e % RB, q = e / RB;
+        TV v = TV(0);
+        if (pr < rows) v = TV(TRANS ? M[(q0 + q) + (p0 + pr) * ld] : M[(p0 + pr) + (q0 + q) * ld]);
+        Tp[q * (RB + 1) + pr] = v;
+    }
+    __syncthreads();
+    const TV* di = inv + size_t(b) * RB * RB;
+    TV* g = G + size_t(b) * RB * RB;
+    const int i = t % RB, qg = t / RB;
+    constexpr int QPT = RB / (NT / RB);
+    TV acc[QPT];
+#pragma unroll
+    for (int k = 0; k < QPT; ++k) acc[k] = TV(0);
+    for (int j = 0; j < rows; ++j) {
+        const TV d = di[j * RB + i];
+        const int pr = BWD ? rows - 1 - j : j;
+#pragma unroll
+        for (int k = 0; k < QPT; ++k) acc[k] = fma_t(d, Tp[(qg * QPT + k) * (RB + 1) + pr], acc[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < QPT; ++k) g[(qg * QPT + k) * RB + i] = acc[k];
+}
+
 // Two device-scope loads issued back to back and waited for once (relaxed
 // atomic loads written in C++ are issued one at a time, each waited for).
 __device__ __forceinline__ void load2_agent(const double* p0, const double* p1, double& a, double& b) {
@@ -161,7 +207,8 @@ __device__ __forceinline__ void load2_agent(const float* p0, const float* p1, fl
 
 template <typename TM, typename TV, bool TRANS, bool UNIT, bool BWD, bool VECOK>
 __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M, i64 ld, i64 n, TV* x,
-                                                        TrsvCtrl* ctrl, const TV* __restrict__ inv) {
+                                                        TrsvCtrl* ctrl, const TV* __restrict__ inv,
+                                                        const TV* __restrict__ G) {
     constexpr int VEC = VECOK ? Vec<TM>::N : 1;
     constexpr int LPC = RB / VEC;              // no-trans: lanes per column
     constexpr int CG = NT / LPC;               // no-trans: column groups
@@ -191,12 +238,24 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
         for (int e = t; e < RB * RB; e += NT) DI[e] = src[e];
     }
 
+    // with G: this thread's part of G_b (row i, columns part, part+4, ...),
+    // loaded before any wait
+    constexpr int GPT = RB / (NT / RB);
+    TV gr[GPT];
+    if (G && b > 0) {
+        const TV* g = G + size_t(b) * RB * RB;
+        const int i = t % RB, part = t / RB;
+#pragma unroll
+        for (int k = 0; k < GPT; ++k) gr[k] = g[(part + 4 * k) * RB + i];
+    }
+    const int cend = G ? b - 1 : b;            // chunks summed before the final step
+
     // 2. off-diagonal block row, chunk by chunk as earlier blocks publish
     TV acc[TRANS ? RPT : VEC];
 #pragma unroll
     for (int r = 0; r < (TRANS ? RPT : VEC); ++r) acc[r] = TV(0);
     int avail = 0;
-    for (int c = 0; c < b;) {
+    for (int c = 0; c < cend;) {
         if (c >= avail) {
             if (t == 0) {
                 unsigned v, spins = 0;
@@ -221,7 +280,7 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
         // x of up to XB published chunks into LDS with one round of
         // device-scope loads (two per thread, issued together);
         // xs[k * RB + e] = element e (physical order) of chunk c + k
-        const int nbat = min(min(avail, b) - c, XB);
+        const int nbat = min(min(avail, cend) - c, XB);
         {
             const int e0 = t, e1 = t + NT;
             const int k0 = e0 / RB, k1 = e1 / RB;
@@ -324,6 +383,55 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
         red[part][i] = a;
     }
     __syncthreads();
+    if (G && b > 0) {
+        // y = inverse * (b - sum over chunks < b-1) -> rv; then x_{b-1}
+        if (t < RB) {
+            TV yi = red[0][t];
+#pragma unroll
+            for (int g = 1; g < KP; ++g) yi += red[g][t];
+            rv[t] = yi;
+        }
+        if (t == 0) {
+            unsigned v, spins = 0;
+            while ((v = __hip_atomic_load(&ctrl->ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < unsigned(b)) {
+                if (__hip_atomic_load(&ctrl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > SPIN_LIMIT) {
+                    __hip_atomic_store(&ctrl->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    v = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            s_ready = int(v);
+        }
+        __syncthreads();
+        if (s_ready < b) return;               // timed out: uniform exit
+        const i64 q0 = BWD ? n - i64(b) * RB : i64(b - 1) * RB;
+        if (t < RB) xs[t] = get_agent(&x[q0 + t]);
+        __syncthreads();
+        {
+            const int i = t % RB, part = t / RB;
+            TV a = TV(0);
+#pragma unroll
+            for (int k = 0; k < GPT; ++k) a = fma_t(gr[k], xs[part + 4 * k], a);
+            red[part][i] = a;
+        }
+        __syncthreads();
+        if (t >= 64) return;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int i = t + 64 * h;
+            if (i < rows) {
+                TV s2 = red[0][i];
+#pragma unroll
+                for (int g = 1; g < KP; ++g) s2 += red[g][i];
+                put_agent(&x[p0 + (BWD ? rows - 1 - i : i)], rv[i] - s2);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);   // x of this block acknowledged before `ready` moves
+        if (t == 0) __hip_atomic_store(&ctrl->ready, unsigned(b + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
     if (t >= 64) return;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -341,18 +449,20 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
 }
 
 template <typename TM, typename TV, bool TRANS, bool UNIT, bool BWD>
-void launch_dir(hipStream_t s, i64 n, const TM* M, i64 ld, TV* x, TrsvCtrl* c, const TV* inv, bool vecok) {
+void launch_dir(hipStream_t s, i64 n, const TM* M, i64 ld, TV* x, TrsvCtrl* c, const TV* inv, const TV* G,
+                bool vecok) {
     const unsigned grid = unsigned(cdiv(n, RB));
     if (vecok)
         hipLaunchKernelGGL((trsv_chain_kernel<TM, TV, TRANS, UNIT, BWD, true>), dim3(grid), dim3(NT), 0, s, M, ld, n,
-                           x, c, inv);
+                           x, c, inv, G);
     else
         hipLaunchKernelGGL((trsv_chain_kernel<TM, TV, TRANS, UNIT, BWD, false>), dim3(grid), dim3(NT), 0, s, M, ld,
-                           n, x, c, inv);
+                           n, x, c, inv, G);
 }
 
 // Workspace: the control words, then the block inverses of the two triangles
-// of a solve (slot 0: the first triangle, slot 1: the second).
+// of a solve (slot 0: the first triangle, slot 1: the second), then their
+// G blocks (slots 2 and 3).
 size_t slot_bytes(i64 n, size_t tv) { return (size_t(cdiv(n, RB)) * RB * RB * tv + 255) & ~size_t(255); }
 template <typename TV>
 TV* slot_of(void* ws, i64 n, int slot) {
@@ -365,12 +475,23 @@ int prepare_t(oxi_handle_t h, hipStream_t s, int tk, i64 n, const TM* M, i64 ld,
     const unsigned grid = unsigned(cdiv(n, RB));
     TV* i0 = slot_of<TV>(ws, n, 0);
     TV* i1 = slot_of<TV>(ws, n, 1);
+    TV* g0 = slot_of<TV>(ws, n, 2);
+    TV* g1 = slot_of<TV>(ws, n, 3);
     if (!tk) {
         hipLaunchKernelGGL((trsv_dinv_kernel<TM, TV, false, true, false>), dim3(grid), dim3(NT), 0, s, M, ld, n, i0);
         hipLaunchKernelGGL((trsv_dinv_kernel<TM, TV, false, false, true>), dim3(grid), dim3(NT), 0, s, M, ld, n, i1);
     } else {
         hipLaunchKernelGGL((trsv_dinv_kernel<TM, TV, true, false, false>), dim3(grid), dim3(NT), 0, s, M, ld, n, i0);
         hipLaunchKernelGGL((trsv_dinv_kernel<TM, TV, true, true, true>), dim3(grid), dim3(NT), 0, s, M, ld, n, i1);
+    }
+    if (grid > 1) {
+        if (!tk) {
+            hipLaunchKernelGGL((trsv_g_kernel<TM, TV, false, false>), dim3(grid - 1), dim3(NT), 0, s, M, ld, n, i0, g0);
+            hipLaunchKernelGGL((trsv_g_kernel<TM, TV, false, true>), dim3(grid - 1), dim3(NT), 0, s, M, ld, n, i1, g1);
+        } else {
+            hipLaunchKernelGGL((trsv_g_kernel<TM, TV, true, false>), dim3(grid - 1), dim3(NT), 0, s, M, ld, n, i0, g0);
+            hipLaunchKernelGGL((trsv_g_kernel<TM, TV, true, true>), dim3(grid - 1), dim3(NT), 0, s, M, ld, n, i1, g1);
+        }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "trsv prepare launch: %s", hipGetErrorString(e));
@@ -384,13 +505,16 @@ int trsv_t(oxi_handle_t h, hipStream_t s, int which, i64 n, const TM* M, i64 ld,
     TrsvCtrl* c = static_cast<TrsvCtrl*>(ws);
     constexpr int VEC = Vec<TM>::N;
     const bool vecok = (reinterpret_cast<uintptr_t>(M) % 16 == 0) && (ld % VEC == 0) && (n % VEC == 0);
-    const TV* inv = slot_of<TV>(ws, n, (which == 0 || which == 2) ? 0 : 1);
+    const int slot = (which == 0 || which == 2) ? 0 : 1;
+    const TV* inv = slot_of<TV>(ws, n, slot);
+    static const bool plain = getenv("OXIBLAS_TRSV_PLAIN") != nullptr;   // tuning: every chunk on the chain
+    const TV* G = plain ? nullptr : slot_of<TV>(ws, n, 2 + slot);
     hipLaunchKernelGGL(trsv_ctrl_reset, dim3(1), dim3(64), 0, s, c);
     switch (which) {
-        case 0: launch_dir<TM, TV, false, true, false>(s, n, M, ld, x, c, inv, vecok); break;
-        case 1: launch_dir<TM, TV, false, false, true>(s, n, M, ld, x, c, inv, vecok); break;
-        case 2: launch_dir<TM, TV, true, false, false>(s, n, M, ld, x, c, inv, vecok); break;
-        case 3: launch_dir<TM, TV, true, true, true>(s, n, M, ld, x, c, inv, vecok); break;
+        case 0: launch_dir<TM, TV, false, true, false>(s, n, M, ld, x, c, inv, G, vecok); break;
+        case 1: launch_dir<TM, TV, false, false, true>(s, n, M, ld, x, c, inv, G, vecok); break;
+        case 2: launch_dir<TM, TV, true, false, false>(s, n, M, ld, x, c, inv, G, vecok); break;
+        case 3: launch_dir<TM, TV, true, true, true>(s, n, M, ld, x, c, inv, G, vecok); break;
         default: return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "trsv: bad triangle %d", which);
     }
     hipError_t e = hipGetLastError();
@@ -400,7 +524,7 @@ int trsv_t(oxi_handle_t h, hipStream_t s, int which, i64 n, const TM* M, i64 ld,
 
 }  // namespace
 
-size_t oxi_trsv_ws_bytes(i64 n, size_t tv) { return 256 + 2 * slot_bytes(std::max<i64>(n, 1), tv); }
+size_t oxi_trsv_ws_bytes(i64 n, size_t tv) { return 256 + 4 * slot_bytes(std::max<i64>(n, 1), tv); }
 
 int oxi_trsv_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const double* M, i64 ld, double*, void* ws) {
     return prepare_t<double, double>(h, s, tk, n, M, ld, ws);

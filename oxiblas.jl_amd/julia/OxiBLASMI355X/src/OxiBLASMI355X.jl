@@ -368,7 +368,9 @@ const _REASONS = (:converged, :iteration_limit, :nonfinite_iteration, :working_f
 Radix equilibration, a low-precision LU on the GPU (FP32 panel; trailing
 products in `lowtype`) and FP64 refinement until the componentwise backward
 error is <= rtol; `solver=:gmres` runs GMRES-IR, `:auto` switches to it when
-classical refinement stalls.  Returns the reference's NamedTuple.
+classical refinement stalls.  `pivot=:rbt` (random butterflies, no
+pivoting) and `pivot=:tournament` (tournament pivoting per 32-column leaf)
+have no reference counterpart.  Returns the reference's NamedTuple.
 """
 function mixed_solve(A::StridedMatrix{Float64}, b::StridedVecOrMat{Float64}; maxiter::Integer=10,
                      rtol::Real=1e-13, lowtype=Float32, solver::Symbol=:ir, gmres_restart::Integer=0,
@@ -385,7 +387,7 @@ function mixed_solve(A::StridedMatrix{Float64}, b::StridedVecOrMat{Float64}; max
                 Int32, Float64, Int32, Int32, Ref{IRInfo}),
                h, _prec(lowtype), Cint(solver === :ir ? 0 : solver === :gmres ? 1 : 2), n, k, Ad.ptr, Ad.ld, Bd.ptr,
                Bd.ld, Xd.ptr, Xd.ld, Int32(maxiter), Float64(rtol), Int32(gmres_restart),
-               Int32(pivot === :rbt ? 1 : 0), info)
+               Int32(pivot === :rbt ? 1 : pivot === :tournament ? 2 : 0), info)
     rc == 6 && throw(ArgumentError("finite mixed input required"))
     _check(h, rc, "mixed_solve")
     r = info[]
