@@ -81,7 +81,7 @@ def mixed_extra(ox, torch, n, kappa, t_fp64, seed=0x0B1A5):
     out = {}
     # (low type, solver, pivot); "tournament": the FP32 factor's leaves pick
     # their pivots by a tournament (one exchange per 32 columns)
-    plan = ((kappa, (("float32", "ir", "partial"), ("float32", "gmres", "partial"), ("float32", "ir", "tournament"),
+    plan = ((kappa, (("float32", "ir", "partial"), ("float32", "gmres", "partial"),
                      ("float32", "gmres", "tournament"), ("bfloat16", "gmres", "partial"),
                      ("float16", "gmres", "partial"), ("float8_e4m3fn", "gmres", "partial"))),
             (1e2, (("bfloat16", "gmres", "partial"), ("float16", "gmres", "partial"),
@@ -157,11 +157,24 @@ def cpu_baseline(n=8192, reps=3, n_openblas=16384):
         ob.append(time.perf_counter() - t0)
     to = sorted(ob)[len(ob) // 2]
     del A, lu
+    # the 512^3 GEMM with beta = 0 (the reference's own CPU-sized case): the
+    # oracle's packed gemm! on all host cores, median of 5 warm runs
+    Ag = rng.standard_normal((512, 512)); Bg = rng.standard_normal((512, 512))
+    Cg = np.asfortranarray(np.zeros((512, 512)))
+    O.dgemm(Cg, Ag, Bg, 1.0, 0.0, threads=cores)
+    gt = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        O.dgemm(Cg, Ag, Bg, 1.0, 0.0, threads=cores)
+        gt.append(time.perf_counter() - t0)
+    tg = sorted(gt)[2]
     return ({"value": round(fl(n) / t / 1e9, 2), "unit": "GFLOP/s", "cores": cores, "kind": "port",
              "sample": f"oracle lu_factor(blocksize=128)+solve, n={n}, 1 rhs, median of {reps} warm runs, "
                        f"{cores} threads (n=16384 takes ~100 s a run)"},
             {"value": round(fl(n_openblas) / to / 1e9, 2), "unit": "GFLOP/s", "threads": cores,
-             "sample": f"scipy-OpenBLAS dgetrf+dgetrs n={n_openblas}, median of {reps} warm runs"})
+             "sample": f"scipy-OpenBLAS dgetrf+dgetrs n={n_openblas}, median of {reps} warm runs",
+             "oracle_gemm_512_gflops": round(2.0 * 512 ** 3 / tg / 1e9, 2),
+             "oracle_gemm_512_sample": f"oracle gemm! 512^3, beta 0, {cores} threads, median of 5 warm runs"})
 
 
 def trailing_shapes(n=16384):
@@ -223,7 +236,15 @@ def run_part(ox, torch, part, n, reps=3):
     stderr: the command the kernel summaries under profiles/ come from."""
     dev = torch.device("cuda:0")
     g = torch.Generator(device=dev).manual_seed(0x0B1A5)
-    if part == "cholesky":
+    if part == "gemm":
+        # the LU's trailing-update shape on the 128 x 128 kernel: C (16384 x
+        # 16384) -= A (16384 x 256) B (256 x 16384), as getrf issues it
+        A = ox.DMatrix(torch.randn(n * 256, dtype=torch.float64, device=dev, generator=g), n, 256)
+        B = ox.DMatrix(torch.randn(256 * n, dtype=torch.float64, device=dev, generator=g), 256, n)
+        C = ox.DMatrix(torch.randn(n * n, dtype=torch.float64, device=dev, generator=g), n, n)
+        work = lambda: ox.gemm_(C, A, B, -1.0, 1.0)             # noqa: E731
+        reps = 10
+    elif part == "cholesky":
         S = ox.DMatrix(torch.randn(n * n, dtype=torch.float64, device=dev, generator=g), n, n)
         St = S.buf.view(n, n)
         St.copy_(St @ St.T)
@@ -272,7 +293,8 @@ def main():
     ap.add_argument("--traffic-summary", nargs=5, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV",
                                                            "COPY_WRITE_CSV", "OUT_JSON"),
                     help="write the HBM traffic per dgemm_m16_kernel launch from counter runs and exit")
-    ap.add_argument("--part", choices=("cholesky", "getrs64", "trsv", "mixed32", "mixed32tp", "lowp"), default=None,
+    ap.add_argument("--part", choices=("cholesky", "getrs64", "trsv", "mixed32", "mixed32tp", "lowp", "gemm"),
+                    default=None,
                     help="only run one extra part a few times (for rocprofv3 summaries of its kernels) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 times): the known-size kernel that scales "
