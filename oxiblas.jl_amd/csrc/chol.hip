@@ -32,61 +32,72 @@ template <> __device__ __forceinline__ bool pos_finite(float d) { return d > 0.0
 
 // A (n x n, n <= 64, lower triangle) in place; `off` is the global index of
 // its first column; *info receives off + c + 1 at the first failing column c.
-// 256 threads: thread i + 64 q takes row i and, in the update of column c,
-// the columns j = c + 1 + q, c + 5 + q, ... up to i.  Per column: the pivot
-// (read by every thread), the column scaled by its square root, a barrier,
-// the rank-1 update of the trailing lower triangle, a barrier.  The square
-// roots go to the diagonal at the end (the column step reads only the
-// diagonal element of its own column).
+// 256 threads: thread i + 64 q holds row i's elements of columns q, q + 4,
+// ..., in registers (a[k] = A(i, 4k + q)).  Per column c: the holders of
+// column c publish it to LDS, a barrier, every thread takes the pivot and its
+// square root, the holders scale the column (l_ic = a_ic / d) and publish the
+// multipliers, a barrier, and every thread updates its elements right of c
+// (a_ij = fma(-l_ic, l_jc, a_ij)) in registers.
 template <typename T>
 __global__ __launch_bounds__(256) void potrf_leaf_kernel(T* __restrict__ A, i64 lda, int n, i64 off,
                                                          i64* __restrict__ info) {
-    __shared__ T S[CLEAF * (CLEAF + 1)];   // S[j * 65 + i] = A(i, j)
-    __shared__ T R[CLEAF];                 // the square roots of the pivots
+    constexpr int KR = CLEAF / 4;          // registers per thread
+    __shared__ T raw[CLEAF];                // column c before scaling
+    __shared__ __align__(16) T lq[4][KR];   // multipliers l_jc at lq[j % 4][j / 4]
+    __shared__ T lrow[CLEAF];               // multipliers by row
     if (*info != 0) return;                 // an earlier leaf failed
     const int t = threadIdx.x, i = t & 63, q = t >> 6;
-    for (int j = q; j < n; j += 4)
-        if (i < n && i >= j) S[j * (CLEAF + 1) + i] = A[i + j * lda];
-    __syncthreads();
-    int done = n;
+    T a[KR];
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+        const int j = 4 * k + q;
+        a[k] = (i < n && j <= i) ? A[i + j * lda] : T(0);
+    }
     for (int c = 0; c < n; ++c) {
-        const T d = S[c * (CLEAF + 1) + c];
+        const int kc = c >> 2;
+        const bool holder = q == (c & 3);
+        if (holder) {
+            T v = a[0];
+#pragma unroll
+            for (int k = 1; k < KR; ++k) v = k == kc ? a[k] : v;
+            raw[i] = v;
+        }
+        __syncthreads();
+        const T d = raw[c];
         if (!pos_finite(d)) {                  // not finite and positive: stop at this column (uniform)
             if (t == 0) *info = off + c + 1;
-            done = c;
             break;
         }
         const T r = sqrt_t(d);
-        if (q == 0) {
-            if (i == c) R[c] = r;
-            else if (i > c && i < n) S[c * (CLEAF + 1) + i] = S[c * (CLEAF + 1) + i] / r;
+        if (holder) {
+            const T l = i == c ? r : raw[i] / r;
+            if (i >= c) {
+#pragma unroll
+                for (int k = 0; k < KR; ++k)
+                    if (k == kc) a[k] = l;
+            }
+            if (i > c && i < n) { lq[i & 3][i >> 2] = l; lrow[i] = l; }
         }
         __syncthreads();
         if (i > c && i < n) {
-            // all loads of the step before its stores (the element lists of
-            // the threads are disjoint, the column read lies left of them)
-            const T lic = S[c * (CLEAF + 1) + i];
-            T lj[CLEAF / 4], aij[CLEAF / 4];
+            const T lic = lrow[i];
+            T lj[KR];
 #pragma unroll
-            for (int k = 0; k < CLEAF / 4; ++k) {
-                const int j = 4 * k + q;
-                lj[k] = S[c * (CLEAF + 1) + j];
-                aij[k] = S[j * (CLEAF + 1) + i];
-            }
+            for (int k = 0; k < KR; ++k) lj[k] = lq[q][k];
 #pragma unroll
-            for (int k = 0; k < CLEAF / 4; ++k) {
+            for (int k = 0; k < KR; ++k) {
                 const int j = 4 * k + q;
-                if (j > c && j <= i) S[j * (CLEAF + 1) + i] = fma_t(-lic, lj[k], aij[k]);
+                if (j > c && j <= i) a[k] = fma_t(-lic, lj[k], a[k]);
             }
         }
-        __syncthreads();
     }
     // the columns before a failing one are final; the rest keep their
     // updated values (the reference's leaf returns with the same contents)
-    if (q == 0 && i < done) S[i * (CLEAF + 1) + i] = R[i];
-    __syncthreads();
-    for (int j = q; j < n; j += 4)
-        if (i < n && i >= j) A[i + j * lda] = S[j * (CLEAF + 1) + i];
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+        const int j = 4 * k + q;
+        if (i < n && j <= i) A[i + j * lda] = a[k];
+    }
 }
 
 // zero the strict upper triangle (the factor's storage contract)
@@ -138,22 +149,79 @@ int potrf_rec(oxi_handle_t h, hipStream_t s, i64 n, T* A, i64 lda, i64 off, i64*
     return potrf_rec(h, s, n - hh, A + hh + hh * lda, lda, off + hh, dinfo);
 }
 
-// Large matrices: the same elimination in blocks of POTRF_NB columns,
-// left to right (the right-looking blocked form of the recursion: each
-// block's diagonal part by the recursion above, its rows below by the
-// right-side TRSM, then one SYRK of the whole trailing matrix).  The
-// recursion's own splits put a TRSM of n/2 columns at the top, one chain of
+// Large matrices: the same elimination in blocks of POTRF_NB columns, left
+// to right (the right-looking blocked form of the recursion), with one block
+// of look-ahead.  A block column is factored (its diagonal block by the
+// recursion above, its rows below by the right-side TRSM) on the side
+// stream, while the main stream, kept off a few CUs as for getrf, applies the
+// previous block's SYRK to the rest of the trailing matrix; the next block
+// column's own update comes first, so that its factorization can start.  The
+// recursion's splits put a TRSM of n/2 columns at the top, one chain of
 // 64-column steps; here every TRSM has POTRF_NB columns and the O(n^3) work
-// is in SYRKs of rank POTRF_NB over the trailing matrix.
+// is in SYRKs of rank POTRF_NB.
 constexpr i64 POTRF_NB = 256;
 
+// A[c0:, c0:c0+w] -= L[c0:, j:j+jb] L[c0:c0+w, j:j+jb]^T: the lower triangle
+// of the diagonal block by SYRK, the rows below it by GEMM (FP64); the FP32
+// working copy takes one product (its upper triangle is never read).
+int col_update(oxi_handle_t h, hipStream_t s, i64 rows, i64 w, i64 jb, const double* L, double* Acol, i64 lda) {
+    OXI_TRY(oxi_launch_dsyrk(h, s, 0, 0, w, jb, -1.0, L, lda, 1.0, Acol, lda));
+    if (rows > w) OXI_TRY(oxi_launch_dgemm(h, s, 0, 1, rows - w, w, jb, -1.0, L + w, lda, L, lda, 1.0, Acol + w, lda));
+    return OXI_OK;
+}
+int col_update(oxi_handle_t h, hipStream_t s, i64 rows, i64 w, i64 jb, const float* L, float* Acol, i64 lda) {
+    return oxi_launch_sgemm(h, s, 0, 1, rows, w, jb, -1.0f, L, lda, L, lda, 1.0f, Acol, lda);
+}
+int rest_update(oxi_handle_t h, hipStream_t s, i64 rows, i64 jb, const double* L, double* A22, i64 lda) {
+    return oxi_launch_dsyrk(h, s, 0, 0, rows, jb, -1.0, L, lda, 1.0, A22, lda);
+}
+int rest_update(oxi_handle_t h, hipStream_t s, i64 rows, i64 jb, const float* L, float* A22, i64 lda) {
+    return oxi_launch_sgemm(h, s, 0, 1, rows, rows, jb, -1.0f, L, lda, L, lda, 1.0f, A22, lda);
+}
+int trsm_rt(oxi_handle_t h, hipStream_t s, i64 rows, i64 w, const double* L11, double* A21, i64 lda) {
+    return oxi_dtrsm_right(h, s, 0, 1, 0, rows, w, L11, lda, A21, lda);
+}
+int trsm_rt(oxi_handle_t h, hipStream_t s, i64 rows, i64 w, const float* L11, float* A21, i64 lda) {
+    return oxi_strsm_right(h, s, 0, 1, 0, rows, w, L11, lda, A21, lda);
+}
+
+// block column c0 .. c0+w-1 (rows c0 .. n-1): diagonal block, then the rows below
 template <typename T>
-int potrf_blocked(oxi_handle_t h, hipStream_t s, i64 n, T* A, i64 lda, i64* dinfo) {
-    for (i64 j = 0; j < n; j += POTRF_NB) {
+int panel_factor(oxi_handle_t h, hipStream_t s, i64 w, i64 rows, T* Acol, i64 lda, i64 off, i64* dinfo) {
+    OXI_TRY(potrf_rec<T>(h, s, w, Acol, lda, off, dinfo));
+    if (rows > w) OXI_TRY(trsm_rt(h, s, rows - w, w, Acol, Acol + w, lda));
+    return OXI_OK;
+}
+
+template <typename T>
+int potrf_blocked(oxi_handle_t h, hipStream_t s_caller, i64 n, T* A, i64 lda, i64* dinfo) {
+    const hipStream_t ms = h->bulk ? h->bulk : s_caller, ss = h->side;
+    if (ms != s_caller) {
+        OXI_HIP(h, hipEventRecord(h->ev_c, s_caller));
+        OXI_HIP(h, hipStreamWaitEvent(ms, h->ev_c, 0));
+    }
+    OXI_TRY(panel_factor<T>(h, ms, std::min(POTRF_NB, n), n, A, lda, 0, dinfo));
+    bool side_pending = false;
+    for (i64 j = 0, next = 0; j < n; j = next) {
         const i64 jb = std::min(POTRF_NB, n - j);
-        T* Ajj = A + j + j * lda;
-        OXI_TRY(potrf_rec<T>(h, s, jb, Ajj, lda, j, dinfo));
-        if (j + jb < n) OXI_TRY(chol_update(h, s, n - j - jb, jb, Ajj, Ajj + jb, Ajj + jb + jb * lda, lda));
+        next = j + jb;
+        if (side_pending) { OXI_HIP(h, hipStreamWaitEvent(ms, h->ev_b, 0)); side_pending = false; }
+        if (next >= n) break;
+        const i64 jn = std::min(POTRF_NB, n - next);
+        T* Ln = A + next + j * lda;                       // L[next:, j:j+jb]
+        OXI_TRY(col_update(h, ms, n - next, jn, jb, Ln, A + next + next * lda, lda));
+        OXI_HIP(h, hipEventRecord(h->ev_a, ms));
+        OXI_HIP(h, hipStreamWaitEvent(ss, h->ev_a, 0));
+        OXI_TRY(panel_factor<T>(h, ss, jn, n - next, A + next + next * lda, lda, next, dinfo));
+        OXI_HIP(h, hipEventRecord(h->ev_b, ss));
+        side_pending = true;
+        const i64 r0 = next + jn;
+        if (r0 < n) OXI_TRY(rest_update(h, ms, n - r0, jb, A + r0 + j * lda, A + r0 + r0 * lda, lda));
+    }
+    if (side_pending) OXI_HIP(h, hipStreamWaitEvent(ms, h->ev_b, 0));
+    if (ms != s_caller) {
+        OXI_HIP(h, hipEventRecord(h->ev_c, ms));
+        OXI_HIP(h, hipStreamWaitEvent(s_caller, h->ev_c, 0));
     }
     return OXI_OK;
 }
