@@ -589,54 +589,119 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     // interchanges of each panel on the columns to its left touch the L21
     // block that the update running beside the panel reads, so the main
     // stream applies them after that update, at the end of the step.
-    if (!panel_on_side) OXI_TRY(panel_top(g, 0, nb0));
-    for (i64 j0 = 0, jn = 0; j0 < q; j0 = jn) {
-        const i64 jb = std::min<i64>(nb_at(j0), q - j0);
-        const i64 j1 = j0 + jb;
-        jn = j1;
-        if (panel_on_side) OXI_HIP(h, hipStreamWaitEvent(s, h->ev_b, 0));
-        const bool next = j1 < q;                         // then j1 < m and j1 < n
-        const i64 j2 = next && lookahead ? std::min<i64>(j1 + nb_at(j1), q) : j1;
-        // this panel's interchanges on the look-ahead block first (the next
-        // panel waits for them), on the rest of the trailing columns after
-        // the next panel has been started
-        OXI_TRY(swap_w(g, j0, j1 - 1, j1, j2 > j1 ? j2 : n, 0));
-        if (j1 >= n) {
-            if (j0 > 0) OXI_TRY(swap_w(g, j0, j1 - 1, 0, j0, 0));
-            continue;
-        }
-        // look-ahead block: the next panel's columns [j1, j2)
-        if (j2 > j1) {
-            OXI_TRY(trsm_ru(h, s, j2 - j1, jb, g.Wm + j0 + j0 * ldw, ldw, g.Wm + j1 + j0 * ldw, ldw));
-            hipEvent_t pe = nullptr;
-            if (timed_update<T>(j2 - j1, m - j1)) OXI_TRY(oxi_prof_begin(h, s, &pe));
-            OXI_TRY(trailing_rows(g, j0, j1, j1, j2));
-            OXI_TRY(oxi_prof_end(h, s, pe, 2.0 * double(j2 - j1) * double(m - j1) * double(jb)));
-            OXI_HIP(h, hipEventRecord(h->ev_a, s));
+    //
+    // The look-ahead block itself goes to the side stream too, when no
+    // workspace is shared between the two streams' updates (FP64, and FP32
+    // with FP32 products; the low-precision update stages its operands in
+    // one buffer): the side stream takes panel j's interchanges, U12 block
+    // and update on the next panel's columns [j1, j2) and then factors them,
+    // while the main stream updates [j2, n), the columns of the panel after
+    // next [j2, j3) first.  Event ev_a marks that first product (the side
+    // stream's next look-ahead block reads its columns), ev_b the panel.
+    // The main stream then runs the large products back to back; per step
+    // it only waits for the panel.  The streams write disjoint columns; the
+    // side stream's TRSMs have fewer than 1024 rows and leave the handle's
+    // shared inverse buffer to the main stream.
+    static const bool main_la = getenv("OXIBLAS_MAIN_LOOKAHEAD") != nullptr;
+    const bool side_la = lookahead && !main_la && !(sizeof(T) == 4 && prec >= 0);
+    if (side_la) {
+        OXI_HIP(h, hipEventRecord(h->ev_a, s));   // the transpose: the side stream's first update reads it
+        if (!panel_on_side) {
             OXI_HIP(h, hipStreamWaitEvent(h->side, h->ev_a, 0));
-            OXI_TRY(panel_top(gs, j1, j2 - j1));
+            OXI_TRY(panel_top(gs, 0, nb0));
             OXI_HIP(h, hipEventRecord(h->ev_b, h->side));
             panel_on_side = true;
         }
-        // the rest of the trailing matrix: columns [j2, n)
-        if (j2 > j1) OXI_TRY(swap_w(g, j0, j1 - 1, j2, n, 0));
-        if (j2 < n) {
-            OXI_TRY(trsm_ru(h, s, n - j2, jb, g.Wm + j0 + j0 * ldw, ldw, g.Wm + j2 + j0 * ldw, ldw));
-            if (j1 < m) {
+        for (i64 j0 = 0, jn = 0; j0 < q; j0 = jn) {
+            const i64 jb = std::min<i64>(nb_at(j0), q - j0);
+            const i64 j1 = j0 + jb;
+            jn = j1;
+            const i64 j2 = j1 < q ? std::min<i64>(j1 + nb_at(j1), q) : j1;   // next panel [j1, j2)
+            const i64 j3 = j2 < q ? std::min<i64>(j2 + nb_at(j2), q) : j2;   // the one after [j2, j3)
+            OXI_HIP(h, hipStreamWaitEvent(s, h->ev_b, 0));                    // panel [j0, j1)
+            if (j2 > j1) {
+                // side: the next panel's columns, then the next panel (it
+                // waits for the main stream's update of [j1, j2) of the
+                // previous step, or the transpose)
+                OXI_HIP(h, hipStreamWaitEvent(h->side, h->ev_a, 0));
+                OXI_TRY(swap_w(gs, j0, j1 - 1, j1, j2, 0));
+                OXI_TRY(trsm_ru(h, gs.s, j2 - j1, jb, g.Wm + j0 + j0 * ldw, ldw, g.Wm + j1 + j0 * ldw, ldw));
                 hipEvent_t pe = nullptr;
-                if (timed_update<T>(n - j2, m - j1)) OXI_TRY(oxi_prof_begin(h, s, &pe));
-                OXI_TRY(trailing_rows(g, j0, j1, j2, n));
-                OXI_TRY(oxi_prof_end(h, s, pe, 2.0 * double(n - j2) * double(m - j1) * double(jb)));
+                if (timed_update<T>(j2 - j1, m - j1)) OXI_TRY(oxi_prof_begin(h, gs.s, &pe));
+                OXI_TRY(trailing_rows(gs, j0, j1, j1, j2));
+                OXI_TRY(oxi_prof_end(h, gs.s, pe, 2.0 * double(j2 - j1) * double(m - j1) * double(jb)));
+                OXI_TRY(panel_top(gs, j1, j2 - j1));
+                OXI_HIP(h, hipEventRecord(h->ev_b, h->side));
             }
+            // main: columns [j2, n)
+            if (j2 < n) {
+                OXI_TRY(swap_w(g, j0, j1 - 1, j2, n, 0));
+                OXI_TRY(trsm_ru(h, s, n - j2, jb, g.Wm + j0 + j0 * ldw, ldw, g.Wm + j2 + j0 * ldw, ldw));
+                if (j1 < m) {
+                    for (int part = 0; part < 2; ++part) {
+                        const i64 r0 = part == 0 ? j2 : j3, r1 = part == 0 ? j3 : n;
+                        if (r1 > r0) {
+                            hipEvent_t pe = nullptr;
+                            if (timed_update<T>(r1 - r0, m - j1)) OXI_TRY(oxi_prof_begin(h, s, &pe));
+                            OXI_TRY(trailing_rows(g, j0, j1, r0, r1));
+                            OXI_TRY(oxi_prof_end(h, s, pe, 2.0 * double(r1 - r0) * double(m - j1) * double(jb)));
+                        }
+                        if (part == 0) OXI_HIP(h, hipEventRecord(h->ev_a, s));
+                    }
+                }
+            }
+            if (j0 > 0) OXI_TRY(swap_w(g, j0, j1 - 1, 0, j0, 0));
         }
-        // this panel's interchanges on the columns to its left: the last
-        // reader of those columns was the previous step's trailing update,
-        // which ran before this point on this stream (per column the
-        // interchanges still arrive in panel order)
-        if (j0 > 0) OXI_TRY(swap_w(g, j0, j1 - 1, 0, j0, 0));
-        if (next && !lookahead) {
-            OXI_TRY(panel_top(g, j1, std::min<i64>(nb_at(j1), q - j1)));
-            panel_on_side = false;
+    } else {
+        if (!panel_on_side) OXI_TRY(panel_top(g, 0, nb0));
+        for (i64 j0 = 0, jn = 0; j0 < q; j0 = jn) {
+            const i64 jb = std::min<i64>(nb_at(j0), q - j0);
+            const i64 j1 = j0 + jb;
+            jn = j1;
+            if (panel_on_side) OXI_HIP(h, hipStreamWaitEvent(s, h->ev_b, 0));
+            const bool next = j1 < q;                         // then j1 < m and j1 < n
+            const i64 j2 = next && lookahead ? std::min<i64>(j1 + nb_at(j1), q) : j1;
+            // this panel's interchanges on the look-ahead block first (the next
+            // panel waits for them), on the rest of the trailing columns after
+            // the next panel has been started
+            OXI_TRY(swap_w(g, j0, j1 - 1, j1, j2 > j1 ? j2 : n, 0));
+            if (j1 >= n) {
+                if (j0 > 0) OXI_TRY(swap_w(g, j0, j1 - 1, 0, j0, 0));
+                continue;
+            }
+            // look-ahead block: the next panel's columns [j1, j2)
+            if (j2 > j1) {
+                OXI_TRY(trsm_ru(h, s, j2 - j1, jb, g.Wm + j0 + j0 * ldw, ldw, g.Wm + j1 + j0 * ldw, ldw));
+                hipEvent_t pe = nullptr;
+                if (timed_update<T>(j2 - j1, m - j1)) OXI_TRY(oxi_prof_begin(h, s, &pe));
+                OXI_TRY(trailing_rows(g, j0, j1, j1, j2));
+                OXI_TRY(oxi_prof_end(h, s, pe, 2.0 * double(j2 - j1) * double(m - j1) * double(jb)));
+                OXI_HIP(h, hipEventRecord(h->ev_a, s));
+                OXI_HIP(h, hipStreamWaitEvent(h->side, h->ev_a, 0));
+                OXI_TRY(panel_top(gs, j1, j2 - j1));
+                OXI_HIP(h, hipEventRecord(h->ev_b, h->side));
+                panel_on_side = true;
+            }
+            // the rest of the trailing matrix: columns [j2, n)
+            if (j2 > j1) OXI_TRY(swap_w(g, j0, j1 - 1, j2, n, 0));
+            if (j2 < n) {
+                OXI_TRY(trsm_ru(h, s, n - j2, jb, g.Wm + j0 + j0 * ldw, ldw, g.Wm + j2 + j0 * ldw, ldw));
+                if (j1 < m) {
+                    hipEvent_t pe = nullptr;
+                    if (timed_update<T>(n - j2, m - j1)) OXI_TRY(oxi_prof_begin(h, s, &pe));
+                    OXI_TRY(trailing_rows(g, j0, j1, j2, n));
+                    OXI_TRY(oxi_prof_end(h, s, pe, 2.0 * double(n - j2) * double(m - j1) * double(jb)));
+                }
+            }
+            // this panel's interchanges on the columns to its left: the last
+            // reader of those columns was the previous step's trailing update,
+            // which ran before this point on this stream (per column the
+            // interchanges still arrive in panel order)
+            if (j0 > 0) OXI_TRY(swap_w(g, j0, j1 - 1, 0, j0, 0));
+            if (next && !lookahead) {
+                OXI_TRY(panel_top(g, j1, std::min<i64>(nb_at(j1), q - j1)));
+                panel_on_side = false;
+            }
         }
     }
     if (panel_on_side) OXI_HIP(h, hipStreamWaitEvent(s, h->ev_b, 0));
