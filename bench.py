@@ -181,8 +181,10 @@ def trailing_shapes(n=16384):
     """(M, N, K) in the factorization's transposed coordinates of the trailing
     updates that run on the 128 x 128 kernel, in issue order: getrf's tuned
     block-size schedule (512 while more than 3/8 of the columns remain, then
-    256) and the GEMM's tile rule (128 x 128 tiles when that grid has at least
-    448 workgroups)."""
+    256), its three updates per panel [j0, j1) (the next panel's columns
+    [j1, j2) on the side stream, then [j2, j3) and [j3, n) on the main
+    stream) and the GEMM's tile rule (128 x 128 tiles when that grid has at
+    least 448 workgroups)."""
     cdiv = lambda a, b: (a + b - 1) // b
     wide_until = n - (3 * n) // 8
     nb_at = lambda j: 512 if j < wide_until else 256
@@ -192,7 +194,8 @@ def trailing_shapes(n=16384):
         if j1 >= n:
             break
         j2 = min(j1 + nb_at(j1), n)
-        for (M, N) in ((j2 - j1, n - j1), (n - j2, n - j1)):
+        j3 = min(j2 + nb_at(j2), n) if j2 < n else j2
+        for (M, N) in ((j2 - j1, n - j1), (j3 - j2, n - j1), (n - j3, n - j1)):
             if M > 0 and cdiv(M, 128) * cdiv(N, 128) >= 448:
                 out.append((M, N, j1 - j0))
         j0 = j1
@@ -210,8 +213,8 @@ def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, out_js
     def rows(path, counter, name):
         return [float(r["Counter_Value"]) for r in csv.DictReader(open(path))
                 if r["Counter_Name"] == counter and name in r["Kernel_Name"]]
-    f = rows(fetch_csv, "FETCH_SIZE", "dgemm_m16_kernel<false, false, false>")
-    w = rows(write_csv, "WRITE_SIZE", "dgemm_m16_kernel<false, false, false>")
+    f = rows(fetch_csv, "FETCH_SIZE", "dgemm_m16_kernel<false, false, false, 0>")
+    w = rows(write_csv, "WRITE_SIZE", "dgemm_m16_kernel<false, false, false, 0>")
     cf = rows(copy_fetch_csv, "FETCH_SIZE", "copyBuffer")
     cw = rows(copy_write_csv, "WRITE_SIZE", "copyBuffer")
     kf = (1 << 20) / (sum(cf) / len(cf))          # KiB per KiB the counter reports
