@@ -536,11 +536,16 @@ static void tp_gepp(std::vector<T>& V, const std::vector<i64>& id, int w, std::v
     }
 }
 
-// The tournament leaf on the mp x np block P (np <= 32): candidates by
-// partial pivoting inside each block of R rows (R = 256, 512 beyond 8192
-// rows), the leaf's pivots by partial pivoting over the candidates' original
-// rows, then every other row eliminated with the pivot rows; rows are stored
-// in the order the interchanges give, pivots as for the partial-pivoting leaf.
+// The tournament leaf on the mp x np block P (np <= 32).  The rows sit in
+// workgroups of R rows (R = 256, 512 beyond 8192 rows), each of four waves;
+// wave v of workgroup g holds rows g*R + s*256 + v*64 + l (s < R/256,
+// l < 64).  Candidates by partial pivoting inside each wave's rows, then by
+// partial pivoting over the union of a workgroup's wave candidates, then over
+// the union of the candidates of four consecutive workgroups (of four
+// consecutive groups, and so on) until one set is left; the leaf's pivots are
+// partial pivoting's over that set, the pivot rows as it leaves them, and
+// every other row is eliminated with them.  Rows are stored in the order the
+// interchanges give, pivots as for the partial-pivoting leaf.
 template <class T>
 void tp_leaf(T* P, i64 ldp, i64 mp, i64 np, i64 offset, LuState& st) {
     const int w = int(np);
@@ -548,19 +553,41 @@ void tp_leaf(T* P, i64 ldp, i64 mp, i64 np, i64 offset, LuState& st) {
     auto load = [&](i64 r, T* dst) {
         for (int c = 0; c < 32; ++c) dst[c] = c < w ? P[r + c * ldp] : T(0);
     };
-    std::vector<i64> cand, pv;
+    auto pivots_of = [&](const std::vector<i64>& rows, std::vector<i64>& pv, std::vector<T>* prow) {
+        std::vector<T> V(rows.size() * 32);
+        for (size_t k = 0; k < rows.size(); ++k) load(rows[k], &V[k * 32]);
+        tp_gepp(V, rows, w, pv, prow);
+    };
+    std::vector<std::vector<i64>> sets;
     for (i64 g0 = 0; g0 < mp; g0 += R) {
-        const i64 g1 = std::min(mp, g0 + R);
-        std::vector<T> V(size_t(g1 - g0) * 32);
-        std::vector<i64> id;
-        for (i64 r = g0; r < g1; ++r) { id.push_back(r); load(r, &V[size_t(r - g0) * 32]); }
-        tp_gepp(V, id, w, pv, (std::vector<T>*)nullptr);
-        cand.insert(cand.end(), pv.begin(), pv.end());
+        std::vector<i64> uni, pv;
+        for (int v = 0; v < 4; ++v) {
+            std::vector<i64> rows;
+            for (i64 s = 0; s < R / 256; ++s)
+                for (int l = 0; l < 64; ++l) {
+                    const i64 r = g0 + s * 256 + v * 64 + l;
+                    if (r < mp) rows.push_back(r);
+                }
+            if (rows.empty()) continue;
+            pivots_of(rows, pv, nullptr);
+            uni.insert(uni.end(), pv.begin(), pv.end());
+        }
+        pivots_of(uni, pv, nullptr);
+        sets.push_back(pv);
     }
-    std::vector<T> V(cand.size() * 32), prow;
-    for (size_t k = 0; k < cand.size(); ++k) load(cand[k], &V[k * 32]);
+    while (sets.size() > 1) {
+        std::vector<std::vector<i64>> next;
+        for (size_t g = 0; g < sets.size(); g += 4) {
+            std::vector<i64> uni, pv;
+            for (size_t i = g; i < std::min(sets.size(), g + 4); ++i) uni.insert(uni.end(), sets[i].begin(), sets[i].end());
+            pivots_of(uni, pv, nullptr);
+            next.push_back(pv);
+        }
+        sets.swap(next);
+    }
+    std::vector<T> prow;
     std::vector<i64> pid;
-    tp_gepp(V, cand, w, pid, &prow);
+    pivots_of(sets[0], pid, &prow);
     for (int c = 0; c < w; ++c)
         if (prow[size_t(c) * 32 + c] == T(0)) {
             if (st.info == 0) st.info = offset + c + 1;

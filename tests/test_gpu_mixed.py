@@ -153,12 +153,13 @@ def test_tournament_lu_matches_oracle(gpu, rng, n):
     Fh = F.factors.numpy()
     e32 = float(np.finfo(np.float32).eps)
     # the first leaf sees A itself, the same operations in both: its pivots
-    # (a tournament over up to 12 blocks of 256 rows here) agree exactly; later
-    # leaves see trailing updates rounded in different orders, and in Float32
-    # the top two candidates of a column lie closer than that rounding beyond
-    # about 1000 rows, as for the partial-pivoting factor
+    # (a tournament over the waves of up to 12 workgroups here) agree
+    # exactly; later leaves see trailing updates rounded in different orders,
+    # and every round of the tournament is a partial-pivoting search in which
+    # the top two candidates of a column can lie closer than that rounding
+    # (seen at n = 1000)
     assert np.array_equal(F.pivots[:32], ipo[:32])
-    if n <= 1000:
+    if n <= 300:
         assert np.array_equal(F.pivots, ipo)
         assert np.abs(Fh.astype(np.float64) - Fo).max() <= 4 * n * e32 * max(1.0, np.abs(Fo).max())
     L = np.tril(Fh.astype(np.float64), -1) + np.eye(n); U = np.triu(Fh.astype(np.float64))
@@ -169,6 +170,20 @@ def test_tournament_lu_matches_oracle(gpu, rng, n):
     Z = A.copy(); Z[:, min(n - 1, 40)] = 0
     Fz = gpu.lu_lowp(Z, "float32", pivot="tournament", check=False)
     assert Fz.info == O.sgetrf(Z, tournament=True)[2] == min(n - 1, 40) + 1
+
+
+@pytest.mark.parametrize("m", [40, 64, 257, 1000, 4097, 8192, 8193, 12345, 16384])
+def test_tournament_leaf_bitwise(gpu, rng, m):
+    # one leaf (m x 32): every round of the tournament -- the waves, the
+    # workgroup, one to three tree levels over up to 32 workgroups -- on the
+    # same data as the oracle, so the pivots and the factors agree bit for bit
+    A = rng.standard_normal((m, 32)).astype(np.float32)
+    A[rng.integers(0, m, 8), rng.integers(0, 32, 8)] *= 64.0      # some dominant entries
+    F = gpu.lu_lowp(A, "float32", pivot="tournament")
+    Fo, ipo, info = O.sgetrf(A, tournament=True)
+    assert F.info == info == 0
+    assert np.array_equal(F.pivots, ipo)
+    assert np.array_equal(F.factors.numpy(), Fo)
 
 
 def test_tournament_lu_n8192_and_solve(gpu):

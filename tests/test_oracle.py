@@ -344,10 +344,10 @@ def test_sgetrf_matches_lapack_sgetrf(rng, m, n, bs):
     assert np.abs(F.astype(np.float64) - lapack_lu).max() <= 64 * max(m, n) * EPS32 * np.abs(lapack_lu).max()
 
 
-@pytest.mark.parametrize("n", [5, 64, 200, 256])
-def test_sgetrf_tournament_one_block_is_partial_pivoting(rng, n):
-    # the tournament leaf of oxi_sgetrf_tournament: with one block of rows
-    # (at most 256) the candidates are the partial-pivoting pivots themselves
+@pytest.mark.parametrize("n", [5, 33, 64])
+def test_sgetrf_tournament_one_wave_is_partial_pivoting(rng, n):
+    # the tournament leaf of oxi_sgetrf_tournament: with the rows of one wave
+    # (at most 64) the candidates are the partial-pivoting pivots themselves
     # and the elimination is partial pivoting's, operation for operation, so
     # the factors are sgetrf's (pinned to LAPACK above) bit for bit
     A = rng.standard_normal((n, n)).astype(np.float32)
@@ -357,36 +357,70 @@ def test_sgetrf_tournament_one_block_is_partial_pivoting(rng, n):
     assert np.array_equal(p2, sl.lu_factor(A, check_finite=False)[1][:n] + 1)
 
 
-@pytest.mark.parametrize("n", [300, 1000])
+def lapack_pivot_rows(A, rows, w=32):
+    """The rows (positions, ascending) that LAPACK sgetrf picks, in order, as
+    the first min(w, len(rows)) pivots of A[rows, :w]: partial pivoting over
+    the set, ties to the first row in position order."""
+    rows = np.sort(np.asarray(rows))
+    _, bp = sl.lu_factor(A[rows, :w], check_finite=False)
+    order = np.arange(len(rows))
+    for j, q in enumerate(bp):
+        order[[j, q]] = order[[q, j]]
+    return rows[order[:min(w, len(rows))]]
+
+
+def tournament_pivot_rows(A, w=32):
+    """The first leaf's pivot rows of the tournament, with LAPACK for every
+    partial-pivoting step: each wave's rows (wave v of workgroup g: rows
+    g*R + s*256 + v*64 + l), then the union of a workgroup's wave candidates,
+    then unions of four consecutive sets until one is left."""
+    m = A.shape[0]
+    R = 512 if m > 8192 else 256
+    sets = []
+    for g0 in range(0, m, R):
+        cands = []
+        for v in range(4):
+            rows = [g0 + s * 256 + v * 64 + l for s in range(R // 256) for l in range(64)]
+            rows = [r for r in rows if r < m]
+            if rows:
+                cands.extend(lapack_pivot_rows(A, rows, w))
+        sets.append(lapack_pivot_rows(A, cands, w))
+    while len(sets) > 1:
+        sets = [lapack_pivot_rows(A, np.concatenate(sets[g:g + 4]), w) for g in range(0, len(sets), 4)]
+    return lapack_pivot_rows(A, sets[0], w)
+
+
+@pytest.mark.parametrize("n", [200, 1000, 2000])
 def test_sgetrf_tournament_first_leaf_and_reconstruction(rng, n):
-    # several blocks of 256 rows: the first leaf's pivot rows are LAPACK
-    # sgetrf's pivots over the union of the pivot rows LAPACK picks in each
-    # block (columns 0..31), and the whole factor reconstructs P*A to the
+    # several waves and workgroups: the first leaf's pivot rows are those of
+    # the same tournament run with LAPACK sgetrf for every partial-pivoting
+    # step (columns 0..31), and the whole factor reconstructs P*A to the
     # Float32 bound of test/lapack/lapack_dense.jl:11
     A = rng.standard_normal((n, n)).astype(np.float32)
     F, ipiv, info = O.sgetrf(A, tournament=True, threads=8)
     assert info == 0
     p = O.perm_from_ipiv(ipiv, n) - 1
     assert np.array_equal(np.sort(p), np.arange(n))
-    cand = []
-    for g0 in range(0, n, 256):
-        blk = A[g0:g0 + 256, :32]
-        _, bp = sl.lu_factor(blk, check_finite=False)
-        order = np.arange(blk.shape[0])
-        for j, q in enumerate(bp):
-            order[[j, q]] = order[[q, j]]
-        cand.extend(g0 + order[:min(32, blk.shape[0])])
-    cand = np.sort(np.array(cand))
-    _, cp = sl.lu_factor(A[cand, :32], check_finite=False)
-    order = np.arange(len(cand))
-    for j, q in enumerate(cp):
-        order[[j, q]] = order[[q, j]]
-    assert np.array_equal(p[:32], cand[order[:32]])
+    assert np.array_equal(p[:32], tournament_pivot_rows(A))
     L = np.tril(F.astype(np.float64), -1) + np.eye(n)
     U = np.triu(F.astype(np.float64))
     A64 = A.astype(np.float64)
     assert np.linalg.norm(L @ U - A64[p]) <= 30 * n * EPS32 * np.linalg.norm(A64)
     assert not np.array_equal(p, O.perm_from_ipiv(O.sgetrf(A)[1], n) - 1)   # not partial pivoting
+
+
+def test_sgetrf_tournament_tree_is_not_flat(rng):
+    # the tree matters: a flat tournament (partial pivoting over all the
+    # wave candidates at once) picks different first-leaf rows on this matrix,
+    # so a dropped tree level would fail the test above
+    n = 2000
+    A = rng.standard_normal((n, n)).astype(np.float32)
+    flat = []
+    for g0 in range(0, n, 256):
+        for v in range(4):
+            rows = [r for r in range(g0 + v * 64, g0 + v * 64 + 64) if r < n]
+            flat.extend(lapack_pivot_rows(A, rows))
+    assert not np.array_equal(lapack_pivot_rows(A, flat), tournament_pivot_rows(A))
 
 
 def test_sgetrf_singular_column_info(rng):
