@@ -32,72 +32,61 @@ template <> __device__ __forceinline__ bool pos_finite(float d) { return d > 0.0
 
 // A (n x n, n <= 64, lower triangle) in place; `off` is the global index of
 // its first column; *info receives off + c + 1 at the first failing column c.
-// 256 threads: thread i + 64 q holds row i's elements of columns q, q + 4,
-// ..., in registers (a[k] = A(i, 4k + q)).  Per column c: the holders of
-// column c publish it to LDS, a barrier, every thread takes the pivot and its
-// square root, the holders scale the column (l_ic = a_ic / d) and publish the
-// multipliers, a barrier, and every thread updates its elements right of c
-// (a_ij = fma(-l_ic, l_jc, a_ij)) in registers.
+// One wave: lane i holds row i (a[k] = A(i, k), k <= i) in registers, and
+// the column loop is unrolled, so every register index is a compile-time
+// constant and no step waits at a workgroup barrier.  Per column c: the
+// pivot from lane c (a readlane), its square root, every lane's multiplier
+// l_ic = a_ic / r into LDS, then each lane updates its row right of c
+// (a_ik = fma(-l_ic, l_kc, a_ik)) with the multipliers read back.  Elements
+// right of the diagonal take the same operations on values nobody stores.
+// A failing pivot (not finite and positive) stops the loop: the columns
+// before it are final, the rest keep their updated values.
 template <typename T>
-__global__ __launch_bounds__(256) void potrf_leaf_kernel(T* __restrict__ A, i64 lda, int n, i64 off,
-                                                         i64* __restrict__ info) {
-    constexpr int KR = CLEAF / 4;          // registers per thread
-    __shared__ T raw[CLEAF];                // column c before scaling
-    __shared__ __align__(16) T lq[4][KR];   // multipliers l_jc at lq[j % 4][j / 4]
-    __shared__ T lrow[CLEAF];               // multipliers by row
-    if (*info != 0) return;                 // an earlier leaf failed
-    const int t = threadIdx.x, i = t & 63, q = t >> 6;
-    T a[KR];
+__device__ __forceinline__ T readlane_t(T x, int l);
+template <>
+__device__ __forceinline__ double readlane_t(double x, int l) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = __builtin_amdgcn_readlane(unsigned(u), l), hi = __builtin_amdgcn_readlane(unsigned(u >> 32), l);
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+template <>
+__device__ __forceinline__ float readlane_t(float x, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void potrf_leaf_kernel(T* __restrict__ A, i64 lda, int n, i64 off,
+                                                        i64* __restrict__ info) {
+    __shared__ __align__(16) T lcol[CLEAF];   // the multipliers of the current column
+    if (*info != 0) return;                   // an earlier leaf failed
+    const int i = threadIdx.x;
+    T a[CLEAF];
 #pragma unroll
-    for (int k = 0; k < KR; ++k) {
-        const int j = 4 * k + q;
-        a[k] = (i < n && j <= i) ? A[i + j * lda] : T(0);
-    }
-    for (int c = 0; c < n; ++c) {
-        const int kc = c >> 2;
-        const bool holder = q == (c & 3);
-        if (holder) {
-            T v = a[0];
+    for (int k = 0; k < CLEAF; ++k) a[k] = (i < n && k <= i) ? A[i + k * lda] : T(0);
+    bool go = true;   // uniform
 #pragma unroll
-            for (int k = 1; k < KR; ++k) v = k == kc ? a[k] : v;
-            raw[i] = v;
-        }
-        __syncthreads();
-        const T d = raw[c];
-        if (!pos_finite(d)) {                  // not finite and positive: stop at this column (uniform)
-            if (t == 0) *info = off + c + 1;
-            break;
-        }
-        const T r = sqrt_t(d);
-        if (holder) {
-            const T l = i == c ? r : raw[i] / r;
-            if (i >= c) {
+    for (int c = 0; c < CLEAF; ++c) {
+        if (go && c < n) {
+            const T d = readlane_t(a[c], c);
+            if (!pos_finite(d)) {
+                if (i == 0) *info = off + c + 1;
+                go = false;
+            } else {
+                const T r = sqrt_t(d);
+                const T l = i > c ? a[c] / r : T(0);
+                a[c] = i == c ? r : i > c ? l : a[c];
+                lcol[i] = l;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");   // the lanes' LDS writes before the reads below
+                __builtin_amdgcn_wave_barrier();
 #pragma unroll
-                for (int k = 0; k < KR; ++k)
-                    if (k == kc) a[k] = l;
-            }
-            if (i > c && i < n) { lq[i & 3][i >> 2] = l; lrow[i] = l; }
-        }
-        __syncthreads();
-        if (i > c && i < n) {
-            const T lic = lrow[i];
-            T lj[KR];
-#pragma unroll
-            for (int k = 0; k < KR; ++k) lj[k] = lq[q][k];
-#pragma unroll
-            for (int k = 0; k < KR; ++k) {
-                const int j = 4 * k + q;
-                if (j > c && j <= i) a[k] = fma_t(-lic, lj[k], a[k]);
+                for (int k = c + 1; k < CLEAF; ++k) a[k] = fma_t(-l, lcol[k], a[k]);
+                __builtin_amdgcn_wave_barrier();   // every read of this column before the next column's writes
             }
         }
     }
-    // the columns before a failing one are final; the rest keep their
-    // updated values (the reference's leaf returns with the same contents)
 #pragma unroll
-    for (int k = 0; k < KR; ++k) {
-        const int j = 4 * k + q;
-        if (i < n && j <= i) A[i + j * lda] = a[k];
-    }
+    for (int k = 0; k < CLEAF; ++k)
+        if (i < n && k <= i) A[i + k * lda] = a[k];
 }
 
 // zero the strict upper triangle (the factor's storage contract)
@@ -139,7 +128,7 @@ template <typename T>
 int potrf_rec(oxi_handle_t h, hipStream_t s, i64 n, T* A, i64 lda, i64 off, i64* dinfo) {
     if (n <= 0) return OXI_OK;
     if (n <= CLEAF) {
-        hipLaunchKernelGGL(potrf_leaf_kernel<T>, dim3(1), dim3(256), 0, s, A, lda, int(n), off, dinfo);
+        hipLaunchKernelGGL(potrf_leaf_kernel<T>, dim3(1), dim3(64), 0, s, A, lda, int(n), off, dinfo);
         return launch_check(h, "potrf leaf");
     }
     i64 hh = n / 2;
