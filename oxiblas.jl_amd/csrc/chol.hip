@@ -189,14 +189,20 @@ int potrf_blocked(oxi_handle_t h, hipStream_t s_caller, i64 n, T* A, i64 lda, i6
         OXI_HIP(h, hipEventRecord(h->ev_c, s_caller));
         OXI_HIP(h, hipStreamWaitEvent(ms, h->ev_c, 0));
     }
-    OXI_TRY(panel_factor<T>(h, ms, std::min(POTRF_NB, n), n, A, lda, 0, dinfo));
+    // 512-column blocks over the first half of large matrices (rank-512
+    // SYRKs run faster while the trailing matrix is large), POTRF_NB after.
+    // Measured at n = 16384: 256 throughout 43.7 ms, 384 throughout 42.5,
+    // 512 throughout 42.7, this schedule 41.6-41.9 (512 then 384: 41.7-42.3).
+    const i64 nb_wide = n >= 8192 ? 2 * POTRF_NB : POTRF_NB;
+    auto nb_at = [&](i64 j) { return std::min<i64>(2 * j < n ? nb_wide : POTRF_NB, n - j); };
+    OXI_TRY(panel_factor<T>(h, ms, nb_at(0), n, A, lda, 0, dinfo));
     bool side_pending = false;
     for (i64 j = 0, next = 0; j < n; j = next) {
-        const i64 jb = std::min(POTRF_NB, n - j);
+        const i64 jb = nb_at(j);
         next = j + jb;
         if (side_pending) { OXI_HIP(h, hipStreamWaitEvent(ms, h->ev_b, 0)); side_pending = false; }
         if (next >= n) break;
-        const i64 jn = std::min(POTRF_NB, n - next);
+        const i64 jn = nb_at(next);
         T* Ln = A + next + j * lda;                       // L[next:, j:j+jb]
         OXI_TRY(col_update(h, ms, n - next, jn, jb, Ln, A + next + next * lda, lda));
         OXI_HIP(h, hipEventRecord(h->ev_a, ms));
