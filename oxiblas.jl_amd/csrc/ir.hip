@@ -885,7 +885,6 @@ extern "C" int oxi_dgesv_mixed(oxi_handle_t h, int prec, int solver, int64_t n, 
     bool xfinite = true, used_gmres = false;
     i64 iterations = 0, inner = 0;
     int mode = solver == 1 ? 1 : 0;
-    const double tau = 1e-8;
     for (int iteration = 0; iteration <= maxiter; ++iteration) {
         OXI_TRY(residual(c, &err, &nerr, &xfinite));
         iterations = iteration;
@@ -896,6 +895,10 @@ extern "C" int oxi_dgesv_mixed(oxi_handle_t h, int prec, int solver, int64_t n, 
             OXI_TRY(correction(c, c.R, n, false));
         } else {
             used_gmres = true;
+            // a GMRES cycle reduces the preconditioned residual by tau: by
+            // 1e-8, or by a hundredth of what the backward error still has
+            // to lose when that asks for less (at most 1e-2)
+            const double tau = std::max(1e-8, std::min(1e-2, 0.01 * rtol / err));
             for (i64 j = 0; j < nrhs; ++j) OXI_TRY(gmres_correction(c, j, tau, &inner));
         }
     }
