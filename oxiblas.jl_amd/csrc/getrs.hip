@@ -54,17 +54,36 @@ int trsm_left(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64
     return oxi_strsm_left(h, s, upper, trans, unit, m, nrhs, A, lda, B, ldb);
 }
 
+int chain_triangle(oxi_handle_t h, hipStream_t s, int upper, i64 n, i64 nrhs, const double* LU, i64 lda, double* B,
+                   i64 ldb, void* mws) {
+    return oxi_trsm_chain(h, s, upper, n, nrhs, LU, lda, B, ldb, mws);
+}
+int chain_triangle(oxi_handle_t h, hipStream_t, int, i64, i64, const float*, i64, float*, i64, void*) {
+    return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "trsm chain: FP64 only");
+}
+
+// Whether a solve takes the chained multi-RHS triangles (their workspace is
+// then prepared for the factors).
+bool use_chain(int tk, i64 nrhs) { return !tk && nrhs > OXI_TRSV_MAX_RHS && nrhs <= OXI_CHAIN_MAX_RHS; }
+
 // Two triangles of one solve: per column with the single-RHS chain kernel
 // when there are few right-hand sides (the reference's vector branch uses
-// trsv!, src/api/linalg_api.jl:168-184), else the blocked trsm.
+// trsv!, src/api/linalg_api.jl:168-184), with the multi-RHS chain up to
+// OXI_CHAIN_MAX_RHS (no transpose, FP64; mws its prepared workspace), else
+// the blocked trsm.
 template <typename T>
 int triangles(oxi_handle_t h, hipStream_t s, int tk, i64 n, i64 nrhs, const T* LU, i64 lda, T* B, i64 ldb,
-              void* ctrl) {
+              void* ctrl, void* mws) {
     if (ctrl && nrhs <= OXI_TRSV_MAX_RHS) {
         for (i64 j = 0; j < nrhs; ++j) {
             OXI_TRY(oxi_trsv(h, s, tk ? 2 : 0, n, LU, lda, B + j * ldb, ctrl));
             OXI_TRY(oxi_trsv(h, s, tk ? 3 : 1, n, LU, lda, B + j * ldb, ctrl));
         }
+        return OXI_OK;
+    }
+    if (mws && use_chain(tk, nrhs)) {
+        OXI_TRY(chain_triangle(h, s, 0, n, nrhs, LU, lda, B, ldb, mws));
+        OXI_TRY(chain_triangle(h, s, 1, n, nrhs, LU, lda, B, ldb, mws));
         return OXI_OK;
     }
     if (!tk) {
@@ -81,17 +100,17 @@ int triangles(oxi_handle_t h, hipStream_t s, int tk, i64 n, i64 nrhs, const T* L
 // no synchronisation.
 template <typename T>
 int getrs_perm(oxi_handle_t h, hipStream_t s, int tk, i64 n, i64 nrhs, const T* LU, i64 lda, const i64* dperm, T* B,
-               i64 ldb, T* Y, void* ctrl) {
+               i64 ldb, T* Y, void* ctrl, void* mws = nullptr) {
     const unsigned pg = unsigned(std::min<i64>(cdiv(n * nrhs, 256), 8192));
     if (!tk) {
         // B <- B[p, :] ; L \ ; U \   (src/api/linalg_api.jl:168-174)
         hipLaunchKernelGGL(permute_rows_kernel<T>, dim3(pg), dim3(256), 0, s, B, ldb, Y, n, n, nrhs, dperm, 0);
         hipLaunchKernelGGL(copy_cols_kernel<T>, dim3(pg), dim3(256), 0, s, Y, n, B, ldb, n, nrhs);
         OXI_TRY(launch_check(h, "permute"));
-        OXI_TRY(triangles<T>(h, s, 0, n, nrhs, LU, lda, B, ldb, ctrl));
+        OXI_TRY(triangles<T>(h, s, 0, n, nrhs, LU, lda, B, ldb, ctrl, mws));
     } else {
         // U^T \ ; L^T \ ; inverse interchanges   (src/api/linalg_api.jl:176-184)
-        OXI_TRY(triangles<T>(h, s, 1, n, nrhs, LU, lda, B, ldb, ctrl));
+        OXI_TRY(triangles<T>(h, s, 1, n, nrhs, LU, lda, B, ldb, ctrl, mws));
         hipLaunchKernelGGL(permute_rows_kernel<T>, dim3(pg), dim3(256), 0, s, B, ldb, Y, n, n, nrhs, dperm, 1);
         hipLaunchKernelGGL(copy_cols_kernel<T>, dim3(pg), dim3(256), 0, s, Y, n, B, ldb, n, nrhs);
         OXI_TRY(launch_check(h, "permute"));
@@ -139,7 +158,9 @@ int getrs_api(oxi_handle_t h, char trans, i64 n, i64 nrhs, const T* LU, i64 lda,
     void* ws;
     const size_t xpad = (xbytes + 255) & ~size_t(255), ppad = (size_t(n) * 8 + 255) & ~size_t(255);
     const bool chain = nrhs <= OXI_TRSV_MAX_RHS;
-    OXI_TRY(oxi_workspace(h, xpad + ppad + (chain ? oxi_trsv_ws_bytes(n, sizeof(T)) : 256), &ws));
+    const bool mchain = sizeof(T) == 8 && use_chain(tk, nrhs);
+    OXI_TRY(oxi_workspace(h, xpad + ppad + (chain ? oxi_trsv_ws_bytes(n, sizeof(T)) : mchain ? oxi_trsm_chain_ws_bytes(n) : 256),
+                          &ws));
     T* Y = (T*)ws;
     i64* dperm = (i64*)((char*)ws + xpad);
     void* ctrl = (char*)ws + xpad + ppad;
@@ -148,8 +169,13 @@ int getrs_api(oxi_handle_t h, char trans, i64 n, i64 nrhs, const T* LU, i64 lda,
         OXI_TRY(oxi_trsv_begin(h, s, ctrl));
         OXI_TRY(oxi_trsv_prepare(h, s, tk, n, LU, lda, (T*)nullptr, ctrl));
     }
-    OXI_TRY(getrs_perm<T>(h, s, tk, n, nrhs, LU, lda, dperm, B, ldb, Y, ctrl));
-    if (chain) return oxi_trsv_check(h, s, ctrl);   // the chain kernel ran: its status
+    if (mchain) {
+        OXI_TRY(oxi_trsm_chain_begin(h, s, ctrl));
+        OXI_TRY(oxi_trsm_chain_prepare(h, s, n, (const double*)LU, lda, ctrl));
+    }
+    OXI_TRY(getrs_perm<T>(h, s, tk, n, nrhs, LU, lda, dperm, B, ldb, Y, chain ? ctrl : nullptr, mchain ? ctrl : nullptr));
+    if (chain) return oxi_trsv_check(h, s, ctrl);    // the chain kernel ran: its status
+    if (mchain) return oxi_trsm_chain_check(h, s, ctrl);
     OXI_HIP(h, hipStreamSynchronize(s));
     return OXI_OK;
 }
@@ -205,6 +231,7 @@ struct oxi_getrs_plan_s {
     i64* dperm = nullptr;
     void* ctrl[2] = {nullptr, nullptr};
     bool prepared[2] = {false, false};   // the inverses of kind 'N' (0) / 'T' (1), made on first use
+    void* mws = nullptr;                 // the multi-RHS chain's workspace, made and prepared on first use
     i64 zero_pivot = 0;       // first zero on U's diagonal (1-based), 0 if none
 };
 
@@ -274,8 +301,25 @@ extern "C" int oxi_dgetrs_plan(oxi_handle_t h, oxi_getrs_plan_t plan, char trans
             plan->prepared[tk] = true;
         }
     }
-    OXI_TRY(getrs_perm<double>(h, s, tk, n, nrhs, plan->LU, plan->lda, plan->dperm, B, ldb, (double*)ws, ctrl));
+    const bool mchain = use_chain(tk, nrhs);
+    if (mchain) {
+        if (!plan->mws) {
+            void* m = nullptr;
+            if (hipMalloc(&m, oxi_trsm_chain_ws_bytes(n)) != hipSuccess)
+                return oxi_set_error(h, OXI_ERR_ALLOC, "getrs plan: allocation failed");
+            if (oxi_trsm_chain_prepare(h, s, n, plan->LU, plan->lda, m) != OXI_OK) {
+                (void)hipStreamSynchronize(s);
+                (void)hipFree(m);
+                return OXI_ERR_HIP;
+            }
+            plan->mws = m;
+        }
+        OXI_TRY(oxi_trsm_chain_begin(h, s, plan->mws));
+    }
+    OXI_TRY(getrs_perm<double>(h, s, tk, n, nrhs, plan->LU, plan->lda, plan->dperm, B, ldb, (double*)ws, ctrl,
+                               mchain ? plan->mws : nullptr));
     if (chain) return oxi_trsv_check(h, s, ctrl);
+    if (mchain) return oxi_trsm_chain_check(h, s, plan->mws);
     OXI_HIP(h, hipStreamSynchronize(s));
     return OXI_OK;
 }
@@ -288,6 +332,7 @@ extern "C" int oxi_dgetrs_plan_destroy(oxi_handle_t h, oxi_getrs_plan_t plan) {
     (void)hipStreamSynchronize(h->stream);
     if (plan->dperm) (void)hipFree(plan->dperm);
     for (auto c : plan->ctrl) if (c) (void)hipFree(c);
+    if (plan->mws) (void)hipFree(plan->mws);
     delete plan;
     return OXI_OK;
 }

@@ -301,6 +301,37 @@ def test_64_rhs_solves_n4096(gpu, rng):
         assert np.linalg.norm(opA @ X - B) <= 30 * n * EPS * np.linalg.norm(A) * np.linalg.norm(X)
 
 
+@pytest.mark.parametrize("n", (1, 63, 64, 65, 200, 1000))
+def test_chained_multi_rhs_solves(gpu, rng, n):
+    """The chained multi-RHS solve (5 to 64 right-hand sides, no transpose):
+    a partial last block of rows, partial groups of right-hand sides and a
+    padded leading dimension, through the solve plan and through oxi_dgetrs,
+    against the oracle's dgetrs; the rows outside B's view stay untouched."""
+    from oxiblas_mi355x import _lib
+    ox = gpu
+    A = rng.standard_normal((n, n))
+    F = ox.lu(A)
+    Fo, ipo = F.factors.numpy(), F.ipiv
+    c = np.linalg.cond(A)
+    h = _lib.handle()
+    for nrhs in (5, 17, 32, 33, 64):
+        B = rng.standard_normal((n, nrhs))
+        Xo = O.dgetrs(Fo, ipo, B, trans="N")
+        tol = 64 * n * EPS * c * np.abs(Xo).max()
+        X = F.solve(B)                                        # the solve plan
+        np.testing.assert_allclose(X, Xo, rtol=0, atol=tol)
+        assert bwd(A @ X - B, A) <= 30 * c
+        pad = 7                                               # oxi_dgetrs on a view with ld = n + 7
+        Big = np.asfortranarray(rng.standard_normal((n + pad, nrhs)))
+        Big[:n] = B
+        Bd = ox.DMatrix.from_numpy(Big)
+        rc = h._L.oxi_dgetrs(h.ptr, b"N", n, nrhs, F.factors.ptr, F.factors.ld, F.ipiv_dev.data_ptr(), Bd.ptr, Bd.ld)
+        h.check(rc, "getrs")
+        out = Bd.numpy()
+        np.testing.assert_allclose(out[:n], Xo, rtol=0, atol=tol)
+        assert np.array_equal(out[n:], Big[n:])
+
+
 def test_pivots_equal_lapack_n16384(gpu):
     """n = 16384 (the benchmark's size): the pivot sequence equals LAPACK
     dgetrf's on the same matrix; the near-tie gap of every step is measured
