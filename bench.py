@@ -380,14 +380,19 @@ def main():
             log(f"gemm {m_}x{n_}x{k_}: {gms:.3f} ms {tf:.1f} TFLOP/s")
         F, _ = step()
         B64 = ox.DMatrix(torch.randn(n * 64, dtype=torch.float64, device=dev, generator=g), n, 64)
+        B64b = B64.copy()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        F.ldiv_(B64)
+        F.ldiv_(B64)                 # the first solve makes the factors' solve plan
+        torch.cuda.synchronize()
+        extra["getrs_64rhs_first_ms"] = round((time.perf_counter() - t1) * 1e3, 3)
+        t1 = time.perf_counter()
+        F.ldiv_(B64b)
         torch.cuda.synchronize()
         ts = time.perf_counter() - t1
         extra["getrs_64rhs_ms"] = round(ts * 1e3, 3)
         extra["getrs_64rhs_gflops"] = round(2.0 * n * n * 64 / ts / 1e9, 1)
-        log(f"getrs 64 rhs n={n}: {ts * 1e3:.3f} ms")
+        log(f"getrs 64 rhs n={n}: {ts * 1e3:.3f} ms (first solve, with the plan: {extra['getrs_64rhs_first_ms']} ms)")
         # a second solve with the same factors reuses their solve plan
         b1 = ox.DMatrix(b0.clone(), n, 1)
         torch.cuda.synchronize()
@@ -395,7 +400,25 @@ def main():
         F.ldiv_(b1)
         torch.cuda.synchronize()
         extra["getrs_1rhs_repeat_ms"] = round((time.perf_counter() - t1) * 1e3, 3)
-        del F, B64, b1
+        del F, B64, B64b, b1
+        # the pullback of X = A \ B at n = 8192 with 64 right-hand sides
+        # (A = N(0,1) + n I): Bbar by the transposed solve on the factors,
+        # Abar = -Bbar X^T
+        na = 8192
+        Aa = ox.DMatrix(torch.randn(na * na, dtype=torch.float64, device=dev, generator=g), na, na)
+        Aa.buf.view(na, na).diagonal().add_(float(na))
+        Fa = ox.lu(Aa)
+        Ba = ox.DMatrix(torch.randn(na * 64, dtype=torch.float64, device=dev, generator=g), na, 64)
+        Xa = Fa.solve(Ba)
+        Xbar = ox.DMatrix(torch.randn(na * 64, dtype=torch.float64, device=dev, generator=g), na, 64)
+        ox.ldiv_pullback(Fa, Xa, Xbar)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ox.ldiv_pullback(Fa, Xa, Xbar)
+        torch.cuda.synchronize()
+        extra["adjoint_ldiv_n8192_64rhs_ms"] = round((time.perf_counter() - t1) * 1e3, 3)
+        log(f"adjoint of A\\B n={na}, 64 rhs: {extra['adjoint_ldiv_n8192_64rhs_ms']} ms")
+        del Aa, Fa, Ba, Xa, Xbar
         # rocSOLVER's getrf through torch on the same box, for comparison (labelled)
         Ac = A0.buf.view(n, n).clone()
         torch.linalg.lu_factor(Ac)

@@ -54,23 +54,23 @@ int trsm_left(oxi_handle_t h, hipStream_t s, int upper, int trans, int unit, i64
     return oxi_strsm_left(h, s, upper, trans, unit, m, nrhs, A, lda, B, ldb);
 }
 
-int chain_triangle(oxi_handle_t h, hipStream_t s, int upper, i64 n, i64 nrhs, const double* LU, i64 lda, double* B,
-                   i64 ldb, void* mws) {
-    return oxi_trsm_chain(h, s, upper, n, nrhs, LU, lda, B, ldb, mws);
+int chain_triangle(oxi_handle_t h, hipStream_t s, int tk, int second, i64 n, i64 nrhs, const double* LU, i64 lda,
+                   double* B, i64 ldb, void* mws) {
+    return oxi_trsm_chain(h, s, tk, second, n, nrhs, LU, lda, B, ldb, mws);
 }
-int chain_triangle(oxi_handle_t h, hipStream_t, int, i64, i64, const float*, i64, float*, i64, void*) {
+int chain_triangle(oxi_handle_t h, hipStream_t, int, int, i64, i64, const float*, i64, float*, i64, void*) {
     return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "trsm chain: FP64 only");
 }
 
 // Whether a solve takes the chained multi-RHS triangles (their workspace is
 // then prepared for the factors).
-bool use_chain(int tk, i64 nrhs) { return !tk && nrhs > OXI_TRSV_MAX_RHS && nrhs <= OXI_CHAIN_MAX_RHS; }
+bool use_chain(i64 nrhs) { return nrhs > OXI_TRSV_MAX_RHS && nrhs <= OXI_CHAIN_MAX_RHS; }
 
 // Two triangles of one solve: per column with the single-RHS chain kernel
 // when there are few right-hand sides (the reference's vector branch uses
 // trsv!, src/api/linalg_api.jl:168-184), with the multi-RHS chain up to
-// OXI_CHAIN_MAX_RHS (no transpose, FP64; mws its prepared workspace), else
-// the blocked trsm.
+// OXI_CHAIN_MAX_RHS (FP64; mws its workspace, prepared for the kind tk),
+// else the blocked trsm.
 template <typename T>
 int triangles(oxi_handle_t h, hipStream_t s, int tk, i64 n, i64 nrhs, const T* LU, i64 lda, T* B, i64 ldb,
               void* ctrl, void* mws) {
@@ -81,9 +81,9 @@ int triangles(oxi_handle_t h, hipStream_t s, int tk, i64 n, i64 nrhs, const T* L
         }
         return OXI_OK;
     }
-    if (mws && use_chain(tk, nrhs)) {
-        OXI_TRY(chain_triangle(h, s, 0, n, nrhs, LU, lda, B, ldb, mws));
-        OXI_TRY(chain_triangle(h, s, 1, n, nrhs, LU, lda, B, ldb, mws));
+    if (mws && use_chain(nrhs)) {
+        OXI_TRY(chain_triangle(h, s, tk, 0, n, nrhs, LU, lda, B, ldb, mws));
+        OXI_TRY(chain_triangle(h, s, tk, 1, n, nrhs, LU, lda, B, ldb, mws));
         return OXI_OK;
     }
     if (!tk) {
@@ -158,7 +158,7 @@ int getrs_api(oxi_handle_t h, char trans, i64 n, i64 nrhs, const T* LU, i64 lda,
     void* ws;
     const size_t xpad = (xbytes + 255) & ~size_t(255), ppad = (size_t(n) * 8 + 255) & ~size_t(255);
     const bool chain = nrhs <= OXI_TRSV_MAX_RHS;
-    const bool mchain = sizeof(T) == 8 && use_chain(tk, nrhs);
+    const bool mchain = sizeof(T) == 8 && use_chain(nrhs);
     OXI_TRY(oxi_workspace(h, xpad + ppad + (chain ? oxi_trsv_ws_bytes(n, sizeof(T)) : mchain ? oxi_trsm_chain_ws_bytes(n) : 256),
                           &ws));
     T* Y = (T*)ws;
@@ -171,7 +171,7 @@ int getrs_api(oxi_handle_t h, char trans, i64 n, i64 nrhs, const T* LU, i64 lda,
     }
     if (mchain) {
         OXI_TRY(oxi_trsm_chain_begin(h, s, ctrl));
-        OXI_TRY(oxi_trsm_chain_prepare(h, s, n, (const double*)LU, lda, ctrl));
+        OXI_TRY(oxi_trsm_chain_prepare(h, s, tk, n, (const double*)LU, lda, ctrl));
     }
     OXI_TRY(getrs_perm<T>(h, s, tk, n, nrhs, LU, lda, dperm, B, ldb, Y, chain ? ctrl : nullptr, mchain ? ctrl : nullptr));
     if (chain) return oxi_trsv_check(h, s, ctrl);    // the chain kernel ran: its status
@@ -231,7 +231,7 @@ struct oxi_getrs_plan_s {
     i64* dperm = nullptr;
     void* ctrl[2] = {nullptr, nullptr};
     bool prepared[2] = {false, false};   // the inverses of kind 'N' (0) / 'T' (1), made on first use
-    void* mws = nullptr;                 // the multi-RHS chain's workspace, made and prepared on first use
+    void* mws[2] = {nullptr, nullptr};   // the multi-RHS chain's workspace per kind, made and prepared on first use
     i64 zero_pivot = 0;       // first zero on U's diagonal (1-based), 0 if none
 };
 
@@ -301,25 +301,25 @@ extern "C" int oxi_dgetrs_plan(oxi_handle_t h, oxi_getrs_plan_t plan, char trans
             plan->prepared[tk] = true;
         }
     }
-    const bool mchain = use_chain(tk, nrhs);
+    const bool mchain = use_chain(nrhs);
     if (mchain) {
-        if (!plan->mws) {
+        if (!plan->mws[tk]) {
             void* m = nullptr;
             if (hipMalloc(&m, oxi_trsm_chain_ws_bytes(n)) != hipSuccess)
                 return oxi_set_error(h, OXI_ERR_ALLOC, "getrs plan: allocation failed");
-            if (oxi_trsm_chain_prepare(h, s, n, plan->LU, plan->lda, m) != OXI_OK) {
+            if (oxi_trsm_chain_prepare(h, s, tk, n, plan->LU, plan->lda, m) != OXI_OK) {
                 (void)hipStreamSynchronize(s);
                 (void)hipFree(m);
                 return OXI_ERR_HIP;
             }
-            plan->mws = m;
+            plan->mws[tk] = m;
         }
-        OXI_TRY(oxi_trsm_chain_begin(h, s, plan->mws));
+        OXI_TRY(oxi_trsm_chain_begin(h, s, plan->mws[tk]));
     }
     OXI_TRY(getrs_perm<double>(h, s, tk, n, nrhs, plan->LU, plan->lda, plan->dperm, B, ldb, (double*)ws, ctrl,
-                               mchain ? plan->mws : nullptr));
+                               mchain ? plan->mws[tk] : nullptr));
     if (chain) return oxi_trsv_check(h, s, ctrl);
-    if (mchain) return oxi_trsm_chain_check(h, s, plan->mws);
+    if (mchain) return oxi_trsm_chain_check(h, s, plan->mws[tk]);
     OXI_HIP(h, hipStreamSynchronize(s));
     return OXI_OK;
 }
@@ -332,7 +332,7 @@ extern "C" int oxi_dgetrs_plan_destroy(oxi_handle_t h, oxi_getrs_plan_t plan) {
     (void)hipStreamSynchronize(h->stream);
     if (plan->dperm) (void)hipFree(plan->dperm);
     for (auto c : plan->ctrl) if (c) (void)hipFree(c);
-    if (plan->mws) (void)hipFree(plan->mws);
+    for (auto m : plan->mws) if (m) (void)hipFree(m);
     delete plan;
     return OXI_OK;
 }

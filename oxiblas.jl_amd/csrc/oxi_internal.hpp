@@ -173,21 +173,21 @@ int oxi_trsv_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const float* 
 // LU solves with at most this many right-hand sides go column by column
 // through oxi_trsv (when the caller supplies ctrl), more through trsm.
 constexpr i64 OXI_TRSV_MAX_RHS = 4;
-// FP64 LU solves (no transpose) with more right-hand sides, up to this many,
-// go through the chained multi-RHS solve (trsm_chain.hip) when the caller
-// supplies its workspace: control words and, for both triangles, the
-// inverses of the 64 x 64 diagonal blocks and each block's coupling to the
-// block solved before it, filled once per factor by oxi_trsm_chain_prepare.
-// oxi_trsm_chain_begin clears the status word once per solve; each
-// oxi_trsm_chain solves one triangle (upper 0: L unit forward, 1: U
-// backward) in place in B; oxi_trsm_chain_check reads the status
-// (synchronises).
+// FP64 LU solves with more right-hand sides, up to this many, go through
+// the chained multi-RHS solve (trsm_chain.hip) when the caller supplies its
+// workspace: control words and, for both triangles of the solve kind tk
+// (0: L then U, 1: U^T then L^T), the inverses of the 64 x 64 diagonal
+// blocks and each block's coupling to the block solved before it, filled
+// once per factor and kind by oxi_trsm_chain_prepare.  oxi_trsm_chain_begin
+// clears the status word once per solve; each oxi_trsm_chain solves one
+// triangle (second 0: the first of the kind, 1: the second) in place in B;
+// oxi_trsm_chain_check reads the status (synchronises).
 constexpr i64 OXI_CHAIN_MAX_RHS = 64;
 size_t oxi_trsm_chain_ws_bytes(i64 n);
-int oxi_trsm_chain_prepare(oxi_handle_t h, hipStream_t s, i64 n, const double* LU, i64 lda, void* ws);
+int oxi_trsm_chain_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const double* LU, i64 lda, void* ws);
 int oxi_trsm_chain_begin(oxi_handle_t h, hipStream_t s, void* ws);
-int oxi_trsm_chain(oxi_handle_t h, hipStream_t s, int upper, i64 n, i64 nrhs, const double* LU, i64 lda, double* B,
-                   i64 ldb, void* ws);
+int oxi_trsm_chain(oxi_handle_t h, hipStream_t s, int tk, int second, i64 n, i64 nrhs, const double* LU, i64 lda,
+                   double* B, i64 ldb, void* ws);
 int oxi_trsm_chain_check(oxi_handle_t h, hipStream_t s, const void* ws);
 
 int oxi_first_zero_diag(oxi_handle_t h, hipStream_t s, const double* T, i64 ldt, i64 n, i64* where);

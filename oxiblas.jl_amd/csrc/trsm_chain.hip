@@ -1,11 +1,12 @@
 // Triangular solves of the LU factors with several right-hand sides (up to
-// OXI_CHAIN_MAX_RHS): the left TRSM of the solve, ldiv!(F, B) with B n x k
-// (src/api/linalg_api.jl:168-174, trsm! of src/blas/level3.jl:309-544 for
-// each triangle), as one launch per triangle on the matrix cores.
+// OXI_CHAIN_MAX_RHS): the left TRSMs of ldiv!(F, B) and
+// ldiv!(transpose(F), B) with B n x k (src/api/linalg_api.jl:168-184,
+// trsm! of src/blas/level3.jl:309-544 for each triangle), as one launch per
+// triangle on the matrix cores.
 //
-// The n rows are cut into blocks of CB = 64 rows in solve order (L: top
-// down, U: bottom up); block b needs X of every earlier block.  Within a
-// block the rows stay in their physical order.
+// The n rows are cut into blocks of CB = 64 rows in solve order (L and U^T:
+// top down, U and L^T: bottom up); block b needs X of every earlier block.
+// Within a block the rows stay in their physical order.
 //
 // Preparation (once per factor, oxi_trsm_chain_prepare): for both triangles
 // the inverse V_b of every diagonal block and G_b = V_b T_{b,b-1}, the
@@ -58,11 +59,12 @@ __device__ __forceinline__ i64 block_row0(bool bwd, i64 n, int b, int rows) {
     return bwd ? n - i64(b) * CB - rows : i64(b) * CB;
 }
 
-// V_b: the inverse of diagonal block b, V[b][k * CB + i] = V_b(i, k) in the
-// block's physical row order, zero outside its rows.  Thread j solves
+// V_b: the inverse of diagonal block b of op(T) (T or, with TRANS, its
+// transpose), V[b][k * CB + i] = V_b(i, k) in the block's physical row
+// order, zero outside its rows.  Thread j solves
 // T_bb x = e_j by substitution (forward for the lower triangle, backward for
 // the upper one) with the block and x in LDS.
-template <bool UPPER, bool UNIT>
+template <bool UPPER, bool UNIT, bool TRANS>
 __global__ __launch_bounds__(CB) void chain_inv_kernel(const double* __restrict__ M, i64 ld, i64 n,
                                                        double* __restrict__ V) {
     __shared__ double Ts[CB][CB + 1];   // Ts[r][c] = T_bb(r, c)
@@ -70,7 +72,8 @@ __global__ __launch_bounds__(CB) void chain_inv_kernel(const double* __restrict_
     const int j = threadIdx.x, b = blockIdx.x;
     const int rows = int(min(i64(CB), n - i64(b) * CB));
     const i64 p0 = block_row0(UPPER, n, b, rows);
-    for (int c = 0; c < rows; ++c) Ts[j][c] = j < rows ? M[(p0 + j) + (p0 + c) * ld] : 0.0;
+    for (int c = 0; c < rows; ++c)
+        Ts[j][c] = j < rows ? (TRANS ? M[(p0 + c) + (p0 + j) * ld] : M[(p0 + j) + (p0 + c) * ld]) : 0.0;
     __syncthreads();
     for (int r = 0; r < CB; ++r) Xs[r][j] = 0.0;
     if (j < rows) {
@@ -97,16 +100,16 @@ __global__ __launch_bounds__(CB) void chain_inv_kernel(const double* __restrict_
 
 // G_b = V_b T_{b,b-1} for b >= 1: G[b][q * CB + i] over physical rows i of
 // block b and physical rows q of block b-1 (a full block).
-template <bool BWD>
+template <bool BWD, bool TRANS>
 __global__ __launch_bounds__(CNT) void chain_g_kernel(const double* __restrict__ M, i64 ld, i64 n,
                                                       const double* __restrict__ V, double* __restrict__ G) {
-    __shared__ double Tb[CB][CB + 1];   // Tb[k][q] = T(row k of block b, row q of block b-1)
+    __shared__ double Tb[CB][CB + 1];   // Tb[k][q] = op(T)(row k of block b, row q of block b-1)
     const int t = threadIdx.x, b = blockIdx.x + 1;
     const int rows = int(min(i64(CB), n - i64(b) * CB));
     const i64 p0 = block_row0(BWD, n, b, rows), q0 = block_row0(BWD, n, b - 1, CB);
     for (int e = t; e < CB * CB; e += CNT) {
         const int k = e % CB, q = e / CB;
-        Tb[k][q] = k < rows ? M[(p0 + k) + (q0 + q) * ld] : 0.0;
+        Tb[k][q] = k < rows ? (TRANS ? M[(q0 + q) + (p0 + k) * ld] : M[(p0 + k) + (q0 + q) * ld]) : 0.0;
     }
     __syncthreads();
     const double* vb = V + size_t(b) * CB * CB;
@@ -187,7 +190,7 @@ __device__ __forceinline__ int wait_ready(ChainCtrl* ctrl, int h, int need, int 
     return r;
 }
 
-template <bool BWD>
+template <bool BWD, bool TRANS>
 __global__ __launch_bounds__(CNT, 2) void trsm_chain_kernel(const double* __restrict__ M, i64 ld, i64 n, double* B,
                                                          i64 ldb, int nrhs, int nchains, ChainCtrl* ctrl,
                                                          const double* __restrict__ V,
@@ -231,7 +234,10 @@ __global__ __launch_bounds__(CNT, 2) void trsm_chain_kernel(const double* __rest
                     const i64 q0 = block_row0(BWD, n, c + k, CB);
                     load_x(xr, ldb, q0, ncol, t, v[k]);
 #pragma unroll
-                    for (int s = 0; s < 16; ++s) ak[k][s] = ra_ok ? M[(p0 + ra) + (q0 + 4 * s + fk) * ld] : 0.0;
+                    for (int s = 0; s < 16; ++s)
+                        ak[k][s] = !ra_ok ? 0.0
+                                          : TRANS ? M[(q0 + 4 * s + fk) + (p0 + ra) * ld]
+                                                  : M[(p0 + ra) + (q0 + 4 * s + fk) * ld];
                 }
             }
 #pragma unroll
@@ -322,15 +328,26 @@ int launch_check(oxi_handle_t h, const char* what) {
 
 size_t oxi_trsm_chain_ws_bytes(i64 n) { return 256 + 4 * slot_doubles(std::max<i64>(n, 1)) * sizeof(double); }
 
-int oxi_trsm_chain_prepare(oxi_handle_t h, hipStream_t s, i64 n, const double* LU, i64 lda, void* ws) {
+int oxi_trsm_chain_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const double* LU, i64 lda, void* ws) {
     if (n <= 0) return OXI_OK;
     const unsigned nb = unsigned(cdiv(n, CB));
-    double *vl = slot_ptr(ws, n, 0), *vu = slot_ptr(ws, n, 1), *gl = slot_ptr(ws, n, 2), *gu = slot_ptr(ws, n, 3);
-    hipLaunchKernelGGL((chain_inv_kernel<false, true>), dim3(nb), dim3(CB), 0, s, LU, lda, n, vl);
-    hipLaunchKernelGGL((chain_inv_kernel<true, false>), dim3(nb), dim3(CB), 0, s, LU, lda, n, vu);
+    // slots: V of the first and second triangle, then their G
+    double *v0 = slot_ptr(ws, n, 0), *v1 = slot_ptr(ws, n, 1), *g0 = slot_ptr(ws, n, 2), *g1 = slot_ptr(ws, n, 3);
+    if (!tk) {   // L (unit, forward), then U (backward)
+        hipLaunchKernelGGL((chain_inv_kernel<false, true, false>), dim3(nb), dim3(CB), 0, s, LU, lda, n, v0);
+        hipLaunchKernelGGL((chain_inv_kernel<true, false, false>), dim3(nb), dim3(CB), 0, s, LU, lda, n, v1);
+    } else {     // U^T (forward), then L^T (unit, backward)
+        hipLaunchKernelGGL((chain_inv_kernel<false, false, true>), dim3(nb), dim3(CB), 0, s, LU, lda, n, v0);
+        hipLaunchKernelGGL((chain_inv_kernel<true, true, true>), dim3(nb), dim3(CB), 0, s, LU, lda, n, v1);
+    }
     if (nb > 1) {
-        hipLaunchKernelGGL(chain_g_kernel<false>, dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, vl, gl);
-        hipLaunchKernelGGL(chain_g_kernel<true>, dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, vu, gu);
+        if (!tk) {
+            hipLaunchKernelGGL((chain_g_kernel<false, false>), dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, v0, g0);
+            hipLaunchKernelGGL((chain_g_kernel<true, false>), dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, v1, g1);
+        } else {
+            hipLaunchKernelGGL((chain_g_kernel<false, true>), dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, v0, g0);
+            hipLaunchKernelGGL((chain_g_kernel<true, true>), dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, v1, g1);
+        }
     }
     return launch_check(h, "trsm chain preparation");
 }
@@ -340,8 +357,8 @@ int oxi_trsm_chain_begin(oxi_handle_t h, hipStream_t s, void* ws) {
     return launch_check(h, "trsm chain status clear");
 }
 
-int oxi_trsm_chain(oxi_handle_t h, hipStream_t s, int upper, i64 n, i64 nrhs, const double* LU, i64 lda, double* B,
-                   i64 ldb, void* ws) {
+int oxi_trsm_chain(oxi_handle_t h, hipStream_t s, int tk, int second, i64 n, i64 nrhs, const double* LU, i64 lda,
+                   double* B, i64 ldb, void* ws) {
     if (n <= 0 || nrhs <= 0) return OXI_OK;
     if (nrhs > OXI_CHAIN_MAX_RHS) return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "trsm chain: too many right-hand sides");
     const i64 nb = cdiv(n, CB);
@@ -350,16 +367,23 @@ int oxi_trsm_chain(oxi_handle_t h, hipStream_t s, int upper, i64 n, i64 nrhs, co
     if ((n + i64(CR) * ldb) * 8 >= (i64(1) << 31) - 1)   // the device-scope loads' offsets
         return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "trsm chain: leading dimension too large");
     ChainCtrl* c = static_cast<ChainCtrl*>(ws);
-    const double* V = slot_ptr(ws, n, upper ? 1 : 0);
-    const double* G = slot_ptr(ws, n, upper ? 3 : 2);
+    const double* V = slot_ptr(ws, n, second ? 1 : 0);
+    const double* G = slot_ptr(ws, n, second ? 3 : 2);
     hipLaunchKernelGGL(chain_ctrl_reset, dim3(1), dim3(64), 0, s, c);
-    const dim3 grid(unsigned(nb * nchains));
-    if (upper)
-        hipLaunchKernelGGL(trsm_chain_kernel<true>, grid, dim3(CNT), 0, s, LU, lda, n, B, ldb, int(nrhs), nchains, c, V,
-                           G);
+    const dim3 grid(unsigned(nb * nchains)), block(CNT);
+    // the first triangle runs forward, the second backward
+    if (!tk && !second)
+        hipLaunchKernelGGL((trsm_chain_kernel<false, false>), grid, block, 0, s, LU, lda, n, B, ldb, int(nrhs), nchains,
+                           c, V, G);
+    else if (!tk)
+        hipLaunchKernelGGL((trsm_chain_kernel<true, false>), grid, block, 0, s, LU, lda, n, B, ldb, int(nrhs), nchains,
+                           c, V, G);
+    else if (!second)
+        hipLaunchKernelGGL((trsm_chain_kernel<false, true>), grid, block, 0, s, LU, lda, n, B, ldb, int(nrhs), nchains,
+                           c, V, G);
     else
-        hipLaunchKernelGGL(trsm_chain_kernel<false>, grid, dim3(CNT), 0, s, LU, lda, n, B, ldb, int(nrhs), nchains, c,
-                           V, G);
+        hipLaunchKernelGGL((trsm_chain_kernel<true, true>), grid, block, 0, s, LU, lda, n, B, ldb, int(nrhs), nchains,
+                           c, V, G);
     return launch_check(h, "trsm chain");
 }
 

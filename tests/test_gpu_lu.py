@@ -303,10 +303,11 @@ def test_64_rhs_solves_n4096(gpu, rng):
 
 @pytest.mark.parametrize("n", (1, 63, 64, 65, 200, 1000))
 def test_chained_multi_rhs_solves(gpu, rng, n):
-    """The chained multi-RHS solve (5 to 64 right-hand sides, no transpose):
-    a partial last block of rows, partial groups of right-hand sides and a
-    padded leading dimension, through the solve plan and through oxi_dgetrs,
-    against the oracle's dgetrs; the rows outside B's view stay untouched."""
+    """The chained multi-RHS solve (5 to 64 right-hand sides, both solve
+    kinds): a partial last block of rows, partial groups of right-hand sides
+    and a padded leading dimension, through the solve plan and through
+    oxi_dgetrs, against the oracle's dgetrs; the rows outside B's view stay
+    untouched."""
     from oxiblas_mi355x import _lib
     ox = gpu
     A = rng.standard_normal((n, n))
@@ -314,22 +315,25 @@ def test_chained_multi_rhs_solves(gpu, rng, n):
     Fo, ipo = F.factors.numpy(), F.ipiv
     c = np.linalg.cond(A)
     h = _lib.handle()
-    for nrhs in (5, 17, 32, 33, 64):
-        B = rng.standard_normal((n, nrhs))
-        Xo = O.dgetrs(Fo, ipo, B, trans="N")
-        tol = 64 * n * EPS * c * np.abs(Xo).max()
-        X = F.solve(B)                                        # the solve plan
-        np.testing.assert_allclose(X, Xo, rtol=0, atol=tol)
-        assert bwd(A @ X - B, A) <= 30 * c
-        pad = 7                                               # oxi_dgetrs on a view with ld = n + 7
-        Big = np.asfortranarray(rng.standard_normal((n + pad, nrhs)))
-        Big[:n] = B
-        Bd = ox.DMatrix.from_numpy(Big)
-        rc = h._L.oxi_dgetrs(h.ptr, b"N", n, nrhs, F.factors.ptr, F.factors.ld, F.ipiv_dev.data_ptr(), Bd.ptr, Bd.ld)
-        h.check(rc, "getrs")
-        out = Bd.numpy()
-        np.testing.assert_allclose(out[:n], Xo, rtol=0, atol=tol)
-        assert np.array_equal(out[n:], Big[n:])
+    for trans in ("N", "T"):
+        opA = A if trans == "N" else A.T
+        for nrhs in (5, 17, 32, 33, 64):
+            B = rng.standard_normal((n, nrhs))
+            Xo = O.dgetrs(Fo, ipo, B, trans=trans)
+            tol = 64 * n * EPS * c * np.abs(Xo).max()
+            X = F.solve(B, trans)                                 # the solve plan
+            np.testing.assert_allclose(X, Xo, rtol=0, atol=tol)
+            assert bwd(opA @ X - B, A) <= 30 * c
+            pad = 7                                               # oxi_dgetrs on a view with ld = n + 7
+            Big = np.asfortranarray(rng.standard_normal((n + pad, nrhs)))
+            Big[:n] = B
+            Bd = ox.DMatrix.from_numpy(Big)
+            rc = h._L.oxi_dgetrs(h.ptr, trans.encode(), n, nrhs, F.factors.ptr, F.factors.ld, F.ipiv_dev.data_ptr(),
+                                 Bd.ptr, Bd.ld)
+            h.check(rc, "getrs")
+            out = Bd.numpy()
+            np.testing.assert_allclose(out[:n], Xo, rtol=0, atol=tol)
+            assert np.array_equal(out[n:], Big[n:])
 
 
 def test_pivots_equal_lapack_n16384(gpu):
