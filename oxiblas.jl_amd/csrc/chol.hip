@@ -288,7 +288,18 @@ extern "C" int oxi_dpotrs(oxi_handle_t h, char uplo, int64_t n, int64_t nrhs, co
     i64 z = 0;
     OXI_TRY(oxi_first_zero_diag(h, s, L, ldl, n, &z));
     if (z) return oxi_set_error(h, OXI_ERR_SINGULAR, "potrs: L(%lld,%lld) is zero", (long long)z, (long long)z);
-    // L Y = B, then L^T X = Y (factorizations.jl:288-300)
+    // L Y = B, then L^T X = Y (factorizations.jl:288-300): up to
+    // OXI_CHAIN_MAX_RHS right-hand sides through the chained solve (its
+    // workspace prepared for this factor), more through the left trsm
+    if (oxi_trsm_chain_fits(n, nrhs, ldb)) {
+        void* ws;
+        OXI_TRY(oxi_workspace(h, oxi_trsm_chain_ws_bytes(n), &ws));
+        OXI_TRY(oxi_trsm_chain_begin(h, s, ws));
+        OXI_TRY(oxi_trsm_chain_prepare(h, s, 2, n, L, ldl, ws));
+        OXI_TRY(oxi_trsm_chain(h, s, 2, 0, n, nrhs, L, ldl, B, ldb, ws));
+        OXI_TRY(oxi_trsm_chain(h, s, 2, 1, n, nrhs, L, ldl, B, ldb, ws));
+        return oxi_trsm_chain_check(h, s, ws);
+    }
     OXI_TRY(oxi_dtrsm_left(h, s, 0, 0, 0, n, nrhs, L, ldl, B, ldb));
     OXI_TRY(oxi_dtrsm_left(h, s, 0, 1, 0, n, nrhs, L, ldl, B, ldb));
     OXI_HIP(h, hipStreamSynchronize(s));

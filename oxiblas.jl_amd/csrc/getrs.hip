@@ -64,7 +64,7 @@ int chain_triangle(oxi_handle_t h, hipStream_t, int, int, i64, i64, const float*
 
 // Whether a solve takes the chained multi-RHS triangles (their workspace is
 // then prepared for the factors).
-bool use_chain(i64 nrhs) { return nrhs > OXI_TRSV_MAX_RHS && nrhs <= OXI_CHAIN_MAX_RHS; }
+bool use_chain(i64 n, i64 nrhs, i64 ldb) { return nrhs > OXI_TRSV_MAX_RHS && oxi_trsm_chain_fits(n, nrhs, ldb); }
 
 // Two triangles of one solve: per column with the single-RHS chain kernel
 // when there are few right-hand sides (the reference's vector branch uses
@@ -81,7 +81,7 @@ int triangles(oxi_handle_t h, hipStream_t s, int tk, i64 n, i64 nrhs, const T* L
         }
         return OXI_OK;
     }
-    if (mws && use_chain(nrhs)) {
+    if (mws && use_chain(n, nrhs, ldb)) {
         OXI_TRY(chain_triangle(h, s, tk, 0, n, nrhs, LU, lda, B, ldb, mws));
         OXI_TRY(chain_triangle(h, s, tk, 1, n, nrhs, LU, lda, B, ldb, mws));
         return OXI_OK;
@@ -158,7 +158,7 @@ int getrs_api(oxi_handle_t h, char trans, i64 n, i64 nrhs, const T* LU, i64 lda,
     void* ws;
     const size_t xpad = (xbytes + 255) & ~size_t(255), ppad = (size_t(n) * 8 + 255) & ~size_t(255);
     const bool chain = nrhs <= OXI_TRSV_MAX_RHS;
-    const bool mchain = sizeof(T) == 8 && use_chain(nrhs);
+    const bool mchain = sizeof(T) == 8 && use_chain(n, nrhs, ldb);
     OXI_TRY(oxi_workspace(h, xpad + ppad + (chain ? oxi_trsv_ws_bytes(n, sizeof(T)) : mchain ? oxi_trsm_chain_ws_bytes(n) : 256),
                           &ws));
     T* Y = (T*)ws;
@@ -301,7 +301,7 @@ extern "C" int oxi_dgetrs_plan(oxi_handle_t h, oxi_getrs_plan_t plan, char trans
             plan->prepared[tk] = true;
         }
     }
-    const bool mchain = use_chain(nrhs);
+    const bool mchain = use_chain(n, nrhs, ldb);
     if (mchain) {
         if (!plan->mws[tk]) {
             void* m = nullptr;

@@ -176,7 +176,8 @@ constexpr i64 OXI_TRSV_MAX_RHS = 4;
 // FP64 LU solves with more right-hand sides, up to this many, go through
 // the chained multi-RHS solve (trsm_chain.hip) when the caller supplies its
 // workspace: control words and, for both triangles of the solve kind tk
-// (0: L then U, 1: U^T then L^T), the inverses of the 64 x 64 diagonal
+// (0: L then U, 1: U^T then L^T, 2: a Cholesky factor's L (non-unit) then
+// L^T), the inverses of the 64 x 64 diagonal
 // blocks and each block's coupling to the block solved before it, filled
 // once per factor and kind by oxi_trsm_chain_prepare.  oxi_trsm_chain_begin
 // clears the status word once per solve; each oxi_trsm_chain solves one
@@ -184,6 +185,8 @@ constexpr i64 OXI_TRSV_MAX_RHS = 4;
 // oxi_trsm_chain_check reads the status (synchronises).
 constexpr i64 OXI_CHAIN_MAX_RHS = 64;
 size_t oxi_trsm_chain_ws_bytes(i64 n);
+// whether the chain takes this solve (else it returns OXI_ERR_UNSUPPORTED)
+bool oxi_trsm_chain_fits(i64 n, i64 nrhs, i64 ldb);
 int oxi_trsm_chain_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const double* LU, i64 lda, void* ws);
 int oxi_trsm_chain_begin(oxi_handle_t h, hipStream_t s, void* ws);
 int oxi_trsm_chain(oxi_handle_t h, hipStream_t s, int tk, int second, i64 n, i64 nrhs, const double* LU, i64 lda,

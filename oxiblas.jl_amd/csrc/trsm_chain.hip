@@ -326,6 +326,12 @@ int launch_check(oxi_handle_t h, const char* what) {
 
 }  // namespace
 
+bool oxi_trsm_chain_fits(i64 n, i64 nrhs, i64 ldb) {
+    // grid size, and the device-scope loads' 32-bit byte offsets
+    return nrhs <= OXI_CHAIN_MAX_RHS && cdiv(n, CB) * cdiv(nrhs, CR) <= 0x7fffffff &&
+           (n + i64(CR) * ldb) * 8 < (i64(1) << 31) - 1;
+}
+
 size_t oxi_trsm_chain_ws_bytes(i64 n) { return 256 + 4 * slot_doubles(std::max<i64>(n, 1)) * sizeof(double); }
 
 int oxi_trsm_chain_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const double* LU, i64 lda, void* ws) {
@@ -333,21 +339,26 @@ int oxi_trsm_chain_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const d
     const unsigned nb = unsigned(cdiv(n, CB));
     // slots: V of the first and second triangle, then their G
     double *v0 = slot_ptr(ws, n, 0), *v1 = slot_ptr(ws, n, 1), *g0 = slot_ptr(ws, n, 2), *g1 = slot_ptr(ws, n, 3);
-    if (!tk) {   // L (unit, forward), then U (backward)
+    if (tk == 0) {          // L (unit, forward), then U (backward)
         hipLaunchKernelGGL((chain_inv_kernel<false, true, false>), dim3(nb), dim3(CB), 0, s, LU, lda, n, v0);
         hipLaunchKernelGGL((chain_inv_kernel<true, false, false>), dim3(nb), dim3(CB), 0, s, LU, lda, n, v1);
-    } else {     // U^T (forward), then L^T (unit, backward)
+    } else if (tk == 1) {   // U^T (forward), then L^T (unit, backward)
         hipLaunchKernelGGL((chain_inv_kernel<false, false, true>), dim3(nb), dim3(CB), 0, s, LU, lda, n, v0);
         hipLaunchKernelGGL((chain_inv_kernel<true, true, true>), dim3(nb), dim3(CB), 0, s, LU, lda, n, v1);
+    } else {                // Cholesky: L (forward), then L^T (backward)
+        hipLaunchKernelGGL((chain_inv_kernel<false, false, false>), dim3(nb), dim3(CB), 0, s, LU, lda, n, v0);
+        hipLaunchKernelGGL((chain_inv_kernel<true, false, true>), dim3(nb), dim3(CB), 0, s, LU, lda, n, v1);
     }
+    // the first triangle is op(T) = T except for U^T, the second T^T except for U
     if (nb > 1) {
-        if (!tk) {
-            hipLaunchKernelGGL((chain_g_kernel<false, false>), dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, v0, g0);
-            hipLaunchKernelGGL((chain_g_kernel<true, false>), dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, v1, g1);
-        } else {
+        if (tk == 1)
             hipLaunchKernelGGL((chain_g_kernel<false, true>), dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, v0, g0);
+        else
+            hipLaunchKernelGGL((chain_g_kernel<false, false>), dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, v0, g0);
+        if (tk == 0)
+            hipLaunchKernelGGL((chain_g_kernel<true, false>), dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, v1, g1);
+        else
             hipLaunchKernelGGL((chain_g_kernel<true, true>), dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, v1, g1);
-        }
     }
     return launch_check(h, "trsm chain preparation");
 }
@@ -363,19 +374,20 @@ int oxi_trsm_chain(oxi_handle_t h, hipStream_t s, int tk, int second, i64 n, i64
     if (nrhs > OXI_CHAIN_MAX_RHS) return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "trsm chain: too many right-hand sides");
     const i64 nb = cdiv(n, CB);
     const int nchains = int(cdiv(nrhs, CR));
-    if (nb * nchains > 0x7fffffff) return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "trsm chain: n too large");
-    if ((n + i64(CR) * ldb) * 8 >= (i64(1) << 31) - 1)   // the device-scope loads' offsets
-        return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "trsm chain: leading dimension too large");
+    if (!oxi_trsm_chain_fits(n, nrhs, ldb))
+        return oxi_set_error(h, OXI_ERR_UNSUPPORTED, "trsm chain: n or the leading dimension too large");
     ChainCtrl* c = static_cast<ChainCtrl*>(ws);
     const double* V = slot_ptr(ws, n, second ? 1 : 0);
     const double* G = slot_ptr(ws, n, second ? 3 : 2);
     hipLaunchKernelGGL(chain_ctrl_reset, dim3(1), dim3(64), 0, s, c);
     const dim3 grid(unsigned(nb * nchains)), block(CNT);
-    // the first triangle runs forward, the second backward
-    if (!tk && !second)
+    // the first triangle runs forward, the second backward; op(T) as in the
+    // preparation
+    const bool trans = second ? tk != 0 : tk == 1;
+    if (!trans && !second)
         hipLaunchKernelGGL((trsm_chain_kernel<false, false>), grid, block, 0, s, LU, lda, n, B, ldb, int(nrhs), nchains,
                            c, V, G);
-    else if (!tk)
+    else if (!trans)
         hipLaunchKernelGGL((trsm_chain_kernel<true, false>), grid, block, 0, s, LU, lda, n, B, ldb, int(nrhs), nchains,
                            c, V, G);
     else if (!second)

@@ -114,3 +114,11 @@ def test_cholesky_n8192(gpu):
     err = torch.linalg.matrix_norm(L @ L.T - A).item() / (torch.linalg.matrix_norm(A).item() * n * EPS)
     assert err <= 30
     assert torch.count_nonzero(torch.triu(L, 1)).item() == 0
+    # 64 right-hand sides (the chained solve): normwise backward error
+    B = torch.randn(64, n, dtype=torch.float64, device="cuda", generator=g)   # B[c, i] = column c
+    X = F.solve(ox.DMatrix(B.reshape(-1).clone(), n, 64))
+    Xt = X.buf[:n * 64].view(64, n)
+    R = A @ Xt.T - B.T
+    berr = torch.linalg.matrix_norm(R).item() / (torch.linalg.matrix_norm(A).item() *
+                                                 torch.linalg.matrix_norm(Xt).item() * n * EPS)
+    assert berr <= 30, berr
