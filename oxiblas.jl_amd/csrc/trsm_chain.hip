@@ -9,12 +9,16 @@
 // Within a block the rows stay in their physical order.
 //
 // Preparation (once per factor, oxi_trsm_chain_prepare): for both triangles
-// the inverse V_b of every diagonal block and G_b = V_b T_{b,b-1}, the
-// block's coupling to the block solved just before it.  Then
-//     X_b = V_b (B_b - sum_{c < b-1} T_{b,c} X_c) - G_b X_{b-1},
-// so only the product G_b X_{b-1} waits for block b-1; the sum over the
-// earlier blocks and its product with V_b are formed while block b-1 is
-// still being solved.  (Inverting the diagonal blocks changes the rounding
+// the inverse V_b of every diagonal block and its couplings to the two
+// blocks solved just before it, G_b = V_b T_{b,b-1} and H_b = V_b T_{b,b-2}.
+// Then
+//     X_b = V_b (B_b - sum_{c < b-2} T_{b,c} X_c) - H_b X_{b-2} - G_b X_{b-1},
+// so only the product G_b X_{b-1} waits for block b-1, and H_b X_{b-2} for
+// block b-2; the sum over the earlier blocks and its product with V_b are
+// formed while those are still being solved.  (With only G_b, the last
+// block of the sum and the V_b product fell on the chain: measured with
+// clock stamps, most workgroups reached their wait for block b-1 after it
+// had published.)  (Inverting the diagonal blocks changes the rounding
 // of a substitution, not the result beyond it; the solve is deterministic.)
 //
 // The solve: a workgroup per block and group of CR = 32 right-hand sides
@@ -35,7 +39,7 @@ namespace {
 constexpr int CB = 64;       // rows per block
 constexpr int CR = 32;       // right-hand sides per workgroup
 constexpr int CNT = 256;     // threads per workgroup
-constexpr int XBC = 2;       // published blocks of X applied per round
+constexpr int XBC = 1;       // published blocks of X applied per round
 constexpr int CHAINS = OXI_CHAIN_MAX_RHS / CR;
 constexpr unsigned CSPIN_LIMIT = 1u << 20;
 static_assert(CNT == 4 * 64 && CB == 4 * 16 && CR == 2 * 16, "four waves of 16 rows, two 16-column tiles");
@@ -98,15 +102,16 @@ __global__ __launch_bounds__(CB) void chain_inv_kernel(const double* __restrict_
     for (int c = 0; c < CB; ++c) out[c * CB + j] = Xs[j][c];
 }
 
-// G_b = V_b T_{b,b-1} for b >= 1: G[b][q * CB + i] over physical rows i of
-// block b and physical rows q of block b-1 (a full block).
+// G_b = V_b T_{b,b-d} for b >= d (d = 1: G_b, d = 2: H_b): G[b][q * CB + i]
+// over physical rows i of block b and physical rows q of block b-d (a full
+// block).
 template <bool BWD, bool TRANS>
-__global__ __launch_bounds__(CNT) void chain_g_kernel(const double* __restrict__ M, i64 ld, i64 n,
+__global__ __launch_bounds__(CNT) void chain_g_kernel(const double* __restrict__ M, i64 ld, i64 n, int d,
                                                       const double* __restrict__ V, double* __restrict__ G) {
-    __shared__ double Tb[CB][CB + 1];   // Tb[k][q] = op(T)(row k of block b, row q of block b-1)
-    const int t = threadIdx.x, b = blockIdx.x + 1;
+    __shared__ double Tb[CB][CB + 1];   // Tb[k][q] = op(T)(row k of block b, row q of block b-d)
+    const int t = threadIdx.x, b = blockIdx.x + d;
     const int rows = int(min(i64(CB), n - i64(b) * CB));
-    const i64 p0 = block_row0(BWD, n, b, rows), q0 = block_row0(BWD, n, b - 1, CB);
+    const i64 p0 = block_row0(BWD, n, b, rows), q0 = block_row0(BWD, n, b - d, CB);
     for (int e = t; e < CB * CB; e += CNT) {
         const int k = e % CB, q = e / CB;
         Tb[k][q] = k < rows ? (TRANS ? M[(q0 + q) + (p0 + k) * ld] : M[(p0 + k) + (q0 + q) * ld]) : 0.0;
@@ -194,8 +199,9 @@ template <bool BWD, bool TRANS>
 __global__ __launch_bounds__(CNT, 2) void trsm_chain_kernel(const double* __restrict__ M, i64 ld, i64 n, double* B,
                                                          i64 ldb, int nrhs, int nchains, ChainCtrl* ctrl,
                                                          const double* __restrict__ V,
-                                                         const double* __restrict__ G) {
-    __shared__ double xs[XBC][CB][CR + 1];   // X of the blocks being applied, physical rows
+                                                         const double* __restrict__ G,
+                                                         const double* __restrict__ H) {
+    __shared__ double xs[2][CB][CR + 1];     // X of the blocks being applied, physical rows
     double (*rs)[CR + 1] = xs[1];            // B_b - sum, physical rows (after the blocks)
     __shared__ int s_tk, s_ready;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -213,12 +219,35 @@ __global__ __launch_bounds__(CNT, 2) void trsm_chain_kernel(const double* __rest
     d4v acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};   // sum over chunks, per column tile
     d4v xb[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};    // X_b
     const __amdgpu_buffer_rsrc_t xr = x_rsrc(B, ldb, j0);
-    double a[16];
+    // everything the block needs besides X of other blocks, loaded first so
+    // that no memory round trip is left between the last published block
+    // and this block's own X: its rows of B, and its operands of V_b, H_b
+    // and G_b (read-only here)
+    double bb[2][4], av[16], ah[16], ag[16];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = wave * 16 + 4 * r + fk, col = ct * 16 + fi;
+            bb[ct][r] = row < rows && col < ncol ? B[(p0 + row) + i64(j0 + col) * ldb] : 0.0;
+        }
+    {
+        const double* vb = V + size_t(b) * CB * CB;
+        const double* hb = H + size_t(b) * CB * CB;
+        const double* gb = G + size_t(b) * CB * CB;
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            av[s] = vb[(4 * s + fk) * CB + ra];
+            ah[s] = b > 1 ? -hb[(4 * s + fk) * CB + ra] : 0.0;
+            ag[s] = b > 0 ? -gb[(4 * s + fk) * CB + ra] : 0.0;
+        }
+    }
 
-    // 1. sum over the blocks c < b - 1 as they are published, up to XBC
+    // 1. sum over the blocks c < b - 2 as they are published, up to XBC
     // published blocks per round, their X and factor blocks loaded together
+    // (more than one per round measured no faster: the workgroups keep up)
     int avail = 0;
-    const int cend = b - 1;
+    const int cend = b - 2;
     for (int c = 0; c < cend;) {
         if (c >= avail) {
             avail = wait_ready(ctrl, h, c, t, &s_ready);
@@ -265,39 +294,52 @@ __global__ __launch_bounds__(CNT, 2) void trsm_chain_kernel(const double* __rest
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = wave * 16 + 4 * r + fk, col = ct * 16 + fi;
-            double v = 0.0;
-            if (row < rows && col < ncol) v = B[(p0 + row) + i64(j0 + col) * ldb] - acc[ct][r];
-            rs[row][col] = v;
+            rs[row][col] = row < rows && col < ncol ? bb[ct][r] - acc[ct][r] : 0.0;
         }
-    const double* vb = V + size_t(b) * CB * CB;
-#pragma unroll
-    for (int s = 0; s < 16; ++s) a[s] = vb[(4 * s + fk) * CB + ra];
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
-        xb[0] = mfma16(a[s], rs[4 * s + fk][fi], xb[0]);
-        xb[1] = mfma16(a[s], rs[4 * s + fk][16 + fi], xb[1]);
+        xb[0] = mfma16(av[s], rs[4 * s + fk][fi], xb[0]);
+        xb[1] = mfma16(av[s], rs[4 * s + fk][16 + fi], xb[1]);
     }
 
-    // 3. X_b -= G_b X_{b-1}: the only product that waits for block b-1
-    if (b > 0) {
-        const double* gb = G + size_t(b) * CB * CB;
-#pragma unroll
-        for (int s = 0; s < 16; ++s) a[s] = -gb[(4 * s + fk) * CB + ra];
-        if (b - 1 >= avail) {
-            avail = wait_ready(ctrl, h, b - 1, t, &s_ready);
+    // 3. X_b -= H_b X_{b-2}, then X_b -= G_b X_{b-1}: the products that
+    // wait for the two blocks solved last (their operands loaded first;
+    // X_{b-2} to xs[0], X_{b-1} to xs[1], which the V_b product has left)
+    if (b > 1) {
+        if (b - 2 >= avail) {
+            avail = wait_ready(ctrl, h, b - 2, t, &s_ready);
             if (avail < 0) return;
         }
         {
             double v[8];
-            load_x(xr, ldb, block_row0(BWD, n, b - 1, CB), ncol, t, v);
+            load_x(xr, ldb, block_row0(BWD, n, b - 2, CB), ncol, t, v);
             store_x(xs[0], ncol, t, v);
         }
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < 16; ++s) {
-            xb[0] = mfma16(a[s], xs[0][4 * s + fk][fi], xb[0]);
-            xb[1] = mfma16(a[s], xs[0][4 * s + fk][16 + fi], xb[1]);
+            xb[0] = mfma16(ah[s], xs[0][4 * s + fk][fi], xb[0]);
+            xb[1] = mfma16(ah[s], xs[0][4 * s + fk][16 + fi], xb[1]);
+        }
+    }
+    if (b > 0) {
+        if (b - 1 >= avail) {
+            avail = wait_ready(ctrl, h, b - 1, t, &s_ready);
+            if (avail < 0) return;
+        } else {
+            __syncthreads();   // every wave has finished the V_b product: xs[1] is free
+        }
+        {
+            double v[8];
+            load_x(xr, ldb, block_row0(BWD, n, b - 1, CB), ncol, t, v);
+            store_x(xs[1], ncol, t, v);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            xb[0] = mfma16(ag[s], xs[1][4 * s + fk][fi], xb[0]);
+            xb[1] = mfma16(ag[s], xs[1][4 * s + fk][16 + fi], xb[1]);
         }
     }
 
@@ -332,13 +374,14 @@ bool oxi_trsm_chain_fits(i64 n, i64 nrhs, i64 ldb) {
            (n + i64(CR) * ldb) * 8 < (i64(1) << 31) - 1;
 }
 
-size_t oxi_trsm_chain_ws_bytes(i64 n) { return 256 + 4 * slot_doubles(std::max<i64>(n, 1)) * sizeof(double); }
+size_t oxi_trsm_chain_ws_bytes(i64 n) { return 256 + 6 * slot_doubles(std::max<i64>(n, 1)) * sizeof(double); }
 
 int oxi_trsm_chain_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const double* LU, i64 lda, void* ws) {
     if (n <= 0) return OXI_OK;
     const unsigned nb = unsigned(cdiv(n, CB));
-    // slots: V of the first and second triangle, then their G
+    // slots: V of the first and second triangle, then their G, then their H
     double *v0 = slot_ptr(ws, n, 0), *v1 = slot_ptr(ws, n, 1), *g0 = slot_ptr(ws, n, 2), *g1 = slot_ptr(ws, n, 3);
+    double *h0 = slot_ptr(ws, n, 4), *h1 = slot_ptr(ws, n, 5);
     if (tk == 0) {          // L (unit, forward), then U (backward)
         hipLaunchKernelGGL((chain_inv_kernel<false, true, false>), dim3(nb), dim3(CB), 0, s, LU, lda, n, v0);
         hipLaunchKernelGGL((chain_inv_kernel<true, false, false>), dim3(nb), dim3(CB), 0, s, LU, lda, n, v1);
@@ -350,15 +393,19 @@ int oxi_trsm_chain_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const d
         hipLaunchKernelGGL((chain_inv_kernel<true, false, true>), dim3(nb), dim3(CB), 0, s, LU, lda, n, v1);
     }
     // the first triangle is op(T) = T except for U^T, the second T^T except for U
-    if (nb > 1) {
+    for (int d = 1; d <= 2; ++d) {
+        if (nb <= d) break;
+        double* c0 = d == 1 ? g0 : h0;
+        double* c1 = d == 1 ? g1 : h1;
+        const dim3 grid(nb - unsigned(d));
         if (tk == 1)
-            hipLaunchKernelGGL((chain_g_kernel<false, true>), dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, v0, g0);
+            hipLaunchKernelGGL((chain_g_kernel<false, true>), grid, dim3(CNT), 0, s, LU, lda, n, d, v0, c0);
         else
-            hipLaunchKernelGGL((chain_g_kernel<false, false>), dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, v0, g0);
+            hipLaunchKernelGGL((chain_g_kernel<false, false>), grid, dim3(CNT), 0, s, LU, lda, n, d, v0, c0);
         if (tk == 0)
-            hipLaunchKernelGGL((chain_g_kernel<true, false>), dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, v1, g1);
+            hipLaunchKernelGGL((chain_g_kernel<true, false>), grid, dim3(CNT), 0, s, LU, lda, n, d, v1, c1);
         else
-            hipLaunchKernelGGL((chain_g_kernel<true, true>), dim3(nb - 1), dim3(CNT), 0, s, LU, lda, n, v1, g1);
+            hipLaunchKernelGGL((chain_g_kernel<true, true>), grid, dim3(CNT), 0, s, LU, lda, n, d, v1, c1);
     }
     return launch_check(h, "trsm chain preparation");
 }
@@ -379,6 +426,7 @@ int oxi_trsm_chain(oxi_handle_t h, hipStream_t s, int tk, int second, i64 n, i64
     ChainCtrl* c = static_cast<ChainCtrl*>(ws);
     const double* V = slot_ptr(ws, n, second ? 1 : 0);
     const double* G = slot_ptr(ws, n, second ? 3 : 2);
+    const double* H = slot_ptr(ws, n, second ? 5 : 4);
     hipLaunchKernelGGL(chain_ctrl_reset, dim3(1), dim3(64), 0, s, c);
     const dim3 grid(unsigned(nb * nchains)), block(CNT);
     // the first triangle runs forward, the second backward; op(T) as in the
@@ -386,16 +434,16 @@ int oxi_trsm_chain(oxi_handle_t h, hipStream_t s, int tk, int second, i64 n, i64
     const bool trans = second ? tk != 0 : tk == 1;
     if (!trans && !second)
         hipLaunchKernelGGL((trsm_chain_kernel<false, false>), grid, block, 0, s, LU, lda, n, B, ldb, int(nrhs), nchains,
-                           c, V, G);
+                           c, V, G, H);
     else if (!trans)
         hipLaunchKernelGGL((trsm_chain_kernel<true, false>), grid, block, 0, s, LU, lda, n, B, ldb, int(nrhs), nchains,
-                           c, V, G);
+                           c, V, G, H);
     else if (!second)
         hipLaunchKernelGGL((trsm_chain_kernel<false, true>), grid, block, 0, s, LU, lda, n, B, ldb, int(nrhs), nchains,
-                           c, V, G);
+                           c, V, G, H);
     else
         hipLaunchKernelGGL((trsm_chain_kernel<true, true>), grid, block, 0, s, LU, lda, n, B, ldb, int(nrhs), nchains,
-                           c, V, G);
+                           c, V, G, H);
     return launch_check(h, "trsm chain");
 }
 
