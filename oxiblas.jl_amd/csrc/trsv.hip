@@ -249,6 +249,10 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
         for (int k = 0; k < GPT; ++k) gr[k] = g[(part + 4 * k) * RB + i];
     }
     const int cend = G ? b - 1 : b;            // chunks summed before the final step
+    // this block's right-hand side, loaded before any wait (its rows are
+    // written only by this workgroup, at the end)
+    TV xb0 = TV(0);
+    if (t < RB && t < rows) xb0 = x[p0 + (BWD ? rows - 1 - t : t)];
 
     // 2. off-diagonal block row, chunk by chunk as earlier blocks publish
     TV acc[TRANS ? RPT : VEC];
@@ -369,7 +373,7 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
             TV sum = red[0][pr];
 #pragma unroll
             for (int g = 1; g < NRED; ++g) sum += red[g][pr];
-            r = x[p0 + pr] - sum;
+            r = xb0 - sum;
         }
         rv[i] = r;
     }
