@@ -254,6 +254,23 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
     TV xb0 = TV(0);
     if (t < RB && t < rows) xb0 = x[p0 + (BWD ? rows - 1 - t : t)];
 
+    // the last chunk summed before the final step (no transpose, vector
+    // loads): its part of the matrix loaded before any wait, so that no HBM
+    // round trip lies between that chunk's x and this block's final step
+    constexpr int NQ = RB / CG;
+    typedef typename Vec<TM>::type TMV;
+    TMV pre[(!TRANS && VECOK) ? NQ : 1];
+    const bool use_pre = !TRANS && VECOK && cend >= 1 && (t % LPC) * VEC < rows;
+    if constexpr (!TRANS && VECOK) {
+        if (use_pre) {
+            const i64 ql = BWD ? n - i64(cend) * RB : i64(cend - 1) * RB;
+            const int pr = (t % LPC) * VEC, cg = t / LPC;
+#pragma unroll
+            for (int i = 0; i < NQ; ++i)
+                pre[i] = *reinterpret_cast<const TMV*>(M + (p0 + pr) + (ql + cg + i * CG) * ld);
+        }
+    }
+
     // 2. off-diagonal block row, chunk by chunk as earlier blocks publish
     TV acc[TRANS ? RPT : VEC];
 #pragma unroll
@@ -302,6 +319,18 @@ __global__ __launch_bounds__(NT) void trsv_chain_kernel(const TM* __restrict__ M
         for (int k = 0; k < nbat; ++k) {
             const i64 q0 = BWD ? n - i64(c + k + 1) * RB : i64(c + k) * RB;   // chunk = full block
             const TV* xc = xs + k * RB;
+            if constexpr (!TRANS && VECOK) {
+                if (use_pre && c + k == cend - 1) {
+                    const int cg = t / LPC;
+#pragma unroll
+                    for (int i = 0; i < NQ; ++i) {
+                        const TV xv = xc[cg + i * CG];
+#pragma unroll
+                        for (int v = 0; v < VEC; ++v) acc[v] = fma_t(TV(pre[i][v]), xv, acc[v]);
+                    }
+                    continue;
+                }
+            }
             if constexpr (!TRANS) {
                 // T = M: block rows (contiguous) x chunk columns
                 const int pr = (t % LPC) * VEC, cg = t / LPC;
