@@ -151,7 +151,8 @@ class LU:
         self.ipiv_dev = ipiv_dev    # torch int64 tensor, 1-based global rows
         self.info = int(info)
         self._ipiv = None
-        self._plan = None           # solve plan of these factors, made by the first solve
+        self._plan = None           # solve plan of these factors, made by the second solve
+        self._solved = False        # a solve has run on these factors
 
     def _solve_plan(self):
         """The factors' solve plan (device permutation and the inverses of
@@ -243,9 +244,17 @@ class LU:
         Bd = as_dmatrix(B)
         if Bd.m != self.factors.m:
             raise DimensionMismatch(f"factorization has {self.factors.m} rows, right-hand side has {Bd.m}")
-        plan = self._solve_plan()
         h = _h()
-        rc = h._L.oxi_dgetrs_plan(h.ptr, plan, t.encode(), Bd.n, Bd.ptr, Bd.ld)
+        if self._plan is None and not self._solved:
+            # the first solve: oxi_dgetrs prepares in the handle's workspace;
+            # the plan, which keeps that preparation for later solves in
+            # buffers of its own, is made from the second solve on
+            self._solved = True
+            rc = h._L.oxi_dgetrs(h.ptr, t.encode(), Bd.m, Bd.n, self.factors.ptr, self.factors.ld,
+                                 self.ipiv_dev.data_ptr(), Bd.ptr, Bd.ld)
+        else:
+            plan = self._solve_plan()
+            rc = h._L.oxi_dgetrs_plan(h.ptr, plan, t.encode(), Bd.n, Bd.ptr, Bd.ld)
         h.check(rc, "ldiv!")
         if host:
             X = Bd.numpy()
