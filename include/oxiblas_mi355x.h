@@ -144,6 +144,13 @@ int oxi_dlaswp_noalloc(oxi_handle_t handle, int64_t ncols, double* A, int64_t ld
 int oxi_dgetrf(oxi_handle_t handle, int64_t m, int64_t n, double* A, int64_t lda, int64_t* ipiv, int64_t nb,
                int64_t* info);
 
+/* lu(A) = lu!(copy(A))  (src/api/linalg_api.jl:217): the same factorization
+ * with A untouched and the factors written to F (m x n, ldf, device), which
+ * must not overlap A (-6).  The copy is the factorization's own change of
+ * layout, so it costs no separate pass over the matrix. */
+int oxi_dgetrf_copy(oxi_handle_t handle, int64_t m, int64_t n, const double* A, int64_t lda, double* F,
+                    int64_t ldf, int64_t* ipiv, int64_t nb, int64_t* info);
+
 /* ldiv!(F, B) and ldiv!(transpose(F), B)  (src/api/linalg_api.jl:168-184):
  * solves op(A)*X = B in place with the factors of oxi_dgetrf.  trans 'N':
  * row interchanges, unit-lower then upper solve; 'T'/'C': upper-transposed,

@@ -473,9 +473,11 @@ int choose_nb(i64 m, i64 n) {
     return 64;
 }
 
+// src: with a source matrix (lds), A only receives the factors (src is read
+// once, by the change of layout); else A is factored in place.
 template <typename T>
 int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64* info, int prec,
-            bool tournament = false) {
+            bool tournament = false, const T* src = nullptr, i64 lds = 0) {
     (void)hipSetDevice(h->device);
     hipStream_t s = h->stream;
     const i64 q = std::min(m, n);
@@ -549,7 +551,23 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
     const i64 nblk = i64(tiles) * (tiles + 1) / 2;
     const i64 nb0 = std::min<i64>(nb_at(0), q);
     const bool early_panel = square && lookahead && q > 0;
-    if (square) {
+    if (square && src) {
+        // out of place: the first panel's rows of W first, then the rest
+        const i64 c0 = early_panel ? nb0 : n;
+        hipLaunchKernelGGL(transpose_kernel<T>, dim3(unsigned(cdiv(n, TT)), unsigned(cdiv(c0, TT))), dim3(256), 0, s,
+                           src, lds, A, lda, n, c0);
+        OXI_TRY(launch_check(h, "transpose"));
+        if (early_panel) {
+            OXI_HIP(h, hipEventRecord(h->ev_a, s));
+            OXI_HIP(h, hipStreamWaitEvent(h->side, h->ev_a, 0));
+            OXI_TRY(panel_top(gs, 0, nb0));
+            OXI_HIP(h, hipEventRecord(h->ev_b, h->side));
+            panel_on_side = true;
+        }
+        if (c0 < n)
+            hipLaunchKernelGGL(transpose_kernel<T>, dim3(unsigned(cdiv(n, TT)), unsigned(cdiv(n - c0, TT))), dim3(256),
+                               0, s, src + c0 * lds, lds, A + c0, lda, n, n - c0);
+    } else if (square) {
         const i64 k0 = early_panel ? std::min<i64>(cdiv(nb0, TT), tiles) : tiles;
         const i64 ids1 = k0 * tiles - k0 * (k0 - 1) / 2;      // the pairs of tile rows < k0
         hipLaunchKernelGGL(transpose_square_kernel<T>, dim3(unsigned(ids1)), dim3(256), 0, s, A, lda, n, tiles,
@@ -566,8 +584,8 @@ int getrf_t(oxi_handle_t h, i64 m, i64 n, T* A, i64 lda, i64* ipiv, i64 nb, i64*
             hipLaunchKernelGGL(transpose_square_kernel<T>, dim3(unsigned(nblk - ids1)), dim3(256), 0, s, A, lda, n,
                                tiles, ids1);
     } else {
-        hipLaunchKernelGGL(transpose_kernel<T>, dim3(unsigned(cdiv(m, TT)), unsigned(cdiv(n, TT))), dim3(256), 0, s, A,
-                           lda, g.Wm, ldw, m, n);
+        hipLaunchKernelGGL(transpose_kernel<T>, dim3(unsigned(cdiv(m, TT)), unsigned(cdiv(n, TT))), dim3(256), 0, s,
+                           src ? src : A, src ? lds : lda, g.Wm, ldw, m, n);
     }
     OXI_TRY(launch_check(h, "transpose"));
 
@@ -750,6 +768,28 @@ extern "C" int oxi_dgetrf(oxi_handle_t h, int64_t m, int64_t n, double* A, int64
     if (!A) return -4;
     if (!ipiv) return -6;
     return getrf_t<double>(h, m, n, A, lda, ipiv, nb, info, -1);
+}
+
+extern "C" int oxi_dgetrf_copy(oxi_handle_t h, int64_t m, int64_t n, const double* A, int64_t lda, double* F,
+                               int64_t ldf, int64_t* ipiv, int64_t nb, int64_t* info) {
+    if (!h) return -1;
+    if (m < 0) return -2;
+    if (n < 0) return -3;
+    if (lda < std::max<i64>(1, m)) return -5;
+    if (ldf < std::max<i64>(1, m)) return -7;
+    if (nb < 0) return -9;
+    if (!info) return -10;
+    std::lock_guard<std::recursive_mutex> lk(h->guard);
+    *info = 0;
+    if (m == 0 || n == 0) return OXI_OK;
+    if (!A) return -4;
+    if (!F) return -6;
+    if (!ipiv) return -8;
+    // the element ranges of A and F must be disjoint
+    const uintptr_t a0 = reinterpret_cast<uintptr_t>(A), a1 = reinterpret_cast<uintptr_t>(A + (lda * (n - 1) + m));
+    const uintptr_t f0 = reinterpret_cast<uintptr_t>(F), f1 = reinterpret_cast<uintptr_t>(F + (ldf * (n - 1) + m));
+    if (a0 < f1 && f0 < a1) return -6;
+    return getrf_t<double>(h, m, n, F, ldf, ipiv, nb, info, -1, false, A, lda);
 }
 
 static int sgetrf_entry(oxi_handle_t h, int prec, int64_t m, int64_t n, float* A, int64_t lda, int64_t* ipiv,

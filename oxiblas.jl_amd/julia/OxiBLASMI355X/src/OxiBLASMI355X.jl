@@ -197,7 +197,23 @@ end
 Partial-pivoted blocked LU of a copy of A (OxiBLAS.lu, src/api/linalg_api.jl:217).
 """
 lu(A::StridedMatrix{Float64}; check::Bool=true, blocksize::Integer=0) = _getrf!(DeviceMatrix(A); check, blocksize)
-lu(A::DeviceMatrix; check::Bool=true, blocksize::Integer=0) = _getrf!(_copy(A); check, blocksize)
+function lu(A::DeviceMatrix; check::Bool=true, blocksize::Integer=0)
+    # the copy is written by the factorization's change of layout (oxi_dgetrf_copy)
+    h = handle()
+    F = DeviceMatrix(A.m, A.n)
+    q = min(A.m, A.n)
+    piv = Ref{Ptr{Cvoid}}(C_NULL)
+    _check(h, ccall((:oxi_malloc, LIB), Cint, (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, UInt64), h, piv, 8 * max(q, 1)), "malloc")
+    info = Ref{Int64}(0)
+    rc = ccall((:oxi_dgetrf_copy, LIB), Cint,
+               (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Int64}, Int64, Ptr{Int64}),
+               h, A.m, A.n, A.ptr, A.ld, F.ptr, F.ld, Ptr{Int64}(piv[]), blocksize, info)
+    _check(h, rc, "lu")
+    Fl = LU(F, Ptr{Int64}(piv[]), Int(info[]))
+    finalizer(f -> ccall((:oxi_free, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), handle(), getfield(f, :ipiv)), Fl)
+    check && info[] != 0 && throw(SingularException(Int(info[])))
+    Fl
+end
 
 """
     lu!(A::DeviceMatrix; check=true) -> LU

@@ -94,6 +94,7 @@ SIGNATURES = {
     "oxi_dlaswp": (C.c_int, [_vp, _i64, _vp, _i64, _i64, _i64, _vp, C.c_int]),
     "oxi_dlaswp_noalloc": (C.c_int, [_vp, _i64, _vp, _i64, _i64, _i64, _vp, C.c_int]),
     "oxi_dgetrf": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, C.POINTER(_i64)]),
+    "oxi_dgetrf_copy": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, C.POINTER(_i64)]),
     "oxi_dgetrs": (C.c_int, [_vp, _ch, _i64, _i64, _vp, _i64, _vp, _vp, _i64]),
     "oxi_dgetrs_plan_create": (C.c_int, [_vp, _i64, _vp, _i64, _vp, C.POINTER(_vp)]),
     "oxi_dgetrs_plan": (C.c_int, [_vp, _vp, _ch, _i64, _vp, _i64]),

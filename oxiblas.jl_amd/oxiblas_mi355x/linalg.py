@@ -358,11 +358,25 @@ def _getrf(Ad, check, blocksize):
 
 
 def lu(A, check=True, blocksize=0):
-    """OxiBLAS.lu(A; check): factors a copy of A (src/api/linalg_api.jl:217)."""
-    Ad = as_dmatrix(A)
-    if isinstance(A, DMatrix):
-        Ad = A.copy()
-    return _getrf(Ad, check, blocksize)
+    """OxiBLAS.lu(A; check): factors a copy of A (src/api/linalg_api.jl:217).
+    A device matrix is read once by oxi_dgetrf_copy, whose change of layout
+    writes the copy; a host matrix is copied to the device and factored
+    there."""
+    if not isinstance(A, DMatrix):
+        return _getrf(as_dmatrix(A), check, blocksize)
+    import torch
+    h = _h()
+    F = DMatrix.empty(A.m, A.n, device=A.buf.device)
+    q = min(A.m, A.n)
+    ipiv = torch.zeros(max(q, 1), dtype=torch.int64, device=A.buf.device)
+    info = C.c_int64(0)
+    rc = h._L.oxi_dgetrf_copy(h.ptr, A.m, A.n, A.ptr, A.ld, F.ptr, F.ld, ipiv.data_ptr(), int(blocksize),
+                              C.byref(info))
+    h.check(rc, "lu")
+    Fl = LU(F, ipiv[:q], info.value)
+    if check and info.value != 0:
+        raise SingularException(info.value)
+    return Fl
 
 
 def lu_(A, check=True, blocksize=0):

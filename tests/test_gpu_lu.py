@@ -301,6 +301,32 @@ def test_64_rhs_solves_n4096(gpu, rng):
         assert np.linalg.norm(opA @ X - B) <= 30 * n * EPS * np.linalg.norm(A) * np.linalg.norm(X)
 
 
+@pytest.mark.parametrize("m,n", ((1, 1), (33, 20), (20, 33), (300, 300), (1000, 1000), (4096, 4096), (2100, 1500)))
+def test_lu_copy_equals_in_place(gpu, rng, m, n):
+    """lu(A) through oxi_dgetrf_copy (the copy made by the factorization's
+    change of layout) gives bitwise the factors, pivots and info of lu! on a
+    copy (oxi_dgetrf), with padded leading dimensions; A stays untouched and
+    an output overlapping A is rejected."""
+    import torch
+    from oxiblas_mi355x import _lib
+    ox = gpu
+    A = rng.standard_normal((m, n))
+    if min(m, n) > 10:
+        A[:, 7] = 0                                          # a zero pivot column: info
+    Ad = ox.DMatrix.from_numpy(A, ld=m + 3)
+    F1 = ox.lu(Ad, check=False)                              # oxi_dgetrf_copy
+    assert np.array_equal(Ad.numpy(), A)
+    F2 = ox.lu_(ox.DMatrix.from_numpy(A, ld=m + 5), check=False)
+    assert np.array_equal(F1.factors.numpy(), F2.factors.numpy())
+    assert np.array_equal(F1.ipiv, F2.ipiv) and F1.info == F2.info
+    h = _lib.handle()
+    info = __import__("ctypes").c_int64(0)
+    piv = torch.zeros(max(min(m, n), 1), dtype=torch.int64, device="cuda")
+    rc = h._L.oxi_dgetrf_copy(h.ptr, m, n, Ad.ptr, Ad.ld, Ad.ptr, Ad.ld, piv.data_ptr(), 0,
+                              __import__("ctypes").byref(info))
+    assert rc == -6 and np.array_equal(Ad.numpy(), A)
+
+
 @pytest.mark.parametrize("n", (1, 63, 64, 65, 200, 1000))
 def test_chained_multi_rhs_solves(gpu, rng, n):
     """The chained multi-RHS solve (5 to 64 right-hand sides, both solve
