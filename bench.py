@@ -393,8 +393,11 @@ def main():
         extra["getrs_64rhs_ms"] = round(ts * 1e3, 3)
         extra["getrs_64rhs_gflops"] = round(2.0 * n * n * 64 / ts / 1e9, 1)
         log(f"getrs 64 rhs n={n}: {ts * 1e3:.3f} ms (first solve, with the plan: {extra['getrs_64rhs_first_ms']} ms)")
-        # a second solve with the same factors reuses their solve plan
+        # a repeated one-RHS solve on the same factors reuses their solve plan
+        # (made by the factors' second solve, its one-RHS part by the first
+        # one-RHS solve through it)
         b1 = ox.DMatrix(b0.clone(), n, 1)
+        F.ldiv_(b1.copy())
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         F.ldiv_(b1)
