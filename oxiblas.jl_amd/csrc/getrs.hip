@@ -73,10 +73,11 @@ bool use_chain(i64 n, i64 nrhs, i64 ldb) { return nrhs > OXI_TRSV_MAX_RHS && oxi
 // else the blocked trsm.
 template <typename T>
 int triangles(oxi_handle_t h, hipStream_t s, int tk, i64 n, i64 nrhs, const T* LU, i64 lda, T* B, i64 ldb,
-              void* ctrl, void* mws) {
+              void* ctrl, void* mws, hipEvent_t second_ready) {
     if (ctrl && nrhs <= OXI_TRSV_MAX_RHS) {
         for (i64 j = 0; j < nrhs; ++j) {
             OXI_TRY(oxi_trsv(h, s, tk ? 2 : 0, n, LU, lda, B + j * ldb, ctrl));
+            if (second_ready && j == 0) OXI_HIP(h, hipStreamWaitEvent(s, second_ready, 0));
             OXI_TRY(oxi_trsv(h, s, tk ? 3 : 1, n, LU, lda, B + j * ldb, ctrl));
         }
         return OXI_OK;
@@ -100,17 +101,17 @@ int triangles(oxi_handle_t h, hipStream_t s, int tk, i64 n, i64 nrhs, const T* L
 // no synchronisation.
 template <typename T>
 int getrs_perm(oxi_handle_t h, hipStream_t s, int tk, i64 n, i64 nrhs, const T* LU, i64 lda, const i64* dperm, T* B,
-               i64 ldb, T* Y, void* ctrl, void* mws = nullptr) {
+               i64 ldb, T* Y, void* ctrl, void* mws = nullptr, hipEvent_t second_ready = nullptr) {
     const unsigned pg = unsigned(std::min<i64>(cdiv(n * nrhs, 256), 8192));
     if (!tk) {
         // B <- B[p, :] ; L \ ; U \   (src/api/linalg_api.jl:168-174)
         hipLaunchKernelGGL(permute_rows_kernel<T>, dim3(pg), dim3(256), 0, s, B, ldb, Y, n, n, nrhs, dperm, 0);
         hipLaunchKernelGGL(copy_cols_kernel<T>, dim3(pg), dim3(256), 0, s, Y, n, B, ldb, n, nrhs);
         OXI_TRY(launch_check(h, "permute"));
-        OXI_TRY(triangles<T>(h, s, 0, n, nrhs, LU, lda, B, ldb, ctrl, mws));
+        OXI_TRY(triangles<T>(h, s, 0, n, nrhs, LU, lda, B, ldb, ctrl, mws, second_ready));
     } else {
         // U^T \ ; L^T \ ; inverse interchanges   (src/api/linalg_api.jl:176-184)
-        OXI_TRY(triangles<T>(h, s, 1, n, nrhs, LU, lda, B, ldb, ctrl, mws));
+        OXI_TRY(triangles<T>(h, s, 1, n, nrhs, LU, lda, B, ldb, ctrl, mws, second_ready));
         hipLaunchKernelGGL(permute_rows_kernel<T>, dim3(pg), dim3(256), 0, s, B, ldb, Y, n, n, nrhs, dperm, 1);
         hipLaunchKernelGGL(copy_cols_kernel<T>, dim3(pg), dim3(256), 0, s, Y, n, B, ldb, n, nrhs);
         OXI_TRY(launch_check(h, "permute"));
@@ -165,15 +166,22 @@ int getrs_api(oxi_handle_t h, char trans, i64 n, i64 nrhs, const T* LU, i64 lda,
     i64* dperm = (i64*)((char*)ws + xpad);
     void* ctrl = (char*)ws + xpad + ppad;
     OXI_HIP(h, hipMemcpyAsync(dperm, perm.data(), size_t(n) * 8, hipMemcpyHostToDevice, s));
+    hipEvent_t second_ready = nullptr;
     if (chain) {
         OXI_TRY(oxi_trsv_begin(h, s, ctrl));
-        OXI_TRY(oxi_trsv_prepare(h, s, tk, n, LU, lda, (T*)nullptr, ctrl));
+        if constexpr (sizeof(T) == 8) {
+            OXI_TRY(oxi_trsv_prepare_split(h, s, tk, n, (const double*)LU, lda, ctrl));
+            second_ready = h->ev_b;
+        } else {
+            OXI_TRY(oxi_trsv_prepare(h, s, tk, n, LU, lda, (T*)nullptr, ctrl));
+        }
     }
     if (mchain) {
         OXI_TRY(oxi_trsm_chain_begin(h, s, ctrl));
         OXI_TRY(oxi_trsm_chain_prepare(h, s, tk, n, (const double*)LU, lda, ctrl));
     }
-    OXI_TRY(getrs_perm<T>(h, s, tk, n, nrhs, LU, lda, dperm, B, ldb, Y, chain ? ctrl : nullptr, mchain ? ctrl : nullptr));
+    OXI_TRY(getrs_perm<T>(h, s, tk, n, nrhs, LU, lda, dperm, B, ldb, Y, chain ? ctrl : nullptr, mchain ? ctrl : nullptr,
+                          second_ready));
     if (chain) return oxi_trsv_check(h, s, ctrl);    // the chain kernel ran: its status
     if (mchain) return oxi_trsm_chain_check(h, s, ctrl);
     OXI_HIP(h, hipStreamSynchronize(s));

@@ -168,6 +168,10 @@ int oxi_trsv_begin(oxi_handle_t h, hipStream_t s, void* ctrl);
 // uses them.
 size_t oxi_trsv_ws_bytes(i64 n, size_t tv);
 int oxi_trsv_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const double* M, i64 ld, double*, void* ws);
+// the same (FP64) with the second triangle's part on the handle's side
+// stream: its end is recorded in h->ev_b, which the second triangle's solve
+// on s must wait for
+int oxi_trsv_prepare_split(oxi_handle_t h, hipStream_t s, int tk, i64 n, const double* M, i64 ld, void* ws);
 int oxi_trsv_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const float* M, i64 ld, float*, void* ws);
 int oxi_trsv_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const float* M, i64 ld, double*, void* ws);
 // LU solves with at most this many right-hand sides go column by column

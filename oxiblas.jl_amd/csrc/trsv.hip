@@ -503,31 +503,41 @@ TV* slot_of(void* ws, i64 n, int slot) {
 }
 
 template <typename TM, typename TV>
-int prepare_t(oxi_handle_t h, hipStream_t s, int tk, i64 n, const TM* M, i64 ld, void* ws) {
+int prepare_t(oxi_handle_t h, hipStream_t s, int tk, i64 n, const TM* M, i64 ld, void* ws, bool split = false) {
     if (n <= 0) return OXI_OK;
     const unsigned grid = unsigned(cdiv(n, RB));
+    // split: the second triangle's part on the side stream (it starts after
+    // the work already on s; its end is recorded in ev_b), so that it runs
+    // beside the first triangle's solve
+    hipStream_t s2 = s;
+    if (split) {
+        OXI_HIP(h, hipEventRecord(h->ev_a, s));
+        OXI_HIP(h, hipStreamWaitEvent(h->side, h->ev_a, 0));
+        s2 = h->side;
+    }
     TV* i0 = slot_of<TV>(ws, n, 0);
     TV* i1 = slot_of<TV>(ws, n, 1);
     TV* g0 = slot_of<TV>(ws, n, 2);
     TV* g1 = slot_of<TV>(ws, n, 3);
     if (!tk) {
         hipLaunchKernelGGL((trsv_dinv_kernel<TM, TV, false, true, false>), dim3(grid), dim3(NT), 0, s, M, ld, n, i0);
-        hipLaunchKernelGGL((trsv_dinv_kernel<TM, TV, false, false, true>), dim3(grid), dim3(NT), 0, s, M, ld, n, i1);
+        hipLaunchKernelGGL((trsv_dinv_kernel<TM, TV, false, false, true>), dim3(grid), dim3(NT), 0, s2, M, ld, n, i1);
     } else {
         hipLaunchKernelGGL((trsv_dinv_kernel<TM, TV, true, false, false>), dim3(grid), dim3(NT), 0, s, M, ld, n, i0);
-        hipLaunchKernelGGL((trsv_dinv_kernel<TM, TV, true, true, true>), dim3(grid), dim3(NT), 0, s, M, ld, n, i1);
+        hipLaunchKernelGGL((trsv_dinv_kernel<TM, TV, true, true, true>), dim3(grid), dim3(NT), 0, s2, M, ld, n, i1);
     }
     if (grid > 1) {
         if (!tk) {
             hipLaunchKernelGGL((trsv_g_kernel<TM, TV, false, false>), dim3(grid - 1), dim3(NT), 0, s, M, ld, n, i0, g0);
-            hipLaunchKernelGGL((trsv_g_kernel<TM, TV, false, true>), dim3(grid - 1), dim3(NT), 0, s, M, ld, n, i1, g1);
+            hipLaunchKernelGGL((trsv_g_kernel<TM, TV, false, true>), dim3(grid - 1), dim3(NT), 0, s2, M, ld, n, i1, g1);
         } else {
             hipLaunchKernelGGL((trsv_g_kernel<TM, TV, true, false>), dim3(grid - 1), dim3(NT), 0, s, M, ld, n, i0, g0);
-            hipLaunchKernelGGL((trsv_g_kernel<TM, TV, true, true>), dim3(grid - 1), dim3(NT), 0, s, M, ld, n, i1, g1);
+            hipLaunchKernelGGL((trsv_g_kernel<TM, TV, true, true>), dim3(grid - 1), dim3(NT), 0, s2, M, ld, n, i1, g1);
         }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "trsv prepare launch: %s", hipGetErrorString(e));
+    if (split) OXI_HIP(h, hipEventRecord(h->ev_b, s2));
     return OXI_OK;
 }
 
@@ -561,6 +571,9 @@ size_t oxi_trsv_ws_bytes(i64 n, size_t tv) { return 256 + 4 * slot_bytes(std::ma
 
 int oxi_trsv_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const double* M, i64 ld, double*, void* ws) {
     return prepare_t<double, double>(h, s, tk, n, M, ld, ws);
+}
+int oxi_trsv_prepare_split(oxi_handle_t h, hipStream_t s, int tk, i64 n, const double* M, i64 ld, void* ws) {
+    return prepare_t<double, double>(h, s, tk, n, M, ld, ws, true);
 }
 int oxi_trsv_prepare(oxi_handle_t h, hipStream_t s, int tk, i64 n, const float* M, i64 ld, float*, void* ws) {
     return prepare_t<float, float>(h, s, tk, n, M, ld, ws);
