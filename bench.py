@@ -247,6 +247,28 @@ def run_part(ox, torch, part, n, reps=3):
         C = ox.DMatrix(torch.randn(n * n, dtype=torch.float64, device=dev, generator=g), n, n)
         work = lambda: ox.gemm_(C, A, B, -1.0, 1.0)             # noqa: E731
         reps = 10
+    elif part == "lowp_gemm":
+        # the mixed LU's low-precision trailing product at its update shape:
+        # C (16384 x 16384, FP32) -= round(A) (16384 x 256) round(B), per low
+        # type, operand rounding included; TF/s against the dense peaks
+        A = ox.DMatrix(torch.randn(n * 256, dtype=torch.float32, device=dev, generator=g), n, 256)
+        B = ox.DMatrix(torch.randn(256 * n, dtype=torch.float32, device=dev, generator=g), 256, n)
+        C = ox.DMatrix(torch.randn(n * n, dtype=torch.float32, device=dev, generator=g), n, n)
+        peak = {"bfloat16": 2.5e15, "float16": 2.5e15, "float8_e4m3fn": 5.0e15}
+        for low in ("bfloat16", "float16", "float8_e4m3fn"):
+            ox.gemm_lowp_update_(C, A, B, lowtype=low)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(10):
+                ox.gemm_lowp_update_(C, A, B, lowtype=low)
+            torch.cuda.synchronize()
+            t = (time.perf_counter() - t0) / 10
+            tf = 2.0 * n * n * 256 / t
+            # HBM: the FP32 C read and written once (the operands are 16 MB)
+            gbs = 2.0 * n * n * 4 / t / 1e9
+            log(f"lowp_gemm {low} {n}x{n}x256: {t * 1e3:.3f} ms {tf / 1e12:.1f} TFLOP/s "
+                f"({100 * tf / peak[low]:.1f} % of the dense peak), C at {gbs:.0f} GB/s")
+        return
     elif part == "cholesky":
         S = ox.DMatrix(torch.randn(n * n, dtype=torch.float64, device=dev, generator=g), n, n)
         St = S.buf.view(n, n)
@@ -296,7 +318,7 @@ def main():
     ap.add_argument("--traffic-summary", nargs=5, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV",
                                                            "COPY_WRITE_CSV", "OUT_JSON"),
                     help="write the HBM traffic per dgemm_m16_kernel launch from counter runs and exit")
-    ap.add_argument("--part", choices=("cholesky", "getrs64", "trsv", "mixed32", "mixed32tp", "lowp", "gemm"),
+    ap.add_argument("--part", choices=("cholesky", "getrs64", "trsv", "mixed32", "mixed32tp", "lowp", "gemm", "lowp_gemm"),
                     default=None,
                     help="only run one extra part a few times (for rocprofv3 summaries of its kernels) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",

@@ -67,12 +67,18 @@ __device__ __forceinline__ void store_rows(unsigned char* __restrict__ S, const 
 }
 
 // PREC: 0 = BF16, 1 = FP16, 2 = FP8 e4m3
+// vec: ldc % 4 == 0 and C 16-byte aligned.  A tile inside C then reads and
+// writes each lane's four rows of a column (D items r = 0..3, consecutive
+// rows) as one 16-byte streaming access: at k = nb the C read and write are
+// the kernel's HBM traffic.  (Loading C before the operand loads, to overlap
+// it with the products, measured no faster and spills.)
 template <int PREC>
-__global__ __launch_bounds__(NT) void lowp_update_kernel(i64 M, i64 N, i64 Kp, float alpha,
-                                                         const unsigned char* __restrict__ A,
-                                                         const unsigned char* __restrict__ B,
-                                                         const float* __restrict__ sa, const float* __restrict__ sb,
-                                                         float* __restrict__ C, i64 ldc, int tiles_m, int tiles_n) {
+__global__ __launch_bounds__(NT, 2) void lowp_update_kernel(i64 M, i64 N, i64 Kp, float alpha,
+                                                            const unsigned char* __restrict__ A,
+                                                            const unsigned char* __restrict__ B,
+                                                            const float* __restrict__ sa, const float* __restrict__ sb,
+                                                            float* __restrict__ C, i64 ldc, int tiles_m, int tiles_n,
+                                                            int vec) {
     __shared__ __attribute__((aligned(16))) unsigned char As[2][BM * ROW_STRIDE];
     __shared__ __attribute__((aligned(16))) unsigned char Bs[2][BN * ROW_STRIDE];
     constexpr int ES = PREC == 2 ? 1 : 2;  // element size
@@ -83,6 +89,9 @@ __global__ __launch_bounds__(NT) void lowp_update_kernel(i64 M, i64 N, i64 Kp, f
     const i64 m0 = i64(tm) * BM, n0 = i64(tn) * BN;
     const i64 ld_bytes = Kp * ES;
     const int nslab = int(ld_bytes / SLAB_BYTES);
+
+    const int li = lane & 15, lg = lane >> 4;
+    const bool full = vec && m0 + BM <= M && n0 + BN <= N;
 
     f4v acc[4][4];
 #pragma unroll
@@ -96,7 +105,6 @@ __global__ __launch_bounds__(NT) void lowp_update_kernel(i64 M, i64 N, i64 Kp, f
     store_rows(As[0], ra, t);
     store_rows(Bs[0], rb, t);
     __syncthreads();
-    const int li = lane & 15, lg = lane >> 4;
     for (int s = 0; s < nslab; ++s) {
         const int cur = s & 1;
         if (s + 1 < nslab) {
@@ -158,11 +166,36 @@ __global__ __launch_bounds__(NT) void lowp_update_kernel(i64 M, i64 N, i64 Kp, f
         }
         __syncthreads();
     }
-    // epilogue: C += alpha * sa(row block) * sb(col block) * acc.  All loads
-    // (C and the scales) are issued before the first store, without branches
-    // (an element outside C reads the nearest element inside it and is not
-    // stored): a load behind a bounds branch is waited for before the next
-    // one is issued, one memory round trip per element.
+    // epilogue: C += alpha * sa(row block) * sb(col block) * acc
+    if (full) {
+        f4v cq[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                cq[i][j] = __builtin_nontemporal_load(
+                    (const f4v*)(C + (n0 + wn * 64 + j * 16 + li) * ldc + m0 + wm * 64 + i * 16 + lg * 4));
+        // a 256-row scale block holds the whole 64-row wave tile (m0 and
+        // wm * 64 are multiples of 64), likewise for the columns
+        const float rs = sa ? sa[(m0 + wm * 64) >> 8] : 1.0f;
+        const float cs = alpha * (sb ? sb[(n0 + wn * 64) >> 8] : 1.0f);
+        const float w = rs * cs;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                f4v v;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) v[r] = __builtin_fmaf(w, acc[i][j][r], cq[i][j][r]);
+                __builtin_nontemporal_store(v, (f4v*)(C + (n0 + wn * 64 + j * 16 + li) * ldc + m0 + wm * 64 + i * 16 + lg * 4));
+            }
+        return;
+    }
+    // edge tiles and unaligned C: all loads (C and the scales) are issued
+    // before the first store, without branches (an element outside C reads
+    // the nearest element inside it and is not stored): a load behind a bounds
+    // branch is waited for before the next one is issued, one memory round
+    // trip per element.
     float cv[4][4][4], rsv[4][4], csv[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -392,7 +425,7 @@ extern "C" int oxi_hgemm(oxi_handle_t h, char transA, char transB, int64_t m, in
     const int tiles_m = int(cdiv(m, BM)), tiles_n = int(cdiv(n, BN));
     hipLaunchKernelGGL(lowp_update_kernel<1>, dim3(unsigned(tiles_m * tiles_n)), dim3(NT), 0, s, m, n, Kp, 1.0f,
                        (const unsigned char*)Ap, (const unsigned char*)Bp, (const float*)nullptr, (const float*)nullptr,
-                       acc, m, tiles_m, tiles_n);
+                       acc, m, tiles_m, tiles_n, int(m % 4 == 0));
     hipLaunchKernelGGL(hgemm_epilogue_kernel, dim3(eg), dim3(256), 0, s, acc, m, n, float(ah), float(bh),
                        int(bh == _Float16(0)), C, ldc);
     hipError_t e = hipGetLastError();
@@ -437,12 +470,13 @@ int oxi_lowp_update(oxi_handle_t h, hipStream_t s, int prec, i64 M, i64 N, i64 K
     }
     const int tiles_m = int(cdiv(M, BM)), tiles_n = int(cdiv(N, BN));
     dim3 grid(unsigned(tiles_m * tiles_n));
+    const int vec = int(ldc % 4 == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0);
     if (prec == 0)
-        hipLaunchKernelGGL(lowp_update_kernel<0>, grid, dim3(NT), 0, s, M, N, Kp, alpha, Ap, Bp, sa, sb, C, ldc, tiles_m, tiles_n);
+        hipLaunchKernelGGL(lowp_update_kernel<0>, grid, dim3(NT), 0, s, M, N, Kp, alpha, Ap, Bp, sa, sb, C, ldc, tiles_m, tiles_n, vec);
     else if (prec == 1)
-        hipLaunchKernelGGL(lowp_update_kernel<1>, grid, dim3(NT), 0, s, M, N, Kp, alpha, Ap, Bp, sa, sb, C, ldc, tiles_m, tiles_n);
+        hipLaunchKernelGGL(lowp_update_kernel<1>, grid, dim3(NT), 0, s, M, N, Kp, alpha, Ap, Bp, sa, sb, C, ldc, tiles_m, tiles_n, vec);
     else
-        hipLaunchKernelGGL(lowp_update_kernel<2>, grid, dim3(NT), 0, s, M, N, Kp, alpha, Ap, Bp, sa, sb, C, ldc, tiles_m, tiles_n);
+        hipLaunchKernelGGL(lowp_update_kernel<2>, grid, dim3(NT), 0, s, M, N, Kp, alpha, Ap, Bp, sa, sb, C, ldc, tiles_m, tiles_n, vec);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return oxi_set_error(h, OXI_ERR_HIP, "lowp update launch: %s", hipGetErrorString(e));
     return OXI_OK;

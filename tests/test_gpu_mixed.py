@@ -253,6 +253,26 @@ def test_lowp_update_vs_emulation(gpu, rng, prec, M, N, K):
 
 
 @pytest.mark.parametrize("prec", ["bfloat16", "float16", "float8_e4m3fn"])
+def test_lowp_update_aligned_and_unaligned_c_agree(gpu, rng, prec):
+    """Tiles inside an aligned C (ld % 4 == 0, 16-byte base) take 16-byte C
+    accesses; the same update on an unaligned C (odd ld, or a base one float
+    off) takes per-element accesses.  Both must give the same bits."""
+    M, N, K = 640, 520, 256
+    A = gpu.DMatrix.from_numpy(rng.standard_normal((M, K)).astype(np.float32), dtype=np.float32)
+    B = gpu.DMatrix.from_numpy((rng.standard_normal((K, N)) * 4).astype(np.float32), dtype=np.float32)
+    C0 = rng.standard_normal((M, N)).astype(np.float32)
+    outs = []
+    for ld, off in ((M, 0), (M + 1, 0), (M + 4, 1)):
+        buf = torch.zeros(off + ld * N, dtype=torch.float32, device="cuda")
+        buf[off:].view(N, ld)[:, :M].copy_(torch.from_numpy(C0.T.copy()))
+        Cd = gpu.DMatrix(buf, M, N, ld, off)
+        gpu.gemm_lowp_update_(Cd, A, B, prec, -0.75)
+        outs.append(Cd.numpy())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    assert not np.array_equal(outs[0], C0)
+
+
+@pytest.mark.parametrize("prec", ["bfloat16", "float16", "float8_e4m3fn"])
 def test_lowp_lu_is_a_good_preconditioner(gpu, rng, prec):
     n = 1024
     A = (rng.standard_normal((n, n)) / np.sqrt(n) + 2 * np.eye(n)).astype(np.float32)
