@@ -283,23 +283,26 @@ __global__ __launch_bounds__(256) void convert_cols_kernel(const float* __restri
 
 // Power-of-two scale per 256-block along one dimension of a column-major
 // block: rows (dim 0) or columns (dim 1); scale = 2^e with max/scale <= 448.
-__global__ __launch_bounds__(256) void block_scale_kernel(const float* __restrict__ X, i64 ldx, i64 rows, i64 cols,
-                                                          int dim, float* __restrict__ scale) {
+// Three steps, so that the maximum is spread over many workgroups:
+// block_max_kernel (blockIdx.x = the 256-block, blockIdx.y = a slice of `cs`
+// columns of it) folds |X| with fmaxf, which skips NaN, so each partial is a
+// non-negative float or +inf, whose bit pattern orders like its value: an
+// unsigned atomicMax on the bits, from zero, gives the block's maximum.
+// scale_from_max_kernel then turns each maximum into its scale in place.
+__global__ __launch_bounds__(256) void block_max_kernel(const float* __restrict__ X, i64 ldx, i64 rows, i64 cols,
+                                                        int dim, int cs, unsigned* __restrict__ bits) {
     __shared__ float red[256];
     const i64 b = blockIdx.x;
     float mx = 0.0f;
     if (dim == 0) {
-        const i64 r0 = b * 256;
-        for (i64 idx = threadIdx.x; idx < 256 * cols; idx += blockDim.x) {
-            const i64 c = idx / 256, r = r0 + idx % 256;
-            if (r < rows) mx = fmaxf(mx, fabsf(X[r + c * ldx]));
-        }
+        const i64 r = b * 256 + threadIdx.x;
+        const i64 ca = i64(blockIdx.y) * cs, ce = min(cols, ca + cs);
+        if (r < rows)
+            for (i64 c = ca; c < ce; ++c) mx = fmaxf(mx, fabsf(X[r + c * ldx]));
     } else {
-        const i64 c0 = b * 256;
-        for (i64 idx = threadIdx.x; idx < 256 * rows; idx += blockDim.x) {
-            const i64 c = c0 + idx / rows, r = idx % rows;
-            if (c < cols) mx = fmaxf(mx, fabsf(X[r + c * ldx]));
-        }
+        const i64 ca = b * 256 + i64(blockIdx.y) * cs, ce = min(min(cols, b * 256 + 256), ca + cs);
+        for (i64 c = ca; c < ce; ++c)
+            for (i64 r = threadIdx.x; r < rows; r += 256) mx = fmaxf(mx, fabsf(X[r + c * ldx]));
     }
     red[threadIdx.x] = mx;
     __syncthreads();
@@ -307,16 +310,19 @@ __global__ __launch_bounds__(256) void block_scale_kernel(const float* __restric
         if (threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        const float m = red[0];
-        float s = 1.0f;
-        if (m > 0.0f && m == m) {
-            int e;
-            frexpf(m / 448.0f, &e);      // m/448 = f * 2^e, 0.5 <= f < 1
-            s = ldexpf(1.0f, e);         // m / s <= 448
-        }
-        scale[b] = s;
+    if (threadIdx.x == 0) atomicMax(bits + b, __float_as_uint(red[0]));
+}
+__global__ void scale_from_max_kernel(float* __restrict__ scale, i64 count) {
+    const i64 i = i64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float m = scale[i];
+    float s = 1.0f;
+    if (m > 0.0f && m == m) {
+        int e;
+        frexpf(m / 448.0f, &e);      // m/448 = f * 2^e, 0.5 <= f < 1
+        s = ldexpf(1.0f, e);         // m / s <= 448
     }
+    scale[i] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -451,8 +457,16 @@ int oxi_lowp_update(oxi_handle_t h, hipStream_t s, int prec, i64 M, i64 N, i64 K
     float* sa = (float*)(Bp + ((bbytes + 255) & ~size_t(255)));
     float* sb = sa + nsa;
     if (prec == 2) {
-        hipLaunchKernelGGL(block_scale_kernel, dim3(unsigned(nsa)), dim3(256), 0, s, XA, ldxa, M, K, 0, sa);
-        hipLaunchKernelGGL(block_scale_kernel, dim3(unsigned(nsb)), dim3(256), 0, s, XB, ldxb, K, N, 1, sb);
+        // sb follows sa: one fill, then the maxima of A's row blocks (slices
+        // of 16 columns) and of B's column blocks (16 columns each)
+        const i64 ns = nsa + nsb;
+        hipLaunchKernelGGL(zero_f32_kernel, dim3(unsigned(cdiv(ns, 256))), dim3(256), 0, s, sa, ns);
+        const int cs = 16;
+        hipLaunchKernelGGL(block_max_kernel, dim3(unsigned(nsa), unsigned(cdiv(K, cs))), dim3(256), 0, s, XA, ldxa, M,
+                           K, 0, cs, (unsigned*)sa);
+        hipLaunchKernelGGL(block_max_kernel, dim3(unsigned(nsb), unsigned(256 / cs)), dim3(256), 0, s, XB, ldxb, K, N,
+                           1, cs, (unsigned*)sb);
+        hipLaunchKernelGGL(scale_from_max_kernel, dim3(unsigned(cdiv(ns, 256))), dim3(256), 0, s, sa, ns);
     } else {
         sa = nullptr; sb = nullptr;
     }
