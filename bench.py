@@ -54,6 +54,31 @@ def gemm_standalone(ox, torch, m, n, k, reps=5):
     return 2.0 * m * n * k / (ms * 1e-3) / 1e12, ms
 
 
+def lowp_update_times(ox, torch, n, reps=10):
+    """The mixed LU's low-precision trailing product at its update shape:
+    C (n x n, FP32) -= round(A) (n x 256) round(B), per low type, operand
+    rounding included.  Returns {type: (ms, TFLOP/s, fraction of the dense
+    peak, C bandwidth in GB/s)}; C is read and written once (8 n^2 bytes)."""
+    g = torch.Generator(device="cuda").manual_seed(0x10F)
+    A = ox.DMatrix(torch.randn(n * 256, dtype=torch.float32, device="cuda", generator=g), n, 256)
+    B = ox.DMatrix(torch.randn(256 * n, dtype=torch.float32, device="cuda", generator=g), 256, n)
+    C = ox.DMatrix(torch.randn(n * n, dtype=torch.float32, device="cuda", generator=g), n, n)
+    peak = {"bfloat16": 2.5e15, "float16": 2.5e15, "float8_e4m3fn": 5.0e15}
+    out = {}
+    for low in ("bfloat16", "float16", "float8_e4m3fn"):
+        ox.gemm_lowp_update_(C, A, B, lowtype=low)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ox.gemm_lowp_update_(C, A, B, lowtype=low)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        tf = 2.0 * n * n * 256 / t
+        out[low] = (t * 1e3, tf / 1e12, tf / peak[low], 8.0 * n * n / t / 1e9)
+    del A, B, C
+    return out
+
+
 def haar_pair(torch, n, seed):
     """U, V: the Q factors of Gaussian matrices with the sign of diag(R) fixed
     (Haar-distributed orthogonal matrices; test data only)."""
@@ -248,26 +273,9 @@ def run_part(ox, torch, part, n, reps=3):
         work = lambda: ox.gemm_(C, A, B, -1.0, 1.0)             # noqa: E731
         reps = 10
     elif part == "lowp_gemm":
-        # the mixed LU's low-precision trailing product at its update shape:
-        # C (16384 x 16384, FP32) -= round(A) (16384 x 256) round(B), per low
-        # type, operand rounding included; TF/s against the dense peaks
-        A = ox.DMatrix(torch.randn(n * 256, dtype=torch.float32, device=dev, generator=g), n, 256)
-        B = ox.DMatrix(torch.randn(256 * n, dtype=torch.float32, device=dev, generator=g), 256, n)
-        C = ox.DMatrix(torch.randn(n * n, dtype=torch.float32, device=dev, generator=g), n, n)
-        peak = {"bfloat16": 2.5e15, "float16": 2.5e15, "float8_e4m3fn": 5.0e15}
-        for low in ("bfloat16", "float16", "float8_e4m3fn"):
-            ox.gemm_lowp_update_(C, A, B, lowtype=low)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(10):
-                ox.gemm_lowp_update_(C, A, B, lowtype=low)
-            torch.cuda.synchronize()
-            t = (time.perf_counter() - t0) / 10
-            tf = 2.0 * n * n * 256 / t
-            # HBM: the FP32 C read and written once (the operands are 16 MB)
-            gbs = 2.0 * n * n * 4 / t / 1e9
-            log(f"lowp_gemm {low} {n}x{n}x256: {t * 1e3:.3f} ms {tf / 1e12:.1f} TFLOP/s "
-                f"({100 * tf / peak[low]:.1f} % of the dense peak), C at {gbs:.0f} GB/s")
+        for low, (ms, tf, frac, gbs) in lowp_update_times(ox, torch, n).items():
+            log(f"lowp_gemm {low} {n}x{n}x256: {ms:.3f} ms {tf:.1f} TFLOP/s "
+                f"({100 * frac:.1f} % of the dense peak), C at {gbs:.0f} GB/s")
         return
     elif part == "cholesky":
         S = ox.DMatrix(torch.randn(n * n, dtype=torch.float64, device=dev, generator=g), n, n)
@@ -400,6 +408,10 @@ def main():
             tf, gms = gemm_standalone(ox, torch, m_, n_, k_)
             extra[f"gemm_{m_}x{n_}x{k_}_tflops"] = round(tf, 2)
             log(f"gemm {m_}x{n_}x{k_}: {gms:.3f} ms {tf:.1f} TFLOP/s")
+        for low, (ms, tf, frac, gbs) in lowp_update_times(ox, torch, n).items():
+            extra[f"lowp_update_{low}_{n}x{n}x256"] = {"ms": round(ms, 3), "tflops": round(tf, 1),
+                                                       "frac_dense_peak": round(frac, 4), "c_gbs": round(gbs)}
+            log(f"lowp update {low} {n}x{n}x256: {ms:.3f} ms {tf:.1f} TFLOP/s, C at {gbs:.0f} GB/s")
         F, _ = step()
         B64 = ox.DMatrix(torch.randn(n * 64, dtype=torch.float64, device=dev, generator=g), n, 64)
         B64b = B64.copy()
