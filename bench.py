@@ -66,7 +66,8 @@ def lowp_update_times(ox, torch, n, reps=10):
     peak = {"bfloat16": 2.5e15, "float16": 2.5e15, "float8_e4m3fn": 5.0e15}
     out = {}
     for low in ("bfloat16", "float16", "float8_e4m3fn"):
-        ox.gemm_lowp_update_(C, A, B, lowtype=low)
+        for _ in range(5):          # warm: the first calls also bring up the clocks
+            ox.gemm_lowp_update_(C, A, B, lowtype=low)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
